@@ -1,0 +1,225 @@
+"""Benchmark of the image-formation hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--K 32]
+
+A step is one full-frequency backprojection of the configuration's synthetic
+scene with the fused normal-vote moment (recurrence kernel), inputs resident
+in HBM.  Prints ONE JSON line:
+
+* value      W / t  in G voxel*row*sample/s (W = V*A*F, t = mean step time)
+* roofline   FP32 VALU: algorithmic FLOPs V*A*(8F+14) per launch over the
+             launch's mean duration from HIP events on the launch stream,
+             against the MI355X FP32 vector peak (157.3 TFLOP/s packed,
+             78.6 non-packed); traffic from the committed rocprofv3 --pmc
+             summary when present
+* plain      the direct-sincos FP32 kernel on the same inputs (fewer steps)
+* backward   one backward (dL/dmeas, dL/dant) of L = sum w|I|^2
+* rsdf_bfs   GPU BFS over the 5%-filtered vote field (components by 6-connected
+             labelling on the host)
+* cpu_baseline  the CPU restatement of the reference's imaging loop
+             (imaging.cpp:60-139, OpenMP) on a bounded voxel sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_PACKED = 157.3e12
+PEAK_NONPACKED = 78.6e12
+
+
+def flops_fwd(V, A, F):
+    return V * A * (8.0 * F + 14.0)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(sc, target_s=12.0):
+    """Time the CPU imaging restatement on a seeded voxel sample (cpu_baseline leg)."""
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    cores = int(os.environ["OMP_NUM_THREADS"])
+    rng = np.random.default_rng(99)
+    V = len(sc.x_locs) * len(sc.y_locs) * len(sc.z_locs)
+    A, F = sc.meas.shape
+    rxo = np.zeros((1, 3))
+    n = cores * 2
+    while True:
+        vidx = np.sort(rng.choice(V, size=n, replace=False))
+        t0 = time.perf_counter()
+        O.bp_cpu_imaging(sc.x_locs, sc.y_locs, sc.z_locs, sc.ant, sc.meas, rxo, sc.slope,
+                         sc.wavelength, vidx=vidx)
+        dt = time.perf_counter() - t0
+        if dt >= target_s or n >= V:
+            break
+        n = int(min(V, max(n * 2, n * target_s / max(dt, 1e-3) * 1.05)))
+    units = n * A * F
+    return {"value": units / dt / 1e9, "unit": "G voxel*row*sample/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{n} random voxels of the {V}-voxel grid x {A} rows x {F} samples "
+                      f"({dt:.1f} s, OMP_NUM_THREADS={cores}, {cpu_model()})"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--K", type=int, default=32)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="forward only")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build measures one GPU; the path shards by voxel "
+                          "slabs across GPUs but no multi-GPU run is part of it",
+                          "n_gpus": a.gpus}))
+        return 2
+
+    import torch
+    from mmnorm_amd import ops, synth
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    cfg = synth.CONFIGS[a.config]
+    t0 = time.perf_counter()
+    sc = synth.make_scene(a.config, device=dev)
+    t_scene = time.perf_counter() - t0
+    T = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x, dtype=dt)).to(dev)
+    meas, ant = T(sc.meas, np.complex64), T(sc.ant, np.float32)
+    xl, yl, zl = T(sc.x_locs, np.float32), T(sc.y_locs, np.float32), T(sc.z_locs, np.float32)
+    V, A, F = cfg.nvox, cfg.rows, cfg.n_freq
+    W = float(V) * A * F
+
+    def fwd(variant):
+        return ops.backproject(meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, variant=variant,
+                               K=a.K, moment=True)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(st)
+            out = fn()
+            e1.record(st)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = [e0.elapsed_time(e1) for e0, e1 in evs]
+        return wall, kms, out
+
+    wall, kms, (img, mom) = timed(lambda: fwd("rec"), a.steps, a.warmup)
+    ms_step = wall / a.steps * 1e3
+    k_avg = float(np.mean(kms)) * 1e-3
+    achieved = flops_fwd(V, A, F) / k_avg
+    rec = {"metric": "G voxel*antenna*freq/s (recurrence fwd+vote, FP32 VALU)",
+           "value": W / (wall / a.steps) / 1e9, "unit": "G voxel*row*sample/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 (FP64 re-anchor)", "data": "synthetic (seeded scene, see synth.py)",
+           "config": {"workload": f"config {cfg.cid}: {cfg.n_ap}x{cfg.n_ap} aperture, "
+                                  f"{F} samples 77-81 GHz, {cfg.grid[0]}x{cfg.grid[1]}x"
+                                  f"{cfg.grid[2]} voxels at {cfg.pitch * 1e3:g} mm, "
+                                  f"{cfg.n_scat} scatterers",
+                      "V": V, "A": A, "F": F, "K": a.K, "kernel": "k_bp_rec",
+                      "parallelism": "single GPU"},
+           "kernel_ms": k_avg * 1e3,
+           "roofline": {"bound": "valu", "achieved": achieved / 1e12,
+                        "peak": PEAK_PACKED / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_PACKED,
+                        "frac_nonpacked": achieved / PEAK_NONPACKED,
+                        "traffic": None}}
+    prof = os.path.join(ROOT, "profiles", f"traffic_bp_rec_config{cfg.cid}.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            tr = json.load(f)
+        rec["roofline"]["traffic"] = tr.get("bytes_per_launch")
+
+    if not a.no_extras:
+        if a.plain_steps > 0:
+            pw, pk, _ = timed(lambda: fwd("plain"), a.plain_steps, 1)
+            p_s = float(np.mean(pk)) * 1e-3
+            rec["plain"] = {"value": W / p_s / 1e9, "kernel_ms": p_s * 1e3,
+                            "tflops": flops_fwd(V, A, F) / p_s / 1e12,
+                            "rec_over_plain": p_s / k_avg}
+        # backward of L = sum w |I|^2 (I = img / A)
+        w = torch.rand(V, device=dev, generator=torch.Generator(dev).manual_seed(1))
+        g = (2.0 * w * img.reshape(-1) / A / A).to(torch.complex64).contiguous()
+        bw, bk, _ = timed(lambda: ops.backproject_backward(g, meas, ant, xl, yl, zl, f0=sc.f0,
+                                                           df=sc.df, scale=1.0 / A, K=32),
+                          2, 1)
+        b_s = float(np.mean(bk)) * 1e-3
+        rec["backward"] = {"ms": b_s * 1e3, "over_forward": b_s / k_avg,
+                           "tflops": V * A * (16.0 * F + 30.0) / b_s / 1e12}
+        rec["rsdf_bfs"] = bfs_stage(img, mom, sc, dev)
+    rec["scene_s"] = t_scene
+    if not a.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+def bfs_stage(img, mom, sc, dev):
+    """5% filter -> votes -> 6-connected components (host) -> GPU FIFO BFS, timed."""
+    import torch
+    from scipy import ndimage
+    from mmnorm_amd import ops, sdf
+    mag = img.abs().reshape(-1)
+    vidx = torch.nonzero(mag > 0.05 * mag.max()).reshape(-1).to(torch.int32)
+    m = mom.reshape(-1, 3)[vidx.long()].contiguous()
+    nrm = ops.sn_from_moment(img.reshape(-1), m, vidx).double()
+    nrm = nrm / torch.linalg.norm(nrm, dim=1, keepdim=True)
+    shape = img.shape
+    normals = torch.full((img.numel(), 3), float("nan"), dtype=torch.float64, device=dev)
+    normals[vidx.long()] = nrm
+    normals = normals.reshape(*shape, 3)
+    mask = (~torch.isnan(normals[..., 0])).cpu().numpy()
+    labeled, nlab = ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
+    labeled = labeled + 1
+    sizes = ndimage.sum(mask, labeled, np.arange(2, nlab + 2))
+    valid = np.arange(2, nlab + 2)[sizes >= 27]
+    nrm_np = normals.cpu().numpy()
+    m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
+    coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
+    seeds = sdf.component_seeds(nrm_np, coords, labeled, valid)
+    lab_t = torch.from_numpy(labeled.astype(np.int32)).to(dev)
+    seeds_t = torch.from_numpy(seeds.astype(np.int32)).to(dev)
+    dx = sc.x_locs[1] - sc.x_locs[0]
+    ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rsdf, hops = ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"ms": dt * 1e3, "voxels_reached": int((hops >= 0).sum().item()),
+            "components": int(len(seeds)), "levels": int(hops.max().item()) + 1,
+            "valid_voxels": int(vidx.numel())}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

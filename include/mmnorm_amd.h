@@ -1,0 +1,167 @@
+/*
+ * mmnorm_amd.h -- C ABI of libmmnorm_hip.so, the MI355X (gfx950) image-formation
+ * hot path of the mmNorm pipeline: SAR backprojection, the surface-normal vote,
+ * and the RSDF breadth-first search.
+ *
+ * Conventions shared by every entry point:
+ *   - All array arguments are DEVICE pointers owned by the caller.  Complex
+ *     arrays are interleaved float pairs (re, im), i.e. the memory layout of
+ *     a complex64 tensor.  The library allocates nothing; buffers it needs
+ *     beyond inputs and outputs come in as a caller-provided workspace whose
+ *     size the matching *_workspace_size() function returns.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Work is enqueued
+ *     on it and the call returns without synchronising, except mmn_rsdf_bfs,
+ *     which reads the frontier size back once per batch of levels.
+ *   - Return value: 0 = success, < 0 = argument error (nothing launched),
+ *     > 0 = a hipError_t from the runtime.  mmn_last_error() gives a
+ *     thread-local message for the last non-zero status.
+ *   - Voxel grids are separable: voxel (x, y, z) sits at
+ *     (x_locs[x], y_locs[y], z_locs[z]) and has linear index
+ *     (x * NY + y) * NZ + z (z fastest), as in imaging_gpu.cu:48-54,121.
+ *   - Antenna rows: row i has TX position ant[i][0..2] and RX offset
+ *     rx_off[i % R][0..2]; its measurement row is meas[i][0..F-1]
+ *     (imaging_gpu.cu:65-76, the layout py_image_gpu.py:228-234 builds).
+ *     The round-trip distance of row i to voxel v is
+ *     |v - p_i| + |v - p_i - o_i| (+ 0.15 m when is_ti_radar != 0).
+ */
+#ifndef MMNORM_AMD_H
+#define MMNORM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMN_ABI_VERSION 1
+
+/* Backprojection variants accepted by mmn_bp_forward. */
+#define MMN_BP_PLAIN 1 /* direct FP32 sincosf per (voxel, row, sample)          */
+#define MMN_BP_REC   2 /* FP32 rotation (Horner) over samples, FP64 re-anchor
+                          every K samples, compensated FP32 accumulation       */
+
+/* Status codes below zero. */
+#define MMN_ERR_ARG      (-1) /* bad size, count or enum value                */
+#define MMN_ERR_NULL     (-2) /* a required pointer is NULL                    */
+#define MMN_ERR_WS       (-3) /* workspace missing or too small                */
+#define MMN_ERR_INPUT    (-4) /* input data violates a documented precondition */
+
+int mmn_abi_version(void);
+const char *mmn_last_error(void);
+
+/*
+ * mmn_cuda_image -- replaces the `cuda_image` kernel launch
+ * (imaging_gpu.cu:11-123, launched at py_image_gpu.py:189-199) over the whole
+ * grid.  Same arithmetic as that kernel: FP32 distance (sqrtf), FP64 phase
+ * and FP64 cos/sin, FP32 accumulation in row order.  use_interpolated != 0
+ * selects the range-bin branch (imaging_gpu.cu:90-105; meas rows hold
+ * `samples` FFT bins), otherwise the full matched filter over `samples`
+ * samples (imaging_gpu.cu:106-117).  Writes the raw sum, un-normalised,
+ * to img[NX*NY*NZ] (complex).
+ */
+int mmn_cuda_image(float *img, const float *x_locs, const float *y_locs,
+                   const float *z_locs, const float *ant, const float *meas,
+                   const float *rx_off, float slope, float wavelength,
+                   float fft_spacing, int nx, int ny, int nz, int num_rows,
+                   int num_rx, int samples, int is_ti_radar,
+                   int use_interpolated, void *stream);
+
+/*
+ * mmn_cuda_sn_est -- replaces the `cuda_sn_est` kernel launch
+ * (surface_normal_est.cu:13-150, launched at py_sn_est_gpu.py:229-247).
+ * For each of the n_valid voxels in vox_idx (linear indices; the reference
+ * passes the same list as three index planes, py_sn_est_gpu.py:327-333)
+ * writes the un-normalised vote  sum_i (p_i + o_i/2 - v) * Re(c_i conj(I_v))/|I_v|
+ * to normals[n][3] (float), with c_i the interpolated range-bin contribution
+ * of row i and I_v = img[vox] (complex).  Same arithmetic as the reference.
+ */
+int mmn_cuda_sn_est(float *normals, const int32_t *vox_idx, int64_t n_valid,
+                    const float *img, const float *x_locs,
+                    const float *y_locs, const float *z_locs, const float *ant,
+                    const float *meas, const float *rx_off, float slope,
+                    float wavelength, float fft_spacing, int nx, int ny,
+                    int nz, int num_rows, int num_rx, int samples,
+                    int is_ti_radar, void *stream);
+
+/*
+ * mmn_bp_forward -- full-frequency backprojection, the sum of
+ * imaging_gpu.cu:106-117 written for MI355X:
+ *     img[v] = sum_i sum_j meas[i][j] * exp(-j 2 pi (f0 + j*df) d_iv / c)
+ * with c = 299792458 m/s.  The reference's phase -2 pi d/lambda - 2 pi d j
+ * slope/c is this with f0 = c/lambda and df = slope.
+ * If vox_idx != NULL the n_vox voxels it lists are computed and img/moment
+ * are indexed by list position; otherwise all NX*NY*NZ voxels (n_vox
+ * ignored).  If moment != NULL it also receives the normal-vote moment
+ *     moment[v][k] = sum_i (p_i + o_i/2 - v)_k * c_iv      (complex, [n][3])
+ * where c_iv is row i's contribution to img[v]; the un-normalised vote of
+ * surface_normal_est.cu:128-144 is then Re(conj(I_v) moment[v]) / |I_v|
+ * (mmn_sn_from_moment).  variant: MMN_BP_PLAIN or MMN_BP_REC; K (REC only)
+ * is the re-anchor interval in samples, one of 8, 16, 32, 64 (or 0 = 32).
+ * No reference counterpart for the fused moment or the variants; the
+ * reference computes the vote only for the interpolated branch
+ * (py_sn_est_gpu.py:404-405).
+ */
+int mmn_bp_forward(float *img, float *moment, const float *x_locs,
+                   const float *y_locs, const float *z_locs,
+                   const int32_t *vox_idx, int64_t n_vox, const float *ant,
+                   const float *meas, const float *rx_off, double f0,
+                   double df, int nx, int ny, int nz, int num_rows,
+                   int num_rx, int samples, int is_ti_radar, int variant,
+                   int K, void *stream);
+
+/*
+ * mmn_sn_from_moment -- normals[n][3] = Re(conj(I) * moment[n][3]) / |I| with
+ * I = img[vox_idx[n]] (img indexed by linear voxel index).  Completes the
+ * full-frequency vote (the reference's weight, surface_normal_est.cu:138-139,
+ * folded over rows by linearity).  A voxel with |I| = 0 gets NaN, as the
+ * reference's division does.
+ */
+int mmn_sn_from_moment(float *normals, const float *img, const float *moment,
+                       const int32_t *vox_idx, int64_t n_valid, void *stream);
+
+/*
+ * mmn_bp_backward -- adjoint of mmn_bp_forward (REC arithmetic) for a real
+ * loss L of the image I = scale * img.  g = dL/dI in PyTorch's convention
+ * (dL/dRe + i dL/dIm), complex [n] over the same voxels as the forward.
+ * Writes
+ *   grad_meas[i][j] = scale * sum_v g_v * exp(+j 2 pi f_j d_iv / c)      (complex [A][F])
+ *   grad_ant[i][k]  = scale * sum_v Im(conj(g_v) Q_iv) * d(d_iv)/dp_ik    (float [A][3])
+ * with Q_iv = sum_j (2 pi f_j / c) meas[i][j] exp(-j 2 pi f_j d_iv / c).
+ * Either output may be NULL (not computed).  No reference counterpart:
+ * the reference computes no gradients.  Deterministic: partial sums go to
+ * the workspace and are reduced in a fixed order.
+ */
+size_t mmn_bp_backward_workspace_size(int num_rows, int samples, int64_t n_vox);
+int mmn_bp_backward(float *grad_meas, float *grad_ant, const float *g,
+                    double scale, const float *x_locs, const float *y_locs,
+                    const float *z_locs, const int32_t *vox_idx,
+                    int64_t n_vox, const float *ant, const float *meas,
+                    const float *rx_off, double f0, double df, int nx,
+                    int ny, int nz, int num_rows, int num_rx, int samples,
+                    int is_ti_radar, int K, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
+/*
+ * mmn_rsdf_bfs -- replaces the per-component FIFO breadth-first search of
+ * _construct_rsdf_components (sdf.py:180-222).  normals: float64 [V][3];
+ * labels: int32 [V]; seeds: the n_seeds start voxels (linear indices), one
+ * per component, in the order the reference visits components.  A voxel is
+ * enterable from u when it is in bounds, not yet visited, has a non-NaN
+ * normals[.][0] and labels equal to labels[u] (sdf.py:169-177).  Outputs
+ * (all V entries written): rsdf float64 [V] (NaN where not reached, 0 at
+ * seeds, otherwise rsdf[parent] + dot(dir * dx, normals[child]) with the
+ * parent FIFO order gives, sdf.py:216-222), hops int32 [V] (BFS level, -1
+ * where not reached).  Bit-exact with the reference's FIFO order.  Calls
+ * hipStreamSynchronize on `stream` once per batch of levels.
+ */
+size_t mmn_rsdf_bfs_workspace_size(int nx, int ny, int nz);
+int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
+                 const int32_t *labels, const int32_t *seeds, int n_seeds,
+                 double dx, int nx, int ny, int nz, void *workspace,
+                 size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MMNORM_AMD_H */

@@ -1,0 +1,450 @@
+// Backprojection kernels for gfx950 (MI355X, wave64).
+//
+//   k_image_mirror  the reference kernel's arithmetic (imaging_gpu.cu:48-122):
+//                   FP32 distance, FP64 phase and cos/sin, FP32 sums in row
+//                   order; both the interpolated and the full branch.
+//   k_bp_plain      full matched filter with one FP32 sincosf per
+//                   (voxel, row, sample): the direct baseline.
+//   k_bp_rec        full matched filter evaluated per (voxel, row) as a
+//                   polynomial in the per-sample phase step r = exp(-j 2pi df d/c)
+//                   (Horner: one complex rotate-and-add, 4 FMAs, per sample),
+//                   re-anchored every K samples by an FP64-reduced phasor,
+//                   rows summed with Kahan compensation, and the normal-vote
+//                   moment accumulated in the same pass.
+//
+// Thread mapping (all three): one lane per voxel, 256-lane workgroups; every
+// lane of a wave walks the same row at the same time, so the row's antenna
+// position, RX offset and measurement samples are wave-uniform and are read
+// through scalar loads straight into SGPR operands of the FMAs -- no LDS
+// staging and no per-lane measurement traffic.
+#include <stdlib.h>
+
+#include "mmn_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr int kBlock = 256;
+
+// --------------------------------------------------------------------------
+// Reference-arithmetic mirror.  Contraction is off so every product and sum
+// rounds separately, as the restatement in oracle/ does; the operation order
+// of each expression follows imaging_gpu.cu:77-117.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_image_mirror(
+    float2 *__restrict__ img, const float *__restrict__ xl,
+    const float *__restrict__ yl, const float *__restrict__ zl,
+    const float *__restrict__ ant, const float2 *__restrict__ meas,
+    const float *__restrict__ rxo, float slope, float wavelength,
+    float fft_spacing, int ny, int nz, int64_t nvox, int rows, int R, int S,
+    int is_ti, int interp) {
+#pragma clang fp contract(off)
+  const int64_t lin = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (lin >= nvox) return;
+  float vx, vy, vz;
+  voxel_coords(xl, yl, zl, ny, nz, lin, vx, vy, vz);
+
+  float sum_r = 0.0f, sum_i = 0.0f;
+  for (int i = 0; i < rows; ++i) {
+    const float ax = vx - ant[3 * i + 0];
+    const float ay = vy - ant[3 * i + 1];
+    const float az = vz - ant[3 * i + 2];
+    const int rx = i % R;
+    const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+    const float fwd = sqrtf(ax * ax + ay * ay + az * az);
+    const float bx = ax - ox, by = ay - oy, bz = az - oz;
+    const float back = sqrtf(bx * bx + by * by + bz * bz);
+    float d = fwd + back;
+    if (is_ti) d = (float)((double)d + 0.15);
+    const float2 *__restrict__ row = meas + (size_t)i * S;
+    if (interp) {
+      if (d < 0.0f || d > fft_spacing * (float)S) continue;
+      const int bin = (int)floorf(d / fft_spacing / 2.0f);
+      const float2 m = row[bin];
+      const double th = -2.0 * kPi * (double)d / (double)wavelength;
+      const float pr = (float)cos(th);
+      const float pi = (float)sin(th);
+      sum_r += m.x * pr - m.y * pi;
+      sum_i += m.x * pi + m.y * pr;
+    } else {
+      const double a0 = -2.0 * kPi * (double)d / (double)wavelength;
+      for (int j = 0; j < S; ++j) {
+        const float2 m = row[j];
+        const double th =
+            a0 + -2.0 * kPi * (double)d * (double)j * (double)slope / MMN_C0;
+        const float pr = (float)cos(th);
+        const float pi = (float)sin(th);
+        sum_r += m.x * pr - m.y * pi;
+        sum_i += m.x * pi + m.y * pr;
+      }
+    }
+  }
+  img[lin] = make_float2(sum_r, sum_i);
+}
+
+// --------------------------------------------------------------------------
+// Shared per-lane voxel set-up for the fast kernels.
+// --------------------------------------------------------------------------
+template <int VPT>
+__device__ __forceinline__ void load_voxels(
+    const float *__restrict__ xl, const float *__restrict__ yl,
+    const float *__restrict__ zl, const int32_t *__restrict__ vidx,
+    int64_t nvox, int ny, int nz, int64_t base, float (&vx)[VPT],
+    float (&vy)[VPT], float (&vz)[VPT]) {
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    int64_t t = base + (int64_t)k * kBlock;
+    if (t >= nvox) t = nvox - 1;  // idle lanes recompute the last voxel
+    const int64_t lin = vidx ? (int64_t)vidx[t] : t;
+    voxel_coords(xl, yl, zl, ny, nz, lin, vx[k], vy[k], vz[k]);
+  }
+}
+
+// Adds one row's contribution c to the image (Kahan) and the vote moment.
+__device__ __forceinline__ void add_row(float cr, float ci, float vecx,
+                                        float vecy, float vecz, float &Ir,
+                                        float &Ii, float &Cr, float &Ci,
+                                        float (&M)[6]) {
+  // Kahan: C carries the low-order part lost by the running sum.
+  const float yr = cr - Cr, yi = ci - Ci;
+  const float tr = Ir + yr, ti = Ii + yi;
+  Cr = (tr - Ir) - yr;
+  Ci = (ti - Ii) - yi;
+  Ir = tr;
+  Ii = ti;
+  M[0] = fmaf(vecx, cr, M[0]);
+  M[1] = fmaf(vecx, ci, M[1]);
+  M[2] = fmaf(vecy, cr, M[2]);
+  M[3] = fmaf(vecy, ci, M[3]);
+  M[4] = fmaf(vecz, cr, M[4]);
+  M[5] = fmaf(vecz, ci, M[5]);
+}
+
+template <int VPT>
+__device__ __forceinline__ void store_voxels(
+    float2 *__restrict__ img, float2 *__restrict__ mom, int64_t nvox,
+    int64_t base, const float (&Ir)[VPT], const float (&Ii)[VPT],
+    const float (&M)[VPT][6]) {
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    const int64_t t = base + (int64_t)k * kBlock;
+    if (t < nvox) {
+      img[t] = make_float2(Ir[k], Ii[k]);
+      if (mom) {
+        mom[3 * t + 0] = make_float2(M[k][0], M[k][1]);
+        mom[3 * t + 1] = make_float2(M[k][2], M[k][3]);
+        mom[3 * t + 2] = make_float2(M[k][4], M[k][5]);
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Plain: FP32 distance, FP32 phase theta = d * 2 pi f_j / c, sincosf.
+// --------------------------------------------------------------------------
+template <int VPT>
+__global__ __launch_bounds__(kBlock) void k_bp_plain(
+    float2 *__restrict__ img, float2 *__restrict__ mom,
+    const float *__restrict__ xl, const float *__restrict__ yl,
+    const float *__restrict__ zl, const int32_t *__restrict__ vidx,
+    int64_t nvox, const float *__restrict__ ant,
+    const float2 *__restrict__ meas, const float *__restrict__ rxo, float k0,
+    float dk, int ny, int nz, int rows, int R, int F, int is_ti) {
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
+  float vx[VPT], vy[VPT], vz[VPT];
+  load_voxels<VPT>(xl, yl, zl, vidx, nvox, ny, nz, base, vx, vy, vz);
+  float Ir[VPT], Ii[VPT], M[VPT][6];
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    Ir[k] = Ii[k] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) M[k][q] = 0.0f;
+  }
+  for (int i = 0; i < rows; ++i) {
+    const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+    const int rx = i % R;
+    const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+    const float2 *__restrict__ row = meas + (size_t)i * F;
+    float d[VPT], cr[VPT], ci[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const float ax = vx[k] - px, ay = vy[k] - py, az = vz[k] - pz;
+      const float bx = ax - ox, by = ay - oy, bz = az - oz;
+      d[k] = sqrtf(ax * ax + ay * ay + az * az) + sqrtf(bx * bx + by * by + bz * bz);
+      if (is_ti) d[k] += 0.15f;
+      cr[k] = ci[k] = 0.0f;
+    }
+    for (int j = 0; j < F; ++j) {
+      const float2 m = row[j];
+      const float kj = fmaf((float)j, dk, k0);
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        float s, c;
+        sincosf(d[k] * kj, &s, &c);
+        // m * exp(-j theta) = m * (c - j s)
+        cr[k] = fmaf(m.x, c, fmaf(m.y, s, cr[k]));
+        ci[k] = fmaf(m.y, c, fmaf(-m.x, s, ci[k]));
+      }
+    }
+    const float vax = px + ox / 2.0f, vay = py + oy / 2.0f, vaz = pz + oz / 2.0f;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      Ir[k] += cr[k];
+      Ii[k] += ci[k];
+      const float vecx = vax - vx[k], vecy = vay - vy[k], vecz = vaz - vz[k];
+      M[k][0] = fmaf(vecx, cr[k], M[k][0]);
+      M[k][1] = fmaf(vecx, ci[k], M[k][1]);
+      M[k][2] = fmaf(vecy, cr[k], M[k][2]);
+      M[k][3] = fmaf(vecy, ci[k], M[k][3]);
+      M[k][4] = fmaf(vecz, cr[k], M[k][4]);
+      M[k][5] = fmaf(vecz, ci[k], M[k][5]);
+    }
+  }
+  store_voxels<VPT>(img, mom, nvox, base, Ir, Ii, M);
+}
+
+// --------------------------------------------------------------------------
+// Recurrence.  For row i and voxel v the samples' phasors are
+//   exp(-j 2pi f_j d/c) = A_b * r^(j - bK),  A_b = exp(-j 2pi f_bK d/c),
+//   r = exp(-j 2pi df d/c),
+// so the row's block sum  sum_j s_j phi_j = A_b * h_b  with the polynomial
+// h_b = sum_t s_{bK+t} r^t evaluated by Horner from the top coefficient down.
+// A_b is recomputed from d in FP64 at every block (the re-anchor).
+// --------------------------------------------------------------------------
+template <int VPT>
+__device__ __forceinline__ void horner_step(float sr, float si,
+                                            const float (&rr)[VPT],
+                                            const float (&ri)[VPT],
+                                            float (&hr)[VPT],
+                                            float (&hi)[VPT]) {
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    const float tr = fmaf(-hi[k], ri[k], fmaf(hr[k], rr[k], sr));
+    const float ti = fmaf(hi[k], rr[k], fmaf(hr[k], ri[k], si));
+    hr[k] = tr;
+    hi[k] = ti;
+  }
+}
+
+template <int K, int VPT>
+__global__ __launch_bounds__(kBlock) void k_bp_rec(
+    float2 *__restrict__ img, float2 *__restrict__ mom,
+    const float *__restrict__ xl, const float *__restrict__ yl,
+    const float *__restrict__ zl, const int32_t *__restrict__ vidx,
+    int64_t nvox, const float *__restrict__ ant,
+    const float2 *__restrict__ meas, const float *__restrict__ rxo,
+    double f0c, double dfc, int ny, int nz, int rows, int R, int F,
+    int is_ti) {
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
+  float vx[VPT], vy[VPT], vz[VPT];
+  load_voxels<VPT>(xl, yl, zl, vidx, nvox, ny, nz, base, vx, vy, vz);
+  float Ir[VPT], Ii[VPT], Cr[VPT], Ci[VPT], M[VPT][6];
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    Ir[k] = Ii[k] = Cr[k] = Ci[k] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) M[k][q] = 0.0f;
+  }
+  const int nb = F / K;
+  const int tail = F - nb * K;
+
+  for (int i = 0; i < rows; ++i) {
+    const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+    const int rx = i % R;
+    const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+    const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
+    const float2 *__restrict__ row = meas + (size_t)i * F;
+
+    double d[VPT];
+    float rr[VPT], ri[VPT], cr[VPT], ci[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      d[k] = roundtrip_dist64(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono);
+      if (is_ti) d[k] += 0.15;
+      phasor_rev32((float)(d[k] * dfc), rr[k], ri[k]);
+      cr[k] = ci[k] = 0.0f;
+    }
+
+    for (int b = 0; b < nb; ++b) {
+      const float2 *__restrict__ blk = row + b * K;
+      float hr[VPT], hi[VPT];
+      {
+        const float2 top = blk[K - 1];
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+          hr[k] = top.x;
+          hi[k] = top.y;
+        }
+      }
+#pragma unroll
+      for (int t = K - 2; t >= 0; --t) {
+        const float2 s = blk[t];
+        horner_step<VPT>(s.x, s.y, rr, ri, hr, hi);
+      }
+      const double fb = fma((double)(b * K), dfc, f0c);
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        float ar, ai;
+        phasor_rev64(d[k] * fb, ar, ai);
+        cr[k] = fmaf(ar, hr[k], fmaf(-ai, hi[k], cr[k]));
+        ci[k] = fmaf(ar, hi[k], fmaf(ai, hr[k], ci[k]));
+      }
+    }
+    if (tail) {
+      const float2 *__restrict__ blk = row + nb * K;
+      float hr[VPT], hi[VPT];
+      {
+        const float2 top = blk[tail - 1];
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+          hr[k] = top.x;
+          hi[k] = top.y;
+        }
+      }
+      for (int t = tail - 2; t >= 0; --t) {
+        const float2 s = blk[t];
+        horner_step<VPT>(s.x, s.y, rr, ri, hr, hi);
+      }
+      const double fb = fma((double)(nb * K), dfc, f0c);
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        float ar, ai;
+        phasor_rev64(d[k] * fb, ar, ai);
+        cr[k] = fmaf(ar, hr[k], fmaf(-ai, hi[k], cr[k]));
+        ci[k] = fmaf(ar, hi[k], fmaf(ai, hr[k], ci[k]));
+      }
+    }
+
+    const float vax = px + ox / 2.0f, vay = py + oy / 2.0f, vaz = pz + oz / 2.0f;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k)
+      add_row(cr[k], ci[k], vax - vx[k], vay - vy[k], vaz - vz[k], Ir[k],
+              Ii[k], Cr[k], Ci[k], M[k]);
+  }
+  store_voxels<VPT>(img, mom, nvox, base, Ir, Ii, M);
+}
+
+int env_int(const char *name, int dflt) {
+  const char *s = getenv(name);
+  return (s && *s) ? atoi(s) : dflt;
+}
+
+template <int K, int VPT>
+void launch_rec(hipStream_t st, float2 *img, float2 *mom, const float *xl,
+                const float *yl, const float *zl, const int32_t *vidx,
+                int64_t nvox, const float *ant, const float2 *meas,
+                const float *rxo, double f0c, double dfc, int ny, int nz,
+                int rows, int R, int F, int is_ti) {
+  hipLaunchKernelGGL((k_bp_rec<K, VPT>), dim3(mmn_blocks(nvox, kBlock * VPT)),
+                     dim3(kBlock), 0, st, img, mom, xl, yl, zl, vidx, nvox,
+                     ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti);
+}
+
+template <int VPT>
+int dispatch_rec(int K, hipStream_t st, float2 *img, float2 *mom,
+                 const float *xl, const float *yl, const float *zl,
+                 const int32_t *vidx, int64_t nvox, const float *ant,
+                 const float2 *meas, const float *rxo, double f0c, double dfc,
+                 int ny, int nz, int rows, int R, int F, int is_ti) {
+  switch (K) {
+    case 8:  launch_rec<8, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
+    case 16: launch_rec<16, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
+    case 32: launch_rec<32, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
+    case 64: launch_rec<64, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
+    default: return -1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int mmn_cuda_image(float *img, const float *x_locs,
+                              const float *y_locs, const float *z_locs,
+                              const float *ant, const float *meas,
+                              const float *rx_off, float slope,
+                              float wavelength, float fft_spacing, int nx,
+                              int ny, int nz, int num_rows, int num_rx,
+                              int samples, int is_ti_radar,
+                              int use_interpolated, void *stream) {
+  MMN_REQUIRE(img && x_locs && y_locs && z_locs && ant && meas && rx_off,
+              MMN_ERR_NULL, "mmn_cuda_image: NULL pointer argument");
+  MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 &&
+                  samples > 0,
+              MMN_ERR_ARG,
+              "mmn_cuda_image: sizes must be positive (nx=%d ny=%d nz=%d "
+              "rows=%d rx=%d samples=%d)",
+              nx, ny, nz, num_rows, num_rx, samples);
+  const int64_t nvox = (int64_t)nx * ny * nz;
+  hipLaunchKernelGGL(k_image_mirror, dim3(mmn_blocks(nvox, kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, (float2 *)img,
+                     x_locs, y_locs, z_locs, ant, (const float2 *)meas, rx_off,
+                     slope, wavelength, fft_spacing, ny, nz, nvox, num_rows,
+                     num_rx, samples, is_ti_radar, use_interpolated);
+  return mmn_check_launch("mmn_cuda_image");
+}
+
+extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
+                              const float *y_locs, const float *z_locs,
+                              const int32_t *vox_idx, int64_t n_vox,
+                              const float *ant, const float *meas,
+                              const float *rx_off, double f0, double df,
+                              int nx, int ny, int nz, int num_rows,
+                              int num_rx, int samples, int is_ti_radar,
+                              int variant, int K, void *stream) {
+  MMN_REQUIRE(img && x_locs && y_locs && z_locs && ant && meas && rx_off,
+              MMN_ERR_NULL, "mmn_bp_forward: NULL pointer argument");
+  MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 &&
+                  samples > 0,
+              MMN_ERR_ARG, "mmn_bp_forward: sizes must be positive");
+  MMN_REQUIRE(variant == MMN_BP_PLAIN || variant == MMN_BP_REC, MMN_ERR_ARG,
+              "mmn_bp_forward: unknown variant %d", variant);
+  if (K == 0) K = 32;
+  MMN_REQUIRE(variant != MMN_BP_REC || K == 8 || K == 16 || K == 32 || K == 64,
+              MMN_ERR_ARG, "mmn_bp_forward: K must be 8, 16, 32 or 64 (got %d)",
+              K);
+  const int64_t ngrid = (int64_t)nx * ny * nz;
+  int64_t nvox = ngrid;
+  if (vox_idx) {
+    MMN_REQUIRE(n_vox >= 0, MMN_ERR_ARG, "mmn_bp_forward: n_vox < 0");
+    nvox = n_vox;
+  }
+  if (nvox == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  float2 *im = (float2 *)img;
+  float2 *mo = (float2 *)moment;
+  const float2 *ms = (const float2 *)meas;
+  const double inv_c = 1.0 / MMN_C0;
+  if (variant == MMN_BP_PLAIN) {
+    const float k0 = (float)(2.0 * kPi * f0 * inv_c);
+    const float dk = (float)(2.0 * kPi * df * inv_c);
+    const int vpt = env_int("MMN_PLAIN_VPT", 1);
+    if (vpt == 2)
+      hipLaunchKernelGGL(k_bp_plain<2>, dim3(mmn_blocks(nvox, kBlock * 2)),
+                         dim3(kBlock), 0, st, im, mo, x_locs, y_locs, z_locs,
+                         vox_idx, nvox, ant, ms, rx_off, k0, dk, ny, nz,
+                         num_rows, num_rx, samples, is_ti_radar);
+    else
+      hipLaunchKernelGGL(k_bp_plain<1>, dim3(mmn_blocks(nvox, kBlock)),
+                         dim3(kBlock), 0, st, im, mo, x_locs, y_locs, z_locs,
+                         vox_idx, nvox, ant, ms, rx_off, k0, dk, ny, nz,
+                         num_rows, num_rx, samples, is_ti_radar);
+    return mmn_check_launch("mmn_bp_forward(plain)");
+  }
+  const double f0c = f0 * inv_c, dfc = df * inv_c;
+  const int vpt = env_int("MMN_REC_VPT", 2);
+  int rc;
+  if (vpt == 1)
+    rc = dispatch_rec<1>(K, st, im, mo, x_locs, y_locs, z_locs, vox_idx, nvox,
+                         ant, ms, rx_off, f0c, dfc, ny, nz, num_rows, num_rx,
+                         samples, is_ti_radar);
+  else if (vpt == 4)
+    rc = dispatch_rec<4>(K, st, im, mo, x_locs, y_locs, z_locs, vox_idx, nvox,
+                         ant, ms, rx_off, f0c, dfc, ny, nz, num_rows, num_rx,
+                         samples, is_ti_radar);
+  else
+    rc = dispatch_rec<2>(K, st, im, mo, x_locs, y_locs, z_locs, vox_idx, nvox,
+                         ant, ms, rx_off, f0c, dfc, ny, nz, num_rows, num_rx,
+                         samples, is_ti_radar);
+  MMN_REQUIRE(rc == 0, MMN_ERR_ARG, "mmn_bp_forward: bad K %d", K);
+  return mmn_check_launch("mmn_bp_forward(rec)");
+}

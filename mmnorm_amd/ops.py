@@ -1,0 +1,233 @@
+"""Torch-level API of the hot path (device tensors in, device tensors out).
+
+Every function here calls libmmnorm_hip.so through the C ABI; there is no CPU
+or PyTorch fallback.  Shapes:
+
+* ``meas``      complex64 (rows, F)       measurement rows (row = tx * R + rx)
+* ``ant``       float32  (rows, 3)        TX position of each row
+* ``rx_off``    float32  (R, 3)           RX offset, row i uses rx_off[i % R]
+* ``x/y/z_locs`` float32 (NX,), (NY,), (NZ,)  separable voxel grid
+* ``vox_idx``   int32    (n,)             optional list of linear voxel indices
+
+Frequencies: sample j is at f0 + j * df (Hz).  The reference's phase
+-2 pi d / lambda - 2 pi d j slope / c (imaging_gpu.cu:112) is this with
+f0 = c / lambda and df = slope.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+C0 = 2.99792458e8
+VARIANTS = {"plain": _lib.MMN_BP_PLAIN, "rec": _lib.MMN_BP_REC}
+
+
+def _check(t, name, dtype, ndim=None, last=None):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be on a ROCm device (got {t.device})")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype} (got {t.dtype})")
+    if ndim is not None and t.dim() != ndim:
+        raise ValueError(f"{name} must have {ndim} dims (got shape {tuple(t.shape)})")
+    if last is not None and t.shape[-1] != last:
+        raise ValueError(f"{name} must have last dim {last} (got shape {tuple(t.shape)})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+def _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx):
+    _lib.require_gpu()
+    _check(meas, "meas", torch.complex64, 2)
+    _check(ant, "ant", torch.float32, 2, 3)
+    for n, t in (("x_locs", x_locs), ("y_locs", y_locs), ("z_locs", z_locs)):
+        _check(t, n, torch.float32, 1)
+        if t.numel() == 0:
+            raise ValueError(f"{n} is empty")
+    if rx_off is None:
+        rx_off = torch.zeros((1, 3), dtype=torch.float32, device=meas.device)
+    _check(rx_off, "rx_off", torch.float32, 2, 3)
+    if ant.shape[0] != meas.shape[0]:
+        raise ValueError(f"ant has {ant.shape[0]} rows, meas has {meas.shape[0]}")
+    if meas.shape[0] == 0 or meas.shape[1] == 0:
+        raise ValueError("meas must have at least one row and one sample")
+    if rx_off.shape[0] == 0:
+        raise ValueError("rx_off must hold at least one RX offset")
+    nvox_grid = x_locs.numel() * y_locs.numel() * z_locs.numel()
+    if vox_idx is not None:
+        _check(vox_idx, "vox_idx", torch.int32, 1)
+    if nvox_grid >= 2 ** 31:
+        raise ValueError("voxel grid too large for int32 voxel indices")
+    return rx_off, nvox_grid
+
+
+def backproject(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
+                variant="rec", K=32, moment=False, vox_idx=None, is_ti=False):
+    """Raw full-frequency sum  I_v = sum_ij meas_ij exp(-j 2 pi (f0 + j df) d_iv / c).
+
+    Returns ``img`` complex64 (NX, NY, NZ) -- or (n,) with ``vox_idx`` -- and, if
+    ``moment``, the vote moment complex64 (.., 3).
+    """
+    rx_off, nvox_grid = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
+    if variant not in VARIANTS:
+        raise ValueError(f"variant must be one of {sorted(VARIANTS)}")
+    dev = meas.device
+    n = vox_idx.numel() if vox_idx is not None else nvox_grid
+    img = torch.empty(n, dtype=torch.complex64, device=dev)
+    mom = torch.empty((n, 3), dtype=torch.complex64, device=dev) if moment else None
+    if n:
+        _lib.call("mmn_bp_forward", img.data_ptr(), _lib.ptr(mom), x_locs.data_ptr(),
+                  y_locs.data_ptr(), z_locs.data_ptr(), _lib.ptr(vox_idx), n,
+                  ant.data_ptr(), meas.data_ptr(), rx_off.data_ptr(), float(f0), float(df),
+                  x_locs.numel(), y_locs.numel(), z_locs.numel(), meas.shape[0],
+                  rx_off.shape[0], meas.shape[1], int(bool(is_ti)), VARIANTS[variant],
+                  int(K), _lib.stream_handle(dev))
+    if vox_idx is None:
+        img = img.view(x_locs.numel(), y_locs.numel(), z_locs.numel())
+        if mom is not None:
+            mom = mom.view(x_locs.numel(), y_locs.numel(), z_locs.numel(), 3)
+    return (img, mom) if moment else img
+
+
+def cuda_image(meas, ant, x_locs, y_locs, z_locs, rx_off, *, slope, wavelength,
+               fft_spacing=1.0, is_ti=False, interpolated=False):
+    """The reference kernel's arithmetic (imaging_gpu.cu:48-122), raw sum (NX,NY,NZ)."""
+    rx_off, nvox = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, None)
+    img = torch.empty(nvox, dtype=torch.complex64, device=meas.device)
+    _lib.call("mmn_cuda_image", img.data_ptr(), x_locs.data_ptr(), y_locs.data_ptr(),
+              z_locs.data_ptr(), ant.data_ptr(), meas.data_ptr(), rx_off.data_ptr(),
+              float(slope), float(wavelength), float(fft_spacing), x_locs.numel(),
+              y_locs.numel(), z_locs.numel(), meas.shape[0], rx_off.shape[0], meas.shape[1],
+              int(bool(is_ti)), int(bool(interpolated)), _lib.stream_handle(meas.device))
+    return img.view(x_locs.numel(), y_locs.numel(), z_locs.numel())
+
+
+def sn_est(img, vox_idx, meas, ant, x_locs, y_locs, z_locs, rx_off, *, wavelength,
+           fft_spacing, is_ti=False):
+    """The reference vote (surface_normal_est.cu:64-149) over vox_idx, float32 (n, 3)."""
+    rx_off, nvox = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
+    _check(img, "img", torch.complex64)
+    if img.numel() != nvox:
+        raise ValueError("img must cover the whole voxel grid")
+    n = vox_idx.numel()
+    out = torch.empty((n, 3), dtype=torch.float32, device=meas.device)
+    _lib.call("mmn_cuda_sn_est", out.data_ptr(), vox_idx.data_ptr(), n, img.data_ptr(),
+              x_locs.data_ptr(), y_locs.data_ptr(), z_locs.data_ptr(), ant.data_ptr(),
+              meas.data_ptr(), rx_off.data_ptr(), 0.0, float(wavelength),
+              float(fft_spacing), x_locs.numel(), y_locs.numel(), z_locs.numel(),
+              meas.shape[0], rx_off.shape[0], meas.shape[1], int(bool(is_ti)),
+              _lib.stream_handle(meas.device))
+    return out
+
+
+def sn_from_moment(img, moment, vox_idx):
+    """Re(conj(I) M) / |I| with I = img.flatten()[vox_idx]; float32 (n, 3)."""
+    _lib.require_gpu()
+    _check(img, "img", torch.complex64)
+    _check(moment, "moment", torch.complex64, 2, 3)
+    _check(vox_idx, "vox_idx", torch.int32, 1)
+    if moment.shape[0] != vox_idx.numel():
+        raise ValueError("moment rows must match vox_idx")
+    n = vox_idx.numel()
+    out = torch.empty((n, 3), dtype=torch.float32, device=img.device)
+    _lib.call("mmn_sn_from_moment", out.data_ptr(), img.data_ptr(), moment.data_ptr(),
+              vox_idx.data_ptr(), n, _lib.stream_handle(img.device))
+    return out
+
+
+def backproject_backward(g, meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
+                         scale=1.0, K=32, vox_idx=None, is_ti=False, need_meas=True,
+                         need_ant=True):
+    """Adjoint of ``scale * backproject``: (dL/dmeas complex64 (rows,F), dL/dant float32 (rows,3))."""
+    rx_off, nvox_grid = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
+    n = vox_idx.numel() if vox_idx is not None else nvox_grid
+    g = g.reshape(-1)
+    _check(g, "grad", torch.complex64, 1)
+    if g.numel() != n:
+        raise ValueError(f"grad has {g.numel()} entries, expected {n}")
+    dev = meas.device
+    A, F = meas.shape
+    gm = torch.empty((A, F), dtype=torch.complex64, device=dev) if need_meas else None
+    ga = torch.empty((A, 3), dtype=torch.float32, device=dev) if need_ant else None
+    lib = _lib.load()
+    ws_bytes = int(lib.mmn_bp_backward_workspace_size(A, F, n))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    _lib.call("mmn_bp_backward", _lib.ptr(gm), _lib.ptr(ga), g.data_ptr(), float(scale),
+              x_locs.data_ptr(), y_locs.data_ptr(), z_locs.data_ptr(), _lib.ptr(vox_idx), n,
+              ant.data_ptr(), meas.data_ptr(), rx_off.data_ptr(), float(f0), float(df),
+              x_locs.numel(), y_locs.numel(), z_locs.numel(), A, rx_off.shape[0], F,
+              int(bool(is_ti)), int(K), ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
+    return gm, ga
+
+
+class Backproject(torch.autograd.Function):
+    """I = scale * backproject(meas, ant, ...), differentiable in meas and ant.
+
+    The backward kernel recomputes the phases (no rows x samples x voxels
+    tensor is stored); gradients follow PyTorch's complex convention.
+    """
+
+    @staticmethod
+    def forward(ctx, meas, ant, x_locs, y_locs, z_locs, rx_off, f0, df, scale, K, vox_idx,
+                is_ti):
+        img = backproject(meas, ant, x_locs, y_locs, z_locs, rx_off, f0=f0, df=df,
+                          variant="rec", K=K, vox_idx=vox_idx, is_ti=is_ti)
+        ctx.save_for_backward(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
+        ctx.args = (f0, df, scale, min(K, 32), is_ti)
+        return img * scale
+
+    @staticmethod
+    def backward(ctx, grad):
+        meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx = ctx.saved_tensors
+        f0, df, scale, K, is_ti = ctx.args
+        need_m, need_a = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_m or need_a):
+            return (None,) * 12
+        gm, ga = backproject_backward(grad.contiguous(), meas, ant, x_locs, y_locs, z_locs,
+                                      rx_off, f0=f0, df=df, scale=scale, K=K,
+                                      vox_idx=vox_idx, is_ti=is_ti, need_meas=need_m,
+                                      need_ant=need_a)
+        return gm, ga, None, None, None, None, None, None, None, None, None, None
+
+
+def backproject_ad(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df, scale=1.0,
+                   K=32, vox_idx=None, is_ti=False):
+    """Differentiable ``scale * backproject`` (recurrence kernel forward)."""
+    if rx_off is None:
+        rx_off = torch.zeros((1, 3), dtype=torch.float32, device=meas.device)
+    return Backproject.apply(meas, ant, x_locs, y_locs, z_locs, rx_off, f0, df, scale, K,
+                             vox_idx, is_ti)
+
+
+def rsdf_bfs(normals, labels, seeds, dx):
+    """FIFO-exact RSDF breadth-first search on the GPU.
+
+    normals float64 (NX,NY,NZ,3), labels int32 (NX,NY,NZ), seeds int32 (n,)
+    linear start voxels in component order.  Returns (rsdf float64, hops int32),
+    both (NX,NY,NZ).
+    """
+    _lib.require_gpu()
+    _check(normals, "normals", torch.float64, 4, 3)
+    _check(labels, "labels", torch.int32, 3)
+    _check(seeds, "seeds", torch.int32, 1)
+    if tuple(normals.shape[:3]) != tuple(labels.shape):
+        raise ValueError("normals and labels must cover the same grid")
+    nx, ny, nz = labels.shape
+    V = nx * ny * nz
+    if seeds.numel():
+        lo, hi = int(seeds.min()), int(seeds.max())
+        if lo < 0 or hi >= V:
+            raise ValueError("seed index outside the grid")
+    dev = labels.device
+    rsdf = torch.empty((nx, ny, nz), dtype=torch.float64, device=dev)
+    hops = torch.empty((nx, ny, nz), dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    ws_bytes = int(lib.mmn_rsdf_bfs_workspace_size(nx, ny, nz))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    _lib.call("mmn_rsdf_bfs", rsdf.data_ptr(), hops.data_ptr(), normals.data_ptr(),
+              labels.data_ptr(), seeds.data_ptr(), seeds.numel(), float(dx), nx, ny, nz,
+              ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
+    return rsdf, hops

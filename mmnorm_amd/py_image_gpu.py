@@ -1,0 +1,124 @@
+"""Drop-in for ``py_image_gpu.image_cuda`` (py_image_gpu.py:308-366).
+
+Same signature, argument meaning, normalisation and return values as the
+reference; NumPy arrays in and out.  The arithmetic runs in
+libmmnorm_hip.so on the current ROCm device:
+
+* full matched filter (``use_interpolated_processing=False``): by default the
+  recurrence kernel (``variant="rec"``, FP32 Horner over samples with FP64
+  re-anchoring every ``K`` samples); ``variant="ref"`` runs the reference
+  kernel's own arithmetic (FP64 phase and cos/sin per sample) and
+  ``variant="plain"`` the direct FP32 sincosf kernel;
+* interpolated range-bin branch: the reference kernel's arithmetic.
+
+Host steps mirror the reference: TX rows repeated per RX and measurements
+reordered to (meas, rx, sample) when ``use_4_rx`` (py_image_gpu.py:228-234);
+division by the TX count (py_image_gpu.py:302), a second time for the full
+branch (py_image_gpu.py:330); antenna groups of 131072 rows for the
+interpolated branch, each normalised by its own TX count before summing
+(py_image_gpu.py:339-362); phase centres (p + (p + o)) / 2
+(py_image_gpu.py:332-336).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, ops
+
+MAX_GROUP_ROWS = 4096 * 32  # py_image_gpu.py:339
+
+
+def _prep_inputs(radar_data, rx_offset, antenna_locs, use_4_rx):
+    """Layout of py_image_gpu.py:219-235: rows = meas x rx, samples last."""
+    radar_data = np.asarray(radar_data)
+    antenna_locs = np.asarray(antenna_locs)
+    if use_4_rx:
+        radar_data = np.transpose(radar_data, (0, 2, 1))
+        antenna_locs = np.repeat(antenna_locs, radar_data.shape[1], axis=0)
+    rx_offset = np.asarray(rx_offset, dtype=np.float64).reshape(-1, 3)
+    meas = np.ascontiguousarray(radar_data.reshape(-1, radar_data.shape[-1]))
+    return rx_offset, meas, antenna_locs
+
+
+def _device():
+    _lib.require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _to_dev(a, dtype, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(dev)
+
+
+def _run_image(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offset, slope,
+               wavelength, bandwidth, num_samples, use_4_rx, is_ti_radar, interpolated,
+               variant, K):
+    """py_image_gpu.py:237-305 on one device: returns (image / N, N, rows' TX positions)."""
+    norm_factor = np.asarray(antenna_locs).shape[0]
+    samples_per_meas = np.asarray(radar_data).shape[1]
+    rx, meas, ant_rows = _prep_inputs(radar_data, rx_offset, antenna_locs, use_4_rx)
+    fft_spacing = np.float32(3e8 / (2 * bandwidth) * num_samples / samples_per_meas)
+    dev = _device()
+    xl, yl, zl = (_to_dev(a, np.float32, dev) for a in (x_locs, y_locs, z_locs))
+    ant_t = _to_dev(np.asarray(ant_rows).reshape(-1, 3), np.float32, dev)
+    rx_t = _to_dev(rx, np.float32, dev)
+    meas_t = _to_dev(meas, np.complex64, dev)
+    if meas_t.shape[0] != ant_t.shape[0]:
+        raise ValueError(f"{meas_t.shape[0]} measurement rows for {ant_t.shape[0]} antenna rows")
+    if interpolated or variant == "ref":
+        raw = ops.cuda_image(meas_t, ant_t, xl, yl, zl, rx_t, slope=np.float32(slope),
+                             wavelength=np.float32(wavelength), fft_spacing=fft_spacing,
+                             is_ti=is_ti_radar, interpolated=interpolated)
+    else:
+        f0 = ops.C0 / float(np.float32(wavelength))
+        df = float(np.float32(slope))
+        raw = ops.backproject(meas_t, ant_t, xl, yl, zl, rx_t, f0=f0, df=df,
+                              variant=variant, K=K, is_ti=is_ti_radar)
+    raw = raw.cpu().numpy()
+    p_xyz = raw.real + 1j * raw.imag  # complex64, as p_xyz_r + 1j*p_xyz_i
+    p_xyz = p_xyz.astype(np.complex64, copy=False)
+    p_xyz /= norm_factor
+    return p_xyz, norm_factor, ant_rows
+
+
+def _phase_centers(locs_rep, rx_offsets):
+    rx_offsets = np.asarray(rx_offsets)
+    rx_rep = np.tile(rx_offsets, reps=(len(locs_rep) // len(rx_offsets), 1))
+    rx_locs = np.array(locs_rep, copy=True) + rx_rep
+    return (locs_rep + rx_locs) / 2
+
+
+def image_cuda(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offsets, slope,
+               wavelength, bandwidth, num_samples, use_4_rx=False, is_ti_radar=False,
+               use_interpolated_processing=False, return_phase_centers=True, *,
+               variant="rec", K=32):
+    """SAR image on the GPU (see module docstring); returns (image, centers) or image."""
+    if variant not in ("rec", "plain", "ref"):
+        raise ValueError("variant must be 'rec', 'plain' or 'ref'")
+    if not use_interpolated_processing:
+        sum_image, norm_factor, locs_rep = _run_image(
+            x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offsets, slope,
+            wavelength, bandwidth, num_samples, use_4_rx, is_ti_radar, False, variant, K)
+        sum_image /= norm_factor
+        center_locs = _phase_centers(locs_rep, rx_offsets)
+    else:
+        num_rx = len(rx_offsets)
+        antenna_locs = np.asarray(antenna_locs)
+        radar_data = np.asarray(radar_data)
+        groups = math.ceil(antenna_locs.shape[0] * num_rx / MAX_GROUP_ROWS)
+        sum_image, centers = None, []
+        for i in range(groups):
+            lo, hi = i * MAX_GROUP_ROWS // num_rx, (i + 1) * MAX_GROUP_ROWS // num_rx
+            img, _, locs_rep = _run_image(
+                x_locs, y_locs, z_locs, antenna_locs[lo:hi], radar_data[lo:hi], rx_offsets,
+                slope, wavelength, bandwidth, num_samples, use_4_rx, is_ti_radar, True,
+                variant, K)
+            if return_phase_centers:
+                centers.append(_phase_centers(locs_rep, rx_offsets))
+            sum_image = img if sum_image is None else sum_image + img
+        center_locs = np.concatenate(centers, axis=0) if centers else None
+    if return_phase_centers:
+        return sum_image, center_locs
+    return sum_image

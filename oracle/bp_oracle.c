@@ -1,0 +1,289 @@
+/*
+ * CPU oracle for the mmNorm image-formation hot path.  TEST INFRASTRUCTURE
+ * ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg, as the checker and the CPU baseline -- never by the
+ * product path (mmnorm_amd/), which has no CPU fallback.
+ *
+ * Built with -ffp-contract=off (no FMA contraction) so each product and sum
+ * rounds on its own, as the reference's expressions are written.
+ *
+ *   or_bp_ref      restates the `cuda_image` kernel (imaging_gpu.cu:48-122):
+ *                  FP32 distance, FP64 phase, FP64 cos/sin narrowed to FP32,
+ *                  FP32 complex sums in row order; both branches.
+ *   or_sn_ref      restates the `cuda_sn_est` kernel
+ *                  (surface_normal_est.cu:64-149).
+ *   or_bp_f64      the FP64 oracle O64 of the full-frequency sum: distance,
+ *                  phase, cos/sin and sums all in FP64 from the FP32 inputs,
+ *                  plus the vote moment sum_i (p_i + o_i/2 - v) c_i.
+ *   or_bp_cpu_imaging  restates the reference's CPU imaging loop nest
+ *                  (imaging.cpp:60-139, full branch, OpenMP over x slices):
+ *                  the host-core baseline that bench.py times.
+ *   or_bfs_fifo    restates the FIFO breadth-first search of
+ *                  _construct_rsdf_components (sdf.py:180-222) for given
+ *                  seeds.
+ *
+ * Parity pins for each function are listed in oracle/README.md.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define C0 2.99792458e8
+static const double PI_D = 3.14159265358979323846264338327950288;
+
+static void coords(const float *xl, const float *yl, const float *zl, int ny,
+                   int nz, int64_t lin, float *vx, float *vy, float *vz) {
+  int64_t yz = (int64_t)ny * nz;
+  int64_t x = lin / yz, r = lin - x * yz;
+  int64_t y = r / nz, z = r - y * nz;
+  *vx = xl[x];
+  *vy = yl[y];
+  *vz = zl[z];
+}
+
+/* FP32 round-trip distance as imaging_gpu.cu:77-88 forms it. */
+static float dist_ref32(float vx, float vy, float vz, const float *p,
+                        const float *o, int is_ti) {
+  float ax = vx - p[0], ay = vy - p[1], az = vz - p[2];
+  float fwd = sqrtf(ax * ax + ay * ay + az * az);
+  float bx = ax - o[0], by = ay - o[1], bz = az - o[2];
+  float back = sqrtf(bx * bx + by * by + bz * bz);
+  float d = fwd + back;
+  if (is_ti) d = (float)((double)d + 0.15);
+  return d;
+}
+
+/* imaging_gpu.cu:48-122.  meas: rows x S complex (interleaved float pairs). */
+void or_bp_ref(float *img, const float *xl, const float *yl, const float *zl,
+               const float *ant, const float *meas, const float *rxo,
+               float slope, float wavelength, float fft_spacing, int nx,
+               int ny, int nz, int rows, int R, int S, int is_ti, int interp,
+               const int64_t *vidx, int64_t nsel) {
+  int64_t n = vidx ? nsel : (int64_t)nx * ny * nz;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t t = 0; t < n; ++t) {
+    int64_t lin = vidx ? vidx[t] : t;
+    float vx, vy, vz;
+    coords(xl, yl, zl, ny, nz, lin, &vx, &vy, &vz);
+    float sr = 0.0f, si = 0.0f;
+    for (int i = 0; i < rows; ++i) {
+      float d = dist_ref32(vx, vy, vz, ant + 3 * i, rxo + 3 * (i % R), is_ti);
+      const float *row = meas + (size_t)2 * i * S;
+      if (interp) {
+        if (d < 0.0f || d > fft_spacing * (float)S) continue;
+        int bin = (int)floorf(d / fft_spacing / 2.0f);
+        float mr = row[2 * bin], mi = row[2 * bin + 1];
+        double th = -2.0 * PI_D * (double)d / (double)wavelength;
+        float pr = (float)cos(th), pi = (float)sin(th);
+        sr += mr * pr - mi * pi;
+        si += mr * pi + mi * pr;
+      } else {
+        double a0 = -2.0 * PI_D * (double)d / (double)wavelength;
+        for (int j = 0; j < S; ++j) {
+          float mr = row[2 * j], mi = row[2 * j + 1];
+          double th = a0 + -2.0 * PI_D * (double)d * (double)j * (double)slope / C0;
+          float pr = (float)cos(th), pi = (float)sin(th);
+          sr += mr * pr - mi * pi;
+          si += mr * pi + mi * pr;
+        }
+      }
+    }
+    img[2 * t] = sr;
+    img[2 * t + 1] = si;
+  }
+}
+
+/* surface_normal_est.cu:64-149.  img indexed by linear voxel index. */
+void or_sn_ref(float *out, const int64_t *vidx, int64_t nvalid,
+               const float *img, const float *xl, const float *yl,
+               const float *zl, const float *ant, const float *meas,
+               const float *rxo, float wavelength, float fft_spacing, int ny,
+               int nz, int rows, int R, int S, int is_ti) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t t = 0; t < nvalid; ++t) {
+    int64_t lin = vidx[t];
+    float vx, vy, vz;
+    coords(xl, yl, zl, ny, nz, lin, &vx, &vy, &vz);
+    float ir = img[2 * lin], ii = img[2 * lin + 1];
+    float mag = sqrtf(ir * ir + ii * ii);
+    float snx = 0.0f, sny = 0.0f, snz = 0.0f;
+    for (int i = 0; i < rows; ++i) {
+      const float *p = ant + 3 * i, *o = rxo + 3 * (i % R);
+      float d = dist_ref32(vx, vy, vz, p, o, is_ti);
+      if (d < 0.0f || d > fft_spacing * (float)S) continue;
+      int bin = (int)floorf(d / fft_spacing / 2.0f);
+      const float *row = meas + (size_t)2 * i * S;
+      float mr = row[2 * bin], mi = row[2 * bin + 1];
+      double th = -2.0 * PI_D * (double)d / (double)wavelength;
+      float pr = (float)cos(th), pi = (float)sin(th);
+      float cr = mr * pr - mi * pi;
+      float ci = mr * pi + mi * pr;
+      float vecx = (p[0] + o[0] / 2.0f) - vx;
+      float vecy = (p[1] + o[1] / 2.0f) - vy;
+      float vecz = (p[2] + o[2] / 2.0f) - vz;
+      float w = (cr * ir) + (ci * ii);
+      w /= mag;
+      snx += vecx * w;
+      sny += vecy * w;
+      snz += vecz * w;
+    }
+    out[3 * t] = snx;
+    out[3 * t + 1] = sny;
+    out[3 * t + 2] = snz;
+  }
+}
+
+/* FP64 oracle O64.  out: n complex doubles; mom (optional): n x 3 complex. */
+void or_bp_f64(double *out, double *mom, const float *xl, const float *yl,
+               const float *zl, const float *ant, const float *meas,
+               const float *rxo, double f0, double df, int nx, int ny, int nz,
+               int rows, int R, int F, int is_ti, const int64_t *vidx,
+               int64_t nsel) {
+  int64_t n = vidx ? nsel : (int64_t)nx * ny * nz;
+  const double k0 = 2.0 * PI_D * f0 / C0, dk = 2.0 * PI_D * df / C0;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t t = 0; t < n; ++t) {
+    int64_t lin = vidx ? vidx[t] : t;
+    float fx, fy, fz;
+    coords(xl, yl, zl, ny, nz, lin, &fx, &fy, &fz);
+    double vx = fx, vy = fy, vz = fz;
+    double ir = 0.0, ii = 0.0, m[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < rows; ++i) {
+      const float *p = ant + 3 * i, *o = rxo + 3 * (i % R);
+      double ax = vx - p[0], ay = vy - p[1], az = vz - p[2];
+      double bx = ax - o[0], by = ay - o[1], bz = az - o[2];
+      double d = sqrt(ax * ax + ay * ay + az * az) + sqrt(bx * bx + by * by + bz * bz);
+      if (is_ti) d += 0.15;
+      const float *row = meas + (size_t)2 * i * F;
+      double cr = 0.0, ci = 0.0;
+      for (int j = 0; j < F; ++j) {
+        double th = (k0 + (double)j * dk) * d;
+        double c = cos(th), s = sin(th);
+        double mr = row[2 * j], mi = row[2 * j + 1];
+        /* s * exp(-j th) */
+        cr += mr * c + mi * s;
+        ci += mi * c - mr * s;
+      }
+      ir += cr;
+      ii += ci;
+      double vecx = ((double)p[0] + (double)o[0] / 2.0) - vx;
+      double vecy = ((double)p[1] + (double)o[1] / 2.0) - vy;
+      double vecz = ((double)p[2] + (double)o[2] / 2.0) - vz;
+      m[0] += vecx * cr; m[1] += vecx * ci;
+      m[2] += vecy * cr; m[3] += vecy * ci;
+      m[4] += vecz * cr; m[5] += vecz * ci;
+    }
+    out[2 * t] = ir;
+    out[2 * t + 1] = ii;
+    if (mom)
+      for (int q = 0; q < 6; ++q) mom[6 * t + q] = m[q];
+  }
+}
+
+/*
+ * imaging.cpp:60-139, full branch: FP32 complex sums, the phase formed in
+ * double and narrowed to float for a complex<float> exponential, per TX and
+ * per RX.  Here meas is rows x F with row = tx * R + rx (the layout of
+ * or_bp_ref), ant holds one position per row.  Output divided by the TX
+ * count as imaging.cpp:119 does.
+ */
+void or_bp_cpu_imaging(float *img, const float *xl, const float *yl,
+                       const float *zl, const float *ant, const float *meas,
+                       const float *rxo, float slope, float wavelength,
+                       int nx, int ny, int nz, int n_tx, int R, int F,
+                       int is_ti, const int64_t *vidx, int64_t nsel) {
+  int64_t n = vidx ? nsel : (int64_t)nx * ny * nz;
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < n; ++t) {
+    int64_t lin = vidx ? vidx[t] : t;
+    float vx, vy, vz;
+    coords(xl, yl, zl, ny, nz, lin, &vx, &vy, &vz);
+    float complex sum = 0.0f;
+    for (int i = 0; i < n_tx; ++i) {
+      const float *p = ant + 3 * (size_t)i * R;
+      float ax = vx - p[0], ay = vy - p[1], az = vz - p[2];
+      float fwd = sqrtf(ax * ax + ay * ay + az * az);
+      if (is_ti) fwd = (float)((double)fwd + 0.15);
+      for (int k = 0; k < R; ++k) {
+        const float *o = rxo + 3 * k;
+        float rxd = vx - (p[0] + o[0]), ryd = vy - (p[1] + o[1]), rzd = vz - (p[2] + o[2]);
+        float back = sqrtf(rxd * rxd + ryd * ryd + rzd * rzd);
+        float d = fwd + back;
+        const float *row = meas + (size_t)2 * ((size_t)i * R + k) * F;
+        float a0 = (float)(-2.0 * PI_D * (double)d / (double)wavelength);
+        float complex e0 = cosf(a0) + I * sinf(a0);
+        for (int j = 0; j < F; ++j) {
+          float a1 = (float)(-2.0 * PI_D * (double)d * (double)j * (double)slope / C0);
+          float complex e1 = cosf(a1) + I * sinf(a1);
+          float complex m = row[2 * j] + I * row[2 * j + 1];
+          sum += m * e1 * e0;
+        }
+      }
+    }
+    sum /= (float)n_tx;
+    img[2 * t] = crealf(sum);
+    img[2 * t + 1] = cimagf(sum);
+  }
+}
+
+/*
+ * sdf.py:180-222 for the given seeds (one per component, in visiting order).
+ * rsdf/hops must hold V entries; they are initialised here (NaN / -1).
+ * visited is shared across components as in sdf.py:142.
+ */
+static const int DXS[6] = {-1, 0, 1, 0, 0, 0};
+static const int DYS[6] = {0, 1, 0, -1, 0, 0};
+static const int DZS[6] = {0, 0, 0, 0, 1, -1};
+
+int or_bfs_fifo(double *rsdf, int32_t *hops, const double *nrm,
+                const int32_t *lab, const int64_t *seeds, int n_seeds,
+                double dx, int nx, int ny, int nz) {
+  int64_t V = (int64_t)nx * ny * nz;
+  uint8_t *visited = calloc((size_t)V, 1);
+  int64_t *q = malloc(sizeof(int64_t) * (size_t)(V > 0 ? V : 1));
+  if (!visited || !q) {
+    free(visited);
+    free(q);
+    return -1;
+  }
+  for (int64_t i = 0; i < V; ++i) {
+    rsdf[i] = NAN;
+    hops[i] = -1;
+  }
+  for (int c = 0; c < n_seeds; ++c) {
+    int64_t s = seeds[c];
+    int32_t label = lab[s];
+    rsdf[s] = 0.0;
+    if (hops[s] < 0) hops[s] = 0;
+    int64_t head = 0, tail = 0;
+    q[tail++] = s;
+    visited[s] = 1;
+    while (head < tail) {
+      int64_t u = q[head++];
+      int x = (int)(u / ((int64_t)ny * nz));
+      int y = (int)((u / nz) % ny);
+      int z = (int)(u % nz);
+      double prev = rsdf[u];
+      for (int d = 0; d < 6; ++d) {
+        int ax = x + DXS[d], ay = y + DYS[d], az = z + DZS[d];
+        if (ax < 0 || ay < 0 || az < 0 || ax >= nx || ay >= ny || az >= nz) continue;
+        int64_t cidx = ((int64_t)ax * ny + ay) * nz + az;
+        if (visited[cidx]) continue;
+        if (isnan(nrm[3 * cidx])) continue;
+        if (lab[cidx] != label) continue;
+        q[tail++] = cidx;
+        visited[cidx] = 1;
+        hops[cidx] = hops[u] + 1;
+        double t0 = ((double)DXS[d] * dx) * nrm[3 * cidx];
+        double t1 = ((double)DYS[d] * dx) * nrm[3 * cidx + 1];
+        double t2 = ((double)DZS[d] * dx) * nrm[3 * cidx + 2];
+        rsdf[cidx] = prev + ((t0 + t1) + t2);
+      }
+    }
+  }
+  free(visited);
+  free(q);
+  return 0;
+}

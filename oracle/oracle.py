@@ -1,0 +1,281 @@
+"""CPU oracle for the image-formation hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, and only as the checker or the timed CPU baseline.  The product
+(package ``mmnorm_amd``) never calls it.
+
+Two independent restatements of the reference's arithmetic live here:
+
+* ``liboracle.so`` (oracle/bp_oracle.c, built by oracle/Makefile), fast,
+  OpenMP over voxels -- or_bp_ref (imaging_gpu.cu:48-122), or_sn_ref
+  (surface_normal_est.cu:64-149), or_bp_f64 (FP64 oracle O64 of the full
+  sum + vote moment), or_bp_cpu_imaging (imaging.cpp:60-139) and
+  or_bfs_fifo (sdf.py:180-222);
+* pure-Python loops below (``py_*``) for tiny inputs, written line by line
+  from the same reference lines, used to cross-check the C code.
+
+Pins (see tests/test_oracle_*.py): closed forms (point-scatterer focus
+I = rho*A*F, single-row magnitude, single-antenna normal direction, symmetric
+aperture normal (0,0,1), constant-normal RSDF = dx*(k-k0), BFS hops =
+Manhattan distance in a full box), invariants (linearity, translation,
+row permutation, label renumbering) and brute force (the py_* loops).  The
+reference ships no golden vectors, tests or fixtures for this path.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from collections import deque
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+C0 = 2.99792458e8
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        p, i32, i64, f32, f64 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                 ctypes.c_float, ctypes.c_double)
+        L.or_bp_ref.argtypes = [p, p, p, p, p, p, p, f32, f32, f32, i32, i32, i32,
+                                i32, i32, i32, i32, i32, p, i64]
+        L.or_sn_ref.argtypes = [p, p, i64, p, p, p, p, p, p, p, f32, f32, i32, i32,
+                                i32, i32, i32, i32]
+        L.or_bp_f64.argtypes = [p, p, p, p, p, p, p, p, f64, f64, i32, i32, i32,
+                                i32, i32, i32, i32, p, i64]
+        L.or_bp_cpu_imaging.argtypes = [p, p, p, p, p, p, p, f32, f32, i32, i32, i32,
+                                        i32, i32, i32, i32, p, i64]
+        L.or_bfs_fifo.argtypes = [p, p, p, p, p, i32, f64, i32, i32, i32]
+        L.or_bfs_fifo.restype = i32
+        for fn in (L.or_bp_ref, L.or_sn_ref, L.or_bp_f64, L.or_bp_cpu_imaging):
+            fn.restype = None
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.complex64)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _sel(vidx):
+    if vidx is None:
+        return None, 0
+    v = np.ascontiguousarray(vidx, dtype=np.int64)
+    return v, len(v)
+
+
+def bp_ref(xl, yl, zl, ant, meas, rxo, slope, wavelength, fft_spacing=1.0,
+           is_ti=False, interp=False, vidx=None):
+    """Raw sum of `cuda_image` (imaging_gpu.cu:48-122), complex64.
+
+    ant: (rows, 3) positions per row; meas: (rows, S) complex; rxo: (R, 3).
+    """
+    xl, yl, zl, ant, rxo = map(_f32, (xl, yl, zl, ant, rxo))
+    meas = _c64(meas)
+    rows, S = meas.shape
+    v, n = _sel(vidx)
+    nv = n if v is not None else len(xl) * len(yl) * len(zl)
+    out = np.zeros(nv, np.complex64)
+    lib().or_bp_ref(_ptr(out), _ptr(xl), _ptr(yl), _ptr(zl), _ptr(ant), _ptr(meas),
+                    _ptr(rxo), float(np.float32(slope)), float(np.float32(wavelength)),
+                    float(np.float32(fft_spacing)), len(xl), len(yl), len(zl), rows,
+                    len(rxo), S, int(bool(is_ti)), int(bool(interp)), _ptr(v), n)
+    return out
+
+
+def sn_ref(vidx, img, xl, yl, zl, ant, meas, rxo, wavelength, fft_spacing, is_ti=False):
+    """Raw vote of `cuda_sn_est` (surface_normal_est.cu:64-149), float32 (n, 3)."""
+    xl, yl, zl, ant, rxo = map(_f32, (xl, yl, zl, ant, rxo))
+    meas, img = _c64(meas), _c64(np.ravel(img))
+    rows, S = meas.shape
+    v = np.ascontiguousarray(vidx, dtype=np.int64)
+    out = np.zeros((len(v), 3), np.float32)
+    lib().or_sn_ref(_ptr(out), _ptr(v), len(v), _ptr(img), _ptr(xl), _ptr(yl), _ptr(zl),
+                    _ptr(ant), _ptr(meas), _ptr(rxo), float(np.float32(wavelength)),
+                    float(np.float32(fft_spacing)), len(yl), len(zl), rows, len(rxo), S,
+                    int(bool(is_ti)))
+    return out
+
+
+def bp_f64(xl, yl, zl, ant, meas, rxo, f0, df, is_ti=False, vidx=None, moment=False):
+    """FP64 oracle O64 of the full sum (and vote moment), from FP32-rounded inputs.
+
+    Returns complex128 (n,) [, complex128 (n, 3)].
+    """
+    xl, yl, zl, ant, rxo = map(_f32, (xl, yl, zl, ant, rxo))
+    meas = _c64(meas)
+    rows, F = meas.shape
+    v, n = _sel(vidx)
+    nv = n if v is not None else len(xl) * len(yl) * len(zl)
+    out = np.zeros(nv, np.complex128)
+    mom = np.zeros((nv, 3), np.complex128) if moment else None
+    lib().or_bp_f64(_ptr(out), _ptr(mom), _ptr(xl), _ptr(yl), _ptr(zl), _ptr(ant),
+                    _ptr(meas), _ptr(rxo), float(f0), float(df), len(xl), len(yl),
+                    len(zl), rows, len(rxo), F, int(bool(is_ti)), _ptr(v), n)
+    return (out, mom) if moment else out
+
+
+def bp_cpu_imaging(xl, yl, zl, ant_tx, meas, rxo, slope, wavelength, is_ti=False,
+                   vidx=None):
+    """imaging.cpp:60-139 full branch (CPU baseline formulation), complex64.
+
+    ant_tx: (n_tx, 3); meas: (n_tx * R, F) with row = tx * R + rx.
+    """
+    xl, yl, zl, rxo = map(_f32, (xl, yl, zl, rxo))
+    R = len(rxo)
+    ant_rows = _f32(np.repeat(np.asarray(ant_tx, np.float32), R, axis=0))
+    meas = _c64(meas)
+    F = meas.shape[1]
+    v, n = _sel(vidx)
+    nv = n if v is not None else len(xl) * len(yl) * len(zl)
+    out = np.zeros(nv, np.complex64)
+    lib().or_bp_cpu_imaging(_ptr(out), _ptr(xl), _ptr(yl), _ptr(zl), _ptr(ant_rows),
+                            _ptr(meas), _ptr(rxo), float(np.float32(slope)),
+                            float(np.float32(wavelength)), len(xl), len(yl), len(zl),
+                            len(ant_tx), R, F, int(bool(is_ti)), _ptr(v), n)
+    return out
+
+
+def bfs_fifo(normals, labels, seeds, dx):
+    """sdf.py:180-222 from given seeds; returns (rsdf float64 V, hops int32 V)."""
+    nx, ny, nz = labels.shape
+    nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+    lab = np.ascontiguousarray(labels, dtype=np.int32).ravel()
+    s = np.ascontiguousarray(seeds, dtype=np.int64)
+    V = nx * ny * nz
+    rsdf = np.empty(V, np.float64)
+    hops = np.empty(V, np.int32)
+    rc = lib().or_bfs_fifo(_ptr(rsdf), _ptr(hops), _ptr(nrm), _ptr(lab), _ptr(s), len(s),
+                           float(dx), nx, ny, nz)
+    if rc != 0:
+        raise MemoryError("or_bfs_fifo: allocation failed")
+    return rsdf.reshape(nx, ny, nz), hops.reshape(nx, ny, nz)
+
+
+# ----------------------------------------------------------------------------
+# Seeds as _construct_rsdf_components picks them (sdf.py:180-191).
+# ----------------------------------------------------------------------------
+def rsdf_seeds(all_normals, coords_full, labeled, valid_labels):
+    """Linear index of each component's start voxel, in valid_labels order."""
+    seeds = []
+    ok = ~np.isnan(all_normals[:, :, :, 0])
+    for comp in valid_labels:
+        mask = np.logical_and(labeled == comp, ok)
+        idxs = np.argwhere(mask)
+        sel = coords_full[mask, :]
+        mean = np.mean(sel, axis=0)[None, :]
+        k = int(np.argmin(np.linalg.norm(sel - mean, axis=1)))
+        i0, j0, k0 = idxs[k]
+        seeds.append(np.ravel_multi_index((i0, j0, k0), labeled.shape))
+    return np.asarray(seeds, np.int64)
+
+
+# ----------------------------------------------------------------------------
+# Pure-Python restatements for tiny inputs (cross-checks of the C code).
+# ----------------------------------------------------------------------------
+def py_bp_ref(xl, yl, zl, ant, meas, rxo, slope, wavelength, fft_spacing=1.0,
+              is_ti=False, interp=False):
+    """imaging_gpu.cu:48-122 with numpy float32/float64 scalars, tiny inputs only."""
+    f32, f64 = np.float32, np.float64
+    xl, yl, zl, ant, rxo = map(_f32, (xl, yl, zl, ant, rxo))
+    meas = _c64(meas)
+    rows, S = meas.shape
+    R = len(rxo)
+    slope, wavelength, fs = f32(slope), f32(wavelength), f32(fft_spacing)
+    out = np.zeros(len(xl) * len(yl) * len(zl), np.complex64)
+    pi = f64(math.pi)
+    for x in range(len(xl)):
+        for y in range(len(yl)):
+            for z in range(len(zl)):
+                sr, si = f32(0), f32(0)
+                for i in range(rows):
+                    dx, dy, dz = xl[x] - ant[i, 0], yl[y] - ant[i, 1], zl[z] - ant[i, 2]
+                    o = rxo[i % R]
+                    fwd = np.sqrt(f32(f32(dx * dx) + f32(dy * dy)) + f32(dz * dz), dtype=f32)
+                    bx, by, bz = f32(dx - o[0]), f32(dy - o[1]), f32(dz - o[2])
+                    back = np.sqrt(f32(f32(bx * bx) + f32(by * by)) + f32(bz * bz), dtype=f32)
+                    d = f32(fwd + back)
+                    if is_ti:
+                        d = f32(f64(d) + 0.15)
+                    if interp:
+                        if d < 0 or d > f32(fs * f32(S)):
+                            continue
+                        b = int(np.floor(f32(f32(d / fs) / f32(2))))
+                        m = meas[i, b]
+                        th = f64(-2.0) * pi * f64(d) / f64(wavelength)
+                        pr, pim = f32(np.cos(th)), f32(np.sin(th))
+                        sr = f32(sr + f32(f32(m.real * pr) - f32(m.imag * pim)))
+                        si = f32(si + f32(f32(m.real * pim) + f32(m.imag * pr)))
+                    else:
+                        for j in range(S):
+                            m = meas[i, j]
+                            th = (f64(-2.0) * pi * f64(d) / f64(wavelength)
+                                  + f64(-2.0) * pi * f64(d) * f64(j) * f64(slope) / C0)
+                            pr, pim = f32(np.cos(th)), f32(np.sin(th))
+                            sr = f32(sr + f32(f32(m.real * pr) - f32(m.imag * pim)))
+                            si = f32(si + f32(f32(m.real * pim) + f32(m.imag * pr)))
+                out[(x * len(yl) + y) * len(zl) + z] = complex(sr, si)
+    return out
+
+
+def py_bfs_fifo(all_normals, labeled, seeds, delta_x):
+    """sdf.py:195-222 literally (deque, isValid, np.dot), from given seeds."""
+    rsdf = np.full(labeled.shape, np.nan)
+    hops = np.full(labeled.shape, -1, np.int32)
+    visited = np.zeros_like(labeled)
+    dX = [-1, 0, 1, 0, 0, 0]
+    dY = [0, 1, 0, -1, 0, 0]
+    dZ = [0, 0, 0, 0, 1, -1]
+
+    def valid(x, y, z, lab):
+        if x < 0 or y < 0 or z < 0 or x >= visited.shape[0] or y >= visited.shape[1] \
+                or z >= visited.shape[2]:
+            return False
+        if visited[x, y, z]:
+            return False
+        if np.isnan(all_normals[x, y, z, 0]):
+            return False
+        return labeled[x, y, z] == lab
+
+    for s in seeds:
+        i0, j0, k0 = np.unravel_index(int(s), labeled.shape)
+        lab = labeled[i0, j0, k0]
+        rsdf[i0, j0, k0] = 0
+        if hops[i0, j0, k0] < 0:
+            hops[i0, j0, k0] = 0
+        q = deque([(i0, j0, k0)])
+        visited[i0, j0, k0] = True
+        while q:
+            i, j, k = q.popleft()
+            prev = rsdf[i, j, k]
+            for d in range(6):
+                a, b, c = i + dX[d], j + dY[d], k + dZ[d]
+                vec = np.array([dX[d], dY[d], dZ[d]])
+                if valid(a, b, c, lab):
+                    q.append((a, b, c))
+                    visited[a, b, c] = True
+                    hops[a, b, c] = hops[i, j, k] + 1
+                    rsdf[a, b, c] = prev + np.dot(vec * delta_x, all_normals[a, b, c])
+    return rsdf, hops

@@ -1,0 +1,60 @@
+"""The C-ABI library loads on a host without a GPU and exports every entry
+point include/mmnorm_amd.h declares (no compute call is made)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mmnorm_amd.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|size_t|const char \*)\s*(mmn_\w+)\(", src, re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for want in ("mmn_cuda_image", "mmn_cuda_sn_est", "mmn_bp_forward", "mmn_sn_from_moment",
+                 "mmn_bp_backward", "mmn_bp_backward_workspace_size", "mmn_rsdf_bfs",
+                 "mmn_rsdf_bfs_workspace_size", "mmn_abi_version", "mmn_last_error"):
+        assert want in names
+
+
+def test_library_exports_every_declared_symbol():
+    from mmnorm_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmmnorm_hip.so not built")
+    lib = _lib.load()
+    for name in declared():
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, name
+    assert lib.mmn_abi_version() == 1
+
+
+def test_workspace_queries_and_argument_errors_without_gpu():
+    from mmnorm_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmmnorm_hip.so not built")
+    lib = _lib.load()
+    assert lib.mmn_rsdf_bfs_workspace_size(4, 4, 4) > 4 * 4 * 4 * 12
+    assert lib.mmn_rsdf_bfs_workspace_size(0, 4, 4) == 0
+    assert lib.mmn_bp_backward_workspace_size(16, 32, 100) > 16 * 32 * 8
+    # argument checks run before any HIP call
+    with pytest.raises(ValueError, match="NULL"):
+        _lib.call("mmn_bp_forward", None, None, None, None, None, None, 0, None, None, None,
+                  77e9, 1e6, 1, 1, 1, 1, 1, 1, 0, 2, 32, None)
+    with pytest.raises(ValueError, match="variant"):
+        _lib.call("mmn_bp_forward", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 1e6, 1, 1, 1,
+                  1, 1, 1, 0, 7, 32, None)
+    with pytest.raises(ValueError, match="K must be"):
+        _lib.call("mmn_bp_forward", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 1e6, 1, 1, 1,
+                  1, 1, 1, 0, 2, 12, None)
+    with pytest.raises(ValueError, match="workspace"):
+        _lib.call("mmn_rsdf_bfs", 8, 8, 8, 8, 8, 1, 1e-3, 4, 4, 4, None, 0, None)
+    with pytest.raises(ValueError, match="sizes"):
+        _lib.call("mmn_cuda_image", 8, 8, 8, 8, 8, 8, 8, 1.0, 1.0, 1.0, 0, 1, 1, 1, 1, 1, 0,
+                  0, None)
+    ctypes.get_errno()

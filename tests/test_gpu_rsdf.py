@@ -1,0 +1,106 @@
+"""GPU RSDF breadth-first search: bit-exact against the FIFO restatement."""
+import numpy as np
+import pytest
+import torch
+from scipy import ndimage
+
+from mmnorm_amd import sdf as msdf
+
+pytestmark = pytest.mark.gpu
+
+SIX = ndimage.generate_binary_structure(3, 1)
+
+
+def random_field(shape, density, seed, blobs=True):
+    rng = np.random.default_rng(seed)
+    if blobs:
+        noise = ndimage.gaussian_filter(rng.normal(size=shape), 1.5)
+        mask = noise > np.quantile(noise, 1 - density)
+    else:
+        mask = rng.uniform(size=shape) < density
+    n = rng.normal(size=shape + (3,))
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    n[~mask] = np.nan
+    labeled, nlab = ndimage.label(mask, structure=SIX)
+    labeled = labeled + 1  # background 1, components 2.. as cc3d + 1
+    return n, labeled, nlab
+
+
+def coords_of(shape, pitch):
+    xl = (np.arange(shape[0]) - (shape[0] - 1) / 2) * pitch
+    yl = (np.arange(shape[1]) - (shape[1] - 1) / 2) * pitch
+    zl = (np.arange(shape[2]) - (shape[2] - 1) / 2) * pitch
+    m1, m2, m3 = np.meshgrid(yl, xl, zl)
+    return np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1), xl
+
+
+@pytest.mark.parametrize("shape,density,seed,blobs", [
+    ((12, 10, 9), 0.5, 1, True),
+    ((20, 24, 16), 0.35, 2, True),
+    ((16, 16, 16), 0.6, 3, False),
+    ((40, 33, 27), 0.3, 4, True),
+])
+def test_bfs_bit_exact_vs_fifo(shape, density, seed, blobs, oracle):
+    n, labeled, nlab = random_field(shape, density, seed, blobs)
+    coords, xl = coords_of(shape, 1.5e-3)
+    sizes = ndimage.sum(np.ones(shape), labeled, np.arange(2, nlab + 2))
+    valid = np.arange(2, nlab + 2)[sizes >= 2]
+    seeds = msdf.component_seeds(n, coords, labeled, valid)
+    np.testing.assert_array_equal(seeds, oracle.rsdf_seeds(n, coords, labeled, valid))
+    got, hops = msdf.rsdf_components(n, labeled, seeds, xl[1] - xl[0], return_hops=True)
+    want, whops = oracle.bfs_fifo(n, labeled, seeds, xl[1] - xl[0])
+    assert got.view(np.uint64).tobytes() == want.view(np.uint64).tobytes()
+    np.testing.assert_array_equal(hops, whops)
+
+
+def test_bfs_matches_python_fifo_small(oracle):
+    n, labeled, nlab = random_field((7, 6, 5), 0.55, 9, False)
+    coords, xl = coords_of((7, 6, 5), 1e-3)
+    valid = np.arange(2, nlab + 2)
+    seeds = msdf.component_seeds(n, coords, labeled, valid)
+    got, hops = msdf.rsdf_components(n, labeled, seeds, xl[1] - xl[0], return_hops=True)
+    want, whops = oracle.py_bfs_fifo(n, labeled, seeds, xl[1] - xl[0])
+    assert np.array_equal(got, want, equal_nan=True)
+    np.testing.assert_array_equal(hops, whops)
+
+
+def test_constant_normal_closed_form():
+    """Normals (0,0,1) on a full box: rsdf = dx * (k - k0), hops = Manhattan distance."""
+    shape = (9, 8, 11)
+    n = np.zeros(shape + (3,))
+    n[..., 2] = 1.0
+    labeled = np.full(shape, 2)
+    dx = 1e-3
+    seed = np.ravel_multi_index((4, 3, 5), shape)
+    rsdf, hops = msdf.rsdf_components(n, labeled, [seed], dx, return_hops=True)
+    i, j, k = np.indices(shape)
+    assert np.allclose(rsdf, dx * (k - 5), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(hops, np.abs(i - 4) + np.abs(j - 3) + np.abs(k - 5))
+
+
+def test_drop_in_construct_rsdf_components(oracle):
+    n, labeled, nlab = random_field((18, 15, 12), 0.45, 21, True)
+    coords, xl = coords_of((18, 15, 12), 1e-3)
+    valid = np.arange(2, nlab + 2)[::2]
+    got = msdf._construct_rsdf_components(n, coords, labeled, valid, xl)
+    seeds = oracle.rsdf_seeds(n, coords, labeled, valid)
+    want, _ = oracle.bfs_fifo(n, labeled, seeds, xl[1] - xl[0])
+    assert got.view(np.uint64).tobytes() == want.view(np.uint64).tobytes()
+
+
+def test_bfs_deterministic_and_empty():
+    n, labeled, nlab = random_field((30, 30, 20), 0.4, 5, True)
+    coords, xl = coords_of((30, 30, 20), 1e-3)
+    valid = np.arange(2, nlab + 2)
+    a = msdf._construct_rsdf_components(n, coords, labeled, valid, xl)
+    b = msdf._construct_rsdf_components(n, coords, labeled, valid, xl)
+    assert a.tobytes() == b.tobytes()
+    e = msdf._construct_rsdf_components(n, coords, labeled, [], xl)
+    assert np.all(np.isnan(e))
+
+
+def test_bfs_rejects_bad_seed():
+    n = np.zeros((3, 3, 3, 3))
+    with pytest.raises(ValueError):
+        msdf.rsdf_components(n, np.ones((3, 3, 3), np.int32), [27], 1e-3)
+    assert torch.cuda.is_available()
