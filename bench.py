@@ -80,6 +80,10 @@ def cpu_baseline(sc, target_s=12.0):
                       f"({dt:.1f} s, OMP_NUM_THREADS={cores}, {cpu_model()})"}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +94,9 @@ def main(argv=None):
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="forward only")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="after the timed region, copy a BYTES-sized buffer once (a copy "
+                         "kernel of known size for scaling rocprofv3 FETCH/WRITE_SIZE)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU; the path shards by voxel "
@@ -106,6 +113,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     sc = synth.make_scene(a.config, device=dev)
     t_scene = time.perf_counter() - t0
+    log(f"scene config {a.config} ready in {t_scene:.1f} s")
     T = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x, dtype=dt)).to(dev)
     meas, ant = T(sc.meas, np.complex64), T(sc.ant, np.float32)
     xl, yl, zl = T(sc.x_locs, np.float32), T(sc.y_locs, np.float32), T(sc.z_locs, np.float32)
@@ -135,6 +143,7 @@ def main(argv=None):
 
     wall, kms, (img, mom) = timed(lambda: fwd("rec"), a.steps, a.warmup)
     ms_step = wall / a.steps * 1e3
+    log(f"rec forward: {ms_step:.2f} ms/step")
     k_avg = float(np.mean(kms)) * 1e-3
     achieved = flops_fwd(V, A, F) / k_avg
     rec = {"metric": "G voxel*antenna*freq/s (recurrence fwd+vote, FP32 VALU)",
@@ -164,6 +173,7 @@ def main(argv=None):
         if a.plain_steps > 0:
             pw, pk, _ = timed(lambda: fwd("plain"), a.plain_steps, 1)
             p_s = float(np.mean(pk)) * 1e-3
+            log(f"plain forward: {p_s * 1e3:.2f} ms")
             rec["plain"] = {"value": W / p_s / 1e9, "kernel_ms": p_s * 1e3,
                             "tflops": flops_fwd(V, A, F) / p_s / 1e12,
                             "rec_over_plain": p_s / k_avg}
@@ -176,9 +186,19 @@ def main(argv=None):
         b_s = float(np.mean(bk)) * 1e-3
         rec["backward"] = {"ms": b_s * 1e3, "over_forward": b_s / k_avg,
                            "tflops": V * A * (16.0 * F + 30.0) / b_s / 1e12}
+        log(f"backward: {b_s * 1e3:.2f} ms")
         rec["rsdf_bfs"] = bfs_stage(img, mom, sc, dev)
+        log(f"rsdf bfs: {rec['rsdf_bfs']}")
     rec["scene_s"] = t_scene
+    if a.copy_calibrate > 0:
+        src = torch.empty(a.copy_calibrate // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        torch.cuda.synchronize()
+        dst = src.clone()
+        torch.cuda.synchronize()
+        rec["copy_calibrate_bytes"] = int(dst.numel() * 4)
+        log(f"copy calibration: {dst.numel() * 4} bytes copied")
     if not a.no_cpu:
+        log("cpu baseline ...")
         rec["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(rec), flush=True)
     return 0

@@ -39,6 +39,16 @@ def _check(t, name, dtype, ndim=None, last=None):
     return t
 
 
+def _check_index(idx, name, bound):
+    """int32 index vector with every entry in [0, bound) (kernels index with it)."""
+    _check(idx, name, torch.int32, 1)
+    if idx.numel():
+        lo, hi = torch.aminmax(idx)
+        if int(lo) < 0 or int(hi) >= bound:
+            raise ValueError(f"{name} entries must lie in [0, {bound})")
+    return idx
+
+
 def _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx):
     _lib.require_gpu()
     _check(meas, "meas", torch.complex64, 2)
@@ -58,7 +68,7 @@ def _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx):
         raise ValueError("rx_off must hold at least one RX offset")
     nvox_grid = x_locs.numel() * y_locs.numel() * z_locs.numel()
     if vox_idx is not None:
-        _check(vox_idx, "vox_idx", torch.int32, 1)
+        _check_index(vox_idx, "vox_idx", nvox_grid)
     if nvox_grid >= 2 ** 31:
         raise ValueError("voxel grid too large for int32 voxel indices")
     return rx_off, nvox_grid
@@ -128,7 +138,7 @@ def sn_from_moment(img, moment, vox_idx):
     _lib.require_gpu()
     _check(img, "img", torch.complex64)
     _check(moment, "moment", torch.complex64, 2, 3)
-    _check(vox_idx, "vox_idx", torch.int32, 1)
+    _check_index(vox_idx, "vox_idx", img.numel())
     if moment.shape[0] != vox_idx.numel():
         raise ValueError("moment rows must match vox_idx")
     n = vox_idx.numel()

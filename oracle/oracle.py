@@ -279,3 +279,49 @@ def py_bfs_fifo(all_normals, labeled, seeds, delta_x):
                     hops[a, b, c] = hops[i, j, k] + 1
                     rsdf[a, b, c] = prev + np.dot(vec * delta_x, all_normals[a, b, c])
     return rsdf, hops
+
+
+def py_sn_ref(vidx, img, xl, yl, zl, ant, meas, rxo, wavelength, fft_spacing, is_ti=False):
+    """surface_normal_est.cu:64-149 with numpy float32/float64 scalars, tiny inputs only."""
+    f32, f64 = np.float32, np.float64
+    xl, yl, zl, ant, rxo = map(_f32, (xl, yl, zl, ant, rxo))
+    meas, img = _c64(meas), _c64(np.ravel(img))
+    rows, S = meas.shape
+    R = len(rxo)
+    wavelength, fs = f32(wavelength), f32(fft_spacing)
+    ny, nz = len(yl), len(zl)
+    out = np.zeros((len(vidx), 3), np.float32)
+    pi = f64(math.pi)
+    for t, lin in enumerate(vidx):
+        x, r = divmod(int(lin), ny * nz)
+        y, z = divmod(r, nz)
+        vx, vy, vz = xl[x], yl[y], zl[z]
+        ir, ii = f32(img[lin].real), f32(img[lin].imag)
+        mag = np.sqrt(f32(f32(ir * ir) + f32(ii * ii)), dtype=f32)
+        sn = [f32(0), f32(0), f32(0)]
+        for i in range(rows):
+            p, o = ant[i], rxo[i % R]
+            dx, dy, dz = f32(vx - p[0]), f32(vy - p[1]), f32(vz - p[2])
+            fwd = np.sqrt(f32(f32(dx * dx) + f32(dy * dy)) + f32(dz * dz), dtype=f32)
+            bx, by, bz = f32(dx - o[0]), f32(dy - o[1]), f32(dz - o[2])
+            back = np.sqrt(f32(f32(bx * bx) + f32(by * by)) + f32(bz * bz), dtype=f32)
+            d = f32(fwd + back)
+            if is_ti:
+                d = f32(f64(d) + 0.15)
+            if d < 0 or d > f32(fs * f32(S)):
+                continue
+            b = int(np.floor(f32(f32(d / fs) / f32(2))))
+            m = meas[i, b]
+            th = f64(-2.0) * pi * f64(d) / f64(wavelength)
+            pr, pim = f32(np.cos(th)), f32(np.sin(th))
+            cr = f32(f32(m.real * pr) - f32(m.imag * pim))
+            ci = f32(f32(m.real * pim) + f32(m.imag * pr))
+            vec = (f32(f32(p[0] + f32(o[0] / f32(2))) - vx),
+                   f32(f32(p[1] + f32(o[1] / f32(2))) - vy),
+                   f32(f32(p[2] + f32(o[2] / f32(2))) - vz))
+            w = f32(f32(cr * ir) + f32(ci * ii))
+            w = f32(w / mag)
+            for k in range(3):
+                sn[k] = f32(sn[k] + f32(vec[k] * w))
+        out[t] = sn
+    return out

@@ -1,0 +1,235 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference has no tests or golden vectors, so each oracle function is
+pinned by at least one check that a plausible mistake (dropped term, wrong
+sign, wrong index, transposed operand) would fail: closed forms, invariants,
+independent implementations (C loops vs pure-Python loops vs NumPy
+broadcasting) and the committed fixtures in tests/golden/.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from mmnorm_amd import synth
+
+C0 = 2.99792458e8
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(os.path.join(GOLD, "bp_tiny.npz"))
+
+
+@pytest.fixture(scope="module")
+def gbfs():
+    return np.load(os.path.join(GOLD, "bfs_tiny.npz"))
+
+
+# ------------------------------------------------------------ backprojection --
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_c_ref_equals_python_ref_golden(oracle, gold, case):
+    g = lambda k: gold[f"{case}_{k}"]
+    got = oracle.bp_ref(g("xl"), g("yl"), g("zl"), g("ant"), g("meas"), g("rxo"), 5861160.0,
+                        float(g("lam")), is_ti=bool(g("ti")))
+    np.testing.assert_array_equal(got, g("full"))
+    got = oracle.bp_ref(g("ixl"), g("iyl"), g("izl"), g("iant"), g("spec"), g("rxo"),
+                        5861160.0, float(g("lam")), float(g("fs")), is_ti=bool(g("ti")),
+                        interp=True)
+    np.testing.assert_array_equal(got, g("interp"))
+    assert np.count_nonzero(got) == got.size
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_c_vote_equals_python_vote_golden(oracle, gold, case):
+    g = lambda k: gold[f"{case}_{k}"]
+    got = oracle.sn_ref(g("vidx"), g("interp"), g("ixl"), g("iyl"), g("izl"), g("iant"),
+                        g("spec"), g("rxo"), float(g("lam")), float(g("fs")),
+                        is_ti=bool(g("ti")))
+    np.testing.assert_array_equal(got, g("sn"))
+    assert len(g("vidx")) > 3
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_f64_oracle_equals_brute_force_golden(oracle, gold, case):
+    g = lambda k: gold[f"{case}_{k}"]
+    f0 = C0 / float(np.float32(g("lam")))
+    img, mom = oracle.bp_f64(g("xl"), g("yl"), g("zl"), g("ant"), g("meas"), g("rxo"), f0,
+                             float(np.float32(5861160.0)), is_ti=bool(g("ti")), moment=True)
+    np.testing.assert_allclose(img, g("f64"), rtol=0, atol=1e-9 * np.abs(g("f64")).max())
+    np.testing.assert_allclose(mom, g("mom64"), rtol=0, atol=1e-9 * np.abs(g("mom64")).max())
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_reference_arithmetic_is_the_same_sum(oracle, gold, case):
+    """Reference-precision restatement vs FP64: only the FP32 floor apart."""
+    g = lambda k: gold[f"{case}_{k}"]
+    full = g("full").astype(np.complex128)
+    assert np.linalg.norm(full - g("f64")) / np.linalg.norm(g("f64")) < 3e-4
+
+
+def test_point_scatterer_focuses_exactly(oracle):
+    """Noise-free scatterer at a voxel centre: I = rho*A*F there, the image max there.
+    A wrong sign, distance or frequency index would defocus it."""
+    cfg = synth.CONFIGS[1]
+    ant = synth.aperture(10, 2e-3, 0.3).astype(np.float32).astype(np.float64)
+    xl, yl, zl = synth.axis(6, 2e-3), synth.axis(5, 2e-3), synth.axis(4, 2e-3)
+    f32 = lambda a: np.float64(np.float32(a))
+    pos = np.array([[f32(xl[4]), f32(yl[1]), f32(zl[2])]])
+    rho = np.array([1.3 + 0.2j])
+    freqs = cfg.f0 + cfg.df * np.arange(cfg.n_freq)
+    s = synth.synthesize(ant, pos, rho, freqs)
+    img = oracle.bp_f64(xl, yl, zl, ant, s.astype(np.complex64), np.zeros((1, 3)), cfg.f0,
+                        cfg.df).reshape(6, 5, 4)
+    want = rho[0] * len(ant) * cfg.n_freq
+    # s was rounded to complex64: allow that rounding only
+    assert abs(img[4, 1, 2] - want) < 1e-6 * abs(want)
+    assert np.unravel_index(np.argmax(np.abs(img)), img.shape) == (4, 1, 2)
+    conj = oracle.bp_f64(xl, yl, zl, ant, np.conj(s).astype(np.complex64), np.zeros((1, 3)),
+                         cfg.f0, cfg.df).reshape(6, 5, 4)
+    assert abs(conj[4, 1, 2]) < 0.2 * abs(want)
+
+
+def test_translation_invariance(oracle, gold):
+    """Antennas and voxels moved together (on a 2^-12 m lattice, so every coordinate
+    and difference stays exact in FP32): the FP64 image is unchanged."""
+    g = lambda k: gold[f"a_{k}"]
+    f0 = C0 / float(np.float32(g("lam")))
+    q = lambda a: np.round(np.asarray(a) * 4096) / 4096
+    xl, yl, zl, ant = q(g("xl")), q(g("yl")), q(g("zl")), q(g("ant"))
+    shift = np.array([0.0625, -0.125, 0.25])
+    a = oracle.bp_f64(xl, yl, zl, ant, g("meas"), g("rxo"), f0, 5.8e6)
+    b = oracle.bp_f64(xl + shift[0], yl + shift[1], zl + shift[2], ant + shift, g("meas"),
+                      g("rxo"), f0, 5.8e6)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * np.abs(a).max())
+
+
+def test_vote_identity_row_by_row(oracle, gold):
+    """Re(conj(I) M)/|I| equals sum_i vec_i Re(c_i conj(I))/|I| computed per row."""
+    g = lambda k: gold[f"b_{k}"]
+    f0 = C0 / float(np.float32(g("lam")))
+    img, mom = oracle.bp_f64(g("xl"), g("yl"), g("zl"), g("ant"), g("meas"), g("rxo"), f0,
+                             5.8e6, is_ti=True, moment=True)
+    per_row = np.zeros((img.size, 3))
+    for i in range(len(g("ant"))):
+        c = oracle.bp_f64(g("xl"), g("yl"), g("zl"), g("ant")[i:i + 1], g("meas")[i:i + 1],
+                          g("rxo")[i % 4:i % 4 + 1], f0, 5.8e6, is_ti=True)
+        X, Y, Z = np.meshgrid(*(np.float32(g(k)).astype(np.float64) for k in ("xl", "yl", "zl")),
+                              indexing="ij")
+        v = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+        p = np.float32(g("ant")[i]).astype(np.float64)
+        o = np.float32(g("rxo")[i % 4]).astype(np.float64)
+        w = (c * np.conj(img)).real / np.abs(img)
+        per_row += ((p + o / 2)[None] - v) * w[:, None]
+    fused = (np.conj(img)[:, None] * mom).real / np.abs(img)[:, None]
+    np.testing.assert_allclose(fused, per_row, rtol=1e-9, atol=1e-12 * np.abs(per_row).max())
+
+
+def test_cpu_imaging_restatement_is_the_same_sum(oracle, gold):
+    """imaging.cpp restatement (single RX, no TI) equals the FP64 sum / n_tx."""
+    g = lambda k: gold[f"a_{k}"]
+    ant_tx = g("ant")[::4]
+    meas = g("meas")[::4]
+    lam = float(np.float32(g("lam")))
+    got = oracle.bp_cpu_imaging(g("xl"), g("yl"), g("zl"), ant_tx, meas, np.zeros((1, 3)),
+                                5861160.0, lam)
+    want = oracle.bp_f64(g("xl"), g("yl"), g("zl"), ant_tx, meas, np.zeros((1, 3)), C0 / lam,
+                         5861160.0) / len(ant_tx)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 5e-4
+
+
+# -------------------------------------------------------------------- BFS --
+def test_bfs_c_equals_python_golden(oracle, gbfs):
+    rsdf, hops = oracle.bfs_fifo(gbfs["normals"], gbfs["labels"], gbfs["seeds"],
+                                 float(gbfs["dx"]))
+    assert rsdf.tobytes() == gbfs["rsdf"].tobytes()
+    np.testing.assert_array_equal(hops, gbfs["hops"])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_bfs_c_equals_python_random(oracle, seed):
+    rng = np.random.default_rng(seed)
+    shape = (8, 7, 6)
+    n = rng.normal(size=shape + (3,))
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    mask = rng.uniform(size=shape) < 0.65
+    n[~mask] = np.nan
+    lab, nl = ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
+    lab = lab + 1
+    X, Y, Z = np.meshgrid(*(np.arange(s) * 1e-3 for s in shape), indexing="ij")
+    coords = np.stack([X, Y, Z], -1)
+    seeds = oracle.rsdf_seeds(n, coords, lab, np.arange(2, nl + 2))
+    a = oracle.bfs_fifo(n, lab, seeds, 1e-3)
+    b = oracle.py_bfs_fifo(n, lab, seeds, 1e-3)
+    assert np.array_equal(a[0], b[0], equal_nan=True)
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_bfs_constant_normal_closed_form(oracle):
+    shape = (6, 5, 7)
+    n = np.zeros(shape + (3,))
+    n[..., 1] = -1.0
+    lab = np.full(shape, 2)
+    seed = np.ravel_multi_index((2, 3, 1), shape)
+    rsdf, hops = oracle.bfs_fifo(n, lab, [seed], 2e-3)
+    i, j, k = np.indices(shape)
+    np.testing.assert_allclose(rsdf, -2e-3 * (j - 3), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(hops, np.abs(i - 2) + np.abs(j - 3) + np.abs(k - 1))
+
+
+def test_bfs_hops_are_shortest_paths(oracle):
+    """Hops equal the level sets of a mask-constrained wavefront (independent)."""
+    rng = np.random.default_rng(9)
+    shape = (12, 11, 10)
+    mask = ndimage.gaussian_filter(rng.normal(size=shape), 1.2) > -0.2
+    lab, nl = ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
+    big = np.argmax(np.bincount(lab.ravel())[1:]) + 1
+    comp = lab == big
+    n = np.where(comp[..., None], 1 / np.sqrt(3), np.nan) * np.ones(shape + (3,))
+    seed = np.flatnonzero(comp.ravel())[len(np.flatnonzero(comp.ravel())) // 2]
+    _, hops = oracle.bfs_fifo(n, np.where(comp, 2, 1), [seed], 1e-3)
+    front = np.zeros(shape, bool)
+    front.flat[seed] = True
+    seen = front.copy()
+    level = 0
+    st = ndimage.generate_binary_structure(3, 1)
+    while front.any():
+        assert np.all(hops[front] == level)
+        nxt = ndimage.binary_dilation(front, st) & comp & ~seen
+        seen |= nxt
+        front, level = nxt, level + 1
+    assert np.all(hops[~seen] == -1)
+
+
+def test_bfs_label_renumbering_invariant(oracle):
+    rng = np.random.default_rng(4)
+    shape = (9, 9, 6)
+    n = rng.normal(size=shape + (3,))
+    mask = rng.uniform(size=shape) < 0.7
+    n[~mask] = np.nan
+    lab, nl = ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
+    lab = lab + 1
+    X, Y, Z = np.meshgrid(*(np.arange(s) * 1e-3 for s in shape), indexing="ij")
+    coords = np.stack([X, Y, Z], -1)
+    valid = np.arange(2, nl + 2)
+    seeds = oracle.rsdf_seeds(n, coords, lab, valid)
+    a, _ = oracle.bfs_fifo(n, lab, seeds, 1e-3)
+    perm = np.concatenate([[0, 1], rng.permutation(np.arange(2, nl + 2)) + 100])
+    lab2 = perm[lab]
+    seeds2 = oracle.rsdf_seeds(n, coords, lab2, perm[valid][::-1])
+    b, _ = oracle.bfs_fifo(n, lab2, seeds2, 1e-3)
+    assert a.tobytes() == b.tobytes()
+
+
+def test_seed_ties_take_the_first_voxel(oracle):
+    """A 2x2x2 cube: all voxels tie with the mean; argmin keeps the first (C order)."""
+    shape = (4, 4, 4)
+    n = np.full(shape + (3,), np.nan)
+    n[1:3, 1:3, 1:3] = 0.5
+    lab = np.ones(shape, int)
+    lab[1:3, 1:3, 1:3] = 2
+    X, Y, Z = np.meshgrid(*(np.arange(4) * 0.25 for _ in range(3)), indexing="ij")
+    seeds = oracle.rsdf_seeds(n, np.stack([X, Y, Z], -1), lab, [2])
+    assert seeds[0] == np.ravel_multi_index((1, 1, 1), shape)
