@@ -97,7 +97,7 @@ int mmn_cuda_sn_est(float *normals, const int32_t *vox_idx, int64_t n_valid,
  * where c_iv is row i's contribution to img[v]; the un-normalised vote of
  * surface_normal_est.cu:128-144 is then Re(conj(I_v) moment[v]) / |I_v|
  * (mmn_sn_from_moment).  variant: MMN_BP_PLAIN or MMN_BP_REC; K (REC only)
- * is the re-anchor interval in samples, one of 8, 16, 32, 64 (or 0 = 32).
+ * is the re-anchor interval in samples, one of 8, 16, 32, 64 (or 0 = 64).
  * No reference counterpart for the fused moment or the variants; the
  * reference computes the vote only for the interpolated branch
  * (py_sn_est_gpu.py:404-405).
@@ -130,7 +130,8 @@ int mmn_sn_from_moment(float *normals, const float *img, const float *moment,
  * with Q_iv = sum_j (2 pi f_j / c) meas[i][j] exp(-j 2 pi f_j d_iv / c).
  * Either output may be NULL (not computed).  No reference counterpart:
  * the reference computes no gradients.  Deterministic: partial sums go to
- * the workspace and are reduced in a fixed order.
+ * the workspace and are reduced in a fixed order.  K (8, 16, 32 or 48;
+ * 0 = 32) is the sample block of one lane and its re-anchor interval.
  */
 size_t mmn_bp_backward_workspace_size(int num_rows, int samples, int64_t n_vox);
 int mmn_bp_backward(float *grad_meas, float *grad_ant, const float *g,

@@ -18,6 +18,8 @@
 // sub-range and writes its partial sums to the workspace; a second kernel
 // reduces the partials in a fixed order (FP64), so results are bitwise
 // reproducible.
+#include <stdlib.h>
+
 #include "mmn_internal.h"
 
 namespace {
@@ -57,6 +59,11 @@ BwdPlan make_plan(int A, int F, int64_t nvox, int K) {
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return (v && *v) ? atoi(v) : dflt;
+}
 
 struct BwdWs {
   VoxRec *rec;    // [nvox]
@@ -100,7 +107,60 @@ __global__ __launch_bounds__(kBlock) void k_pack(
   rec[t] = r;
 }
 
-template <int K>
+// Per (lane row a, voxel v) state of one sample block: the conjugate phasors
+// z_t = g_v exp(+j 2 pi f_t d_av / c) for t = 0..K-1 satisfy, because the
+// step exp(+j w) has modulus one, the real-coefficient recurrence
+//     z_{t+1} = 2 cos(w) z_t - z_{t-1}
+// (one FMA per component and step).  z_0 comes from the FP64-reduced anchor
+// and z_1 = z_0 exp(+j w).  Rounding in this recurrence accumulates at most
+// linearly, amplified by 1/sin(w); for the bands and ranges here
+// (w = 2 pi df d / c between 0.05 and 1 rad) that stays below 1e-5 over a
+// 32-sample block.
+struct VoxStart {
+  float z0r, z0i, z1r, z1i, c2;  // c2 = 2 cos(w)
+  float gx, gy, gz;              // grad_p d
+};
+
+__device__ __forceinline__ VoxStart vox_start(const VoxRec &r, float px, float py,
+                                              float pz, float ox, float oy, float oz,
+                                              bool mono, int is_ti, double fc,
+                                              double dfc, float weight) {
+  VoxStart v;
+  double d = roundtrip_dist64(r.vx, r.vy, r.vz, px, py, pz, ox, oy, oz, mono);
+  if (is_ti) d += 0.15;
+  float ar, ai, rr, ri;
+  phasor_hw(d * fc, ar, ai);  // exp(-j ...); conjugated below
+  phasor_hw(d * dfc, rr, ri);
+  ai = -ai;
+  ri = -ri;
+  const float gr = r.gr * weight, gi = r.gi * weight;
+  v.z0r = gr * ar - gi * ai;
+  v.z0i = gr * ai + gi * ar;
+  v.z1r = v.z0r * rr - v.z0i * ri;
+  v.z1i = v.z0r * ri + v.z0i * rr;
+  v.c2 = rr + rr;
+  const float ax = r.vx - px, ay = r.vy - py, az = r.vz - pz;
+  const float n1 = rsqrtf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
+  float gx = ax * n1, gy = ay * n1, gz = az * n1;
+  if (mono) {
+    gx += gx;
+    gy += gy;
+    gz += gz;
+  } else {
+    const float bx = ax - ox, by = ay - oy, bz = az - oz;
+    const float n2 = rsqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
+    gx = fmaf(bx, n2, gx);
+    gy = fmaf(by, n2, gy);
+    gz = fmaf(bz, n2, gz);
+  }
+  // d(d)/dp = -(unit vectors): the sign is applied with the position term.
+  v.gx = gx;
+  v.gy = gy;
+  v.gz = gz;
+  return v;
+}
+
+template <int K, int VV>
 __global__ __launch_bounds__(kBlock) void k_bwd(
     float2 *__restrict__ gs, float *__restrict__ gp,
     const VoxRec *__restrict__ rec, int64_t nvox, int64_t chunk,
@@ -133,51 +193,73 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
     accr[t] = acci[t] = 0.0f;
   }
   float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
-  const double fb_c = fma((double)f_lo, dfc, f0c);
+  const double fc = fma((double)f_lo, dfc, f0c);
 
   const int64_t v0 = (int64_t)sub * chunk;
   const int64_t v1 = min(nvox, v0 + chunk);
-  for (int64_t v = v0; v < v1; ++v) {
-    const VoxRec r = rec[v];
-    double d = roundtrip_dist64(r.vx, r.vy, r.vz, px, py, pz, ox, oy, oz, mono);
-    if (is_ti) d += 0.15;
-    // conj phasors: exp(+j 2 pi t)
-    float ar, ai, rr, ri;
-    phasor_rev64(d * fb_c, ar, ai);
-    ai = -ai;
-    phasor_rev32((float)(d * dfc), rr, ri);
-    ri = -ri;
-    float zr = r.gr * ar - r.gi * ai;
-    float zi = r.gr * ai + r.gi * ar;
-    float tim = 0.0f;
+  // VV voxels per iteration: VV independent recurrences per lane.  A short
+  // last group repeats its final voxel with zero weight.
+  // Voxel records are read with vector loads (every lane the same address),
+  // whose in-order completion lets the next group's records be requested one
+  // iteration ahead: the wait for them then falls after a full group of work.
+  // `zero` is a lane value the compiler cannot see through, which keeps these
+  // uniform loads on the vector path instead of scalar loads (whose waits
+  // cover every outstanding scalar load and so cannot be overlapped).
+  int zero;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+  const VoxRec *__restrict__ rv = rec + zero;
+  VoxRec nxt[VV];
+  if (v0 < v1) {
+#pragma unroll
+    for (int u = 0; u < VV; ++u) nxt[u] = rv[min(v0 + u, v1 - 1)];
+  }
+  for (int64_t v = v0; v < v1; v += VV) {
+    VoxRec cur[VV];
+#pragma unroll
+    for (int u = 0; u < VV; ++u) cur[u] = nxt[u];
+    if (v + VV < v1) {
+#pragma unroll
+      for (int u = 0; u < VV; ++u) nxt[u] = rv[min(v + VV + u, v1 - 1)];
+    }
+    VoxStart p[VV];
+    float w[VV];
+#pragma unroll
+    for (int u = 0; u < VV; ++u) {
+      w[u] = (v + u < v1) ? 1.0f : 0.0f;
+      p[u] = vox_start(cur[u], px, py, pz, ox, oy, oz, mono, is_ti, fc, dfc, w[u]);
+    }
+    float r0[VV], i0[VV], r1[VV], i1[VV], t1[VV], t2[VV];
+#pragma unroll
+    for (int u = 0; u < VV; ++u) {
+      r0[u] = p[u].z0r; i0[u] = p[u].z0i; r1[u] = p[u].z1r; i1[u] = p[u].z1i;
+      t1[u] = t2[u] = 0.0f;
+    }
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-      accr[t] += zr;
-      acci[t] += zi;
-      tim = fmaf(spi[t], zr, fmaf(-spr[t], zi, tim));
-      const float nzr = fmaf(zr, rr, -zi * ri);
-      const float nzi = fmaf(zr, ri, zi * rr);
-      zr = nzr;
-      zi = nzi;
+      float sr = r0[0], si = i0[0];
+#pragma unroll
+      for (int u = 1; u < VV; ++u) {
+        sr += r0[u];
+        si += i0[u];
+      }
+      accr[t] += sr;
+      acci[t] += si;
+#pragma unroll
+      for (int u = 0; u < VV; ++u) {
+        t1[u] = fmaf(spi[t], r0[u], t1[u]);
+        t2[u] = fmaf(spr[t], i0[u], t2[u]);
+        const float r2 = fmaf(p[u].c2, r1[u], -r0[u]);
+        const float i2 = fmaf(p[u].c2, i1[u], -i0[u]);
+        r0[u] = r1[u]; i0[u] = i1[u]; r1[u] = r2; i1[u] = i2;
+      }
     }
-    // grad_p d = -(v - p)/|v - p| - (v - p - o)/|v - p - o|
-    const float ax = r.vx - px, ay = r.vy - py, az = r.vz - pz;
-    const float n1 = rsqrtf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
-    float gx = ax * n1, gy = ay * n1, gz = az * n1;
-    if (mono) {
-      gx += gx;
-      gy += gy;
-      gz += gz;
-    } else {
-      const float bx = ax - ox, by = ay - oy, bz = az - oz;
-      const float n2 = rsqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
-      gx = fmaf(bx, n2, gx);
-      gy = fmaf(by, n2, gy);
-      gz = fmaf(bz, n2, gz);
+#pragma unroll
+    for (int u = 0; u < VV; ++u) {
+      const float tt = t1[u] - t2[u];
+      gpx = fmaf(-tt, p[u].gx, gpx);
+      gpy = fmaf(-tt, p[u].gy, gpy);
+      gpz = fmaf(-tt, p[u].gz, gpz);
     }
-    gpx = fmaf(-tim, gx, gpx);
-    gpy = fmaf(-tim, gy, gpy);
-    gpz = fmaf(-tim, gz, gpz);
   }
   if (live_row) {
     float2 *__restrict__ out = gs + ((size_t)sub * A + a) * F + f_lo;
@@ -240,8 +322,8 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
                   samples > 0,
               MMN_ERR_ARG, "mmn_bp_backward: sizes must be positive");
   if (K == 0) K = 32;
-  MMN_REQUIRE(K == 8 || K == 16 || K == 32, MMN_ERR_ARG,
-              "mmn_bp_backward: K must be 8, 16 or 32 (got %d)", K);
+  MMN_REQUIRE(K == 8 || K == 16 || K == 32 || K == 48, MMN_ERR_ARG,
+              "mmn_bp_backward: K must be 8, 16, 32 or 48 (got %d)", K);
   const int64_t nvox = vox_idx ? n_vox : (int64_t)nx * ny * nz;
   MMN_REQUIRE(nvox >= 0, MMN_ERR_ARG, "mmn_bp_backward: n_vox < 0");
   hipStream_t st = (hipStream_t)stream;
@@ -271,14 +353,16 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   const float k0 = (float)(two_pi * f0 * inv_c);
   const float dk = (float)(two_pi * df * inv_c);
   const dim3 grid((A + kWave - 1) / kWave, p.nfb, p.sub / kWavesPerBlock);
-#define MMN_BWD_LAUNCH(KK)                                                    \
-  hipLaunchKernelGGL(k_bwd<KK>, grid, dim3(kBlock), 0, st, w.gs, w.gp, w.rec, \
+#define MMN_BWD_LAUNCH(KK, VV)                                                \
+  hipLaunchKernelGGL((k_bwd<KK, VV>), grid, dim3(kBlock), 0, st, w.gs, w.gp, w.rec, \
                      nvox, p.chunk, ant, (const float2 *)meas, rx_off,        \
-                     f0 * inv_c, df * inv_c, k0, dk, A, num_rx, F,            \
-                     is_ti_radar, p.nfbmax)
-  if (K == 8) MMN_BWD_LAUNCH(8);
-  else if (K == 16) MMN_BWD_LAUNCH(16);
-  else MMN_BWD_LAUNCH(32);
+                     f0 * inv_c, df * inv_c, k0, dk, A, num_rx,               \
+                     F, is_ti_radar, p.nfbmax)
+  const int vv = env_int("MMN_BWD_VV", 2);
+  if (K == 8) MMN_BWD_LAUNCH(8, 2);
+  else if (K == 16) { if (vv == 1) MMN_BWD_LAUNCH(16, 1); else MMN_BWD_LAUNCH(16, 2); }
+  else if (K == 48) { if (vv == 2) MMN_BWD_LAUNCH(48, 2); else MMN_BWD_LAUNCH(48, 1); }
+  else { if (vv == 1) MMN_BWD_LAUNCH(32, 1); else MMN_BWD_LAUNCH(32, 2); }
 #undef MMN_BWD_LAUNCH
   if (grad_meas)
     hipLaunchKernelGGL(k_reduce_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)),

@@ -209,7 +209,9 @@ __global__ __launch_bounds__(kBlock) void k_bp_plain(
 //   r = exp(-j 2pi df d/c),
 // so the row's block sum  sum_j s_j phi_j = A_b * h_b  with the polynomial
 // h_b = sum_t s_{bK+t} r^t evaluated by Horner from the top coefficient down.
-// A_b is recomputed from d in FP64 at every block (the re-anchor).
+// A_b is recomputed from d in FP64 at every block (the re-anchor).  NB blocks
+// are evaluated side by side so that each lane carries 2*VPT*NB independent
+// FMA chains (a Horner step is two dependent FMAs per component).
 // --------------------------------------------------------------------------
 template <int VPT>
 __device__ __forceinline__ void horner_step(float sr, float si,
@@ -226,7 +228,78 @@ __device__ __forceinline__ void horner_step(float sr, float si,
   }
 }
 
-template <int K, int VPT>
+// cr/ci += A(d * fb) * h, with fb the block's first frequency over c
+// (revolutions per metre): the FP64 re-anchor.
+template <int VPT>
+__device__ __forceinline__ void anchor_add(const double (&d)[VPT], double fb,
+                                           const float (&hr)[VPT],
+                                           const float (&hi)[VPT],
+                                           float (&cr)[VPT], float (&ci)[VPT]) {
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    float ar, ai;
+    phasor_hw(d[k] * fb, ar, ai);
+    cr[k] = fmaf(ar, hr[k], fmaf(-ai, hi[k], cr[k]));
+    ci[k] = fmaf(ar, hi[k], fmaf(ai, hr[k], ci[k]));
+  }
+}
+
+// One full block of K samples starting at sample b0.  S = 1: plain Horner in
+// r.  S = 2: the even and odd coefficients as two Horner chains in r^2,
+// h = E(r^2) + r O(r^2) -- the same 4 FMAs per sample with twice the
+// independent chains per lane.
+template <int K, int VPT, int S>
+__device__ __forceinline__ void rec_block(const float2 *__restrict__ row, int b0,
+                                          double f0c, double dfc,
+                                          const double (&d)[VPT],
+                                          const float (&rr)[VPT],
+                                          const float (&ri)[VPT],
+                                          const float (&r2r)[VPT],
+                                          const float (&r2i)[VPT],
+                                          float (&cr)[VPT], float (&ci)[VPT]) {
+  const float2 *__restrict__ blk = row + b0;
+  if constexpr (S == 1) {
+    float hr[VPT], hi[VPT];
+    const float2 top = blk[K - 1];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      hr[k] = top.x;
+      hi[k] = top.y;
+    }
+#pragma unroll
+    for (int t = K - 2; t >= 0; --t) {
+      const float2 s = blk[t];
+      horner_step<VPT>(s.x, s.y, rr, ri, hr, hi);
+    }
+    anchor_add<VPT>(d, fma((double)b0, dfc, f0c), hr, hi, cr, ci);
+  } else {
+    static_assert(K % 2 == 0, "split Horner needs an even block");
+    float er[VPT], ei[VPT], orr[VPT], oi[VPT];
+    const float2 te = blk[K - 2], to = blk[K - 1];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      er[k] = te.x;
+      ei[k] = te.y;
+      orr[k] = to.x;
+      oi[k] = to.y;
+    }
+#pragma unroll
+    for (int t = K / 2 - 2; t >= 0; --t) {
+      const float2 se = blk[2 * t], so = blk[2 * t + 1];
+      horner_step<VPT>(se.x, se.y, r2r, r2i, er, ei);
+      horner_step<VPT>(so.x, so.y, r2r, r2i, orr, oi);
+    }
+    float hr[VPT], hi[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      hr[k] = fmaf(rr[k], orr[k], fmaf(-ri[k], oi[k], er[k]));
+      hi[k] = fmaf(rr[k], oi[k], fmaf(ri[k], orr[k], ei[k]));
+    }
+    anchor_add<VPT>(d, fma((double)b0, dfc, f0c), hr, hi, cr, ci);
+  }
+}
+
+template <int K, int VPT, int S>
 __global__ __launch_bounds__(kBlock) void k_bp_rec(
     float2 *__restrict__ img, float2 *__restrict__ mom,
     const float *__restrict__ xl, const float *__restrict__ yl,
@@ -256,40 +329,19 @@ __global__ __launch_bounds__(kBlock) void k_bp_rec(
     const float2 *__restrict__ row = meas + (size_t)i * F;
 
     double d[VPT];
-    float rr[VPT], ri[VPT], cr[VPT], ci[VPT];
+    float rr[VPT], ri[VPT], r2r[VPT], r2i[VPT], cr[VPT], ci[VPT];
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       d[k] = roundtrip_dist64(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono);
       if (is_ti) d[k] += 0.15;
-      phasor_rev32((float)(d[k] * dfc), rr[k], ri[k]);
+      phasor_quarter(4.0 * d[k] * dfc, rr[k], ri[k]);
+      r2r[k] = fmaf(rr[k], rr[k], -ri[k] * ri[k]);
+      r2i[k] = 2.0f * rr[k] * ri[k];
       cr[k] = ci[k] = 0.0f;
     }
 
-    for (int b = 0; b < nb; ++b) {
-      const float2 *__restrict__ blk = row + b * K;
-      float hr[VPT], hi[VPT];
-      {
-        const float2 top = blk[K - 1];
-#pragma unroll
-        for (int k = 0; k < VPT; ++k) {
-          hr[k] = top.x;
-          hi[k] = top.y;
-        }
-      }
-#pragma unroll
-      for (int t = K - 2; t >= 0; --t) {
-        const float2 s = blk[t];
-        horner_step<VPT>(s.x, s.y, rr, ri, hr, hi);
-      }
-      const double fb = fma((double)(b * K), dfc, f0c);
-#pragma unroll
-      for (int k = 0; k < VPT; ++k) {
-        float ar, ai;
-        phasor_rev64(d[k] * fb, ar, ai);
-        cr[k] = fmaf(ar, hr[k], fmaf(-ai, hi[k], cr[k]));
-        ci[k] = fmaf(ar, hi[k], fmaf(ai, hr[k], ci[k]));
-      }
-    }
+    for (int b = 0; b < nb; ++b)
+      rec_block<K, VPT, S>(row, b * K, f0c, dfc, d, rr, ri, r2r, r2i, cr, ci);
     if (tail) {
       const float2 *__restrict__ blk = row + nb * K;
       float hr[VPT], hi[VPT];
@@ -305,14 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_rec(
         const float2 s = blk[t];
         horner_step<VPT>(s.x, s.y, rr, ri, hr, hi);
       }
-      const double fb = fma((double)(nb * K), dfc, f0c);
-#pragma unroll
-      for (int k = 0; k < VPT; ++k) {
-        float ar, ai;
-        phasor_rev64(d[k] * fb, ar, ai);
-        cr[k] = fmaf(ar, hr[k], fmaf(-ai, hi[k], cr[k]));
-        ci[k] = fmaf(ar, hi[k], fmaf(ai, hr[k], ci[k]));
-      }
+      anchor_add<VPT>(d, fma((double)(nb * K), dfc, f0c), hr, hi, cr, ci);
     }
 
     const float vax = px + ox / 2.0f, vay = py + oy / 2.0f, vaz = pz + oz / 2.0f;
@@ -329,31 +374,50 @@ int env_int(const char *name, int dflt) {
   return (s && *s) ? atoi(s) : dflt;
 }
 
-template <int K, int VPT>
-void launch_rec(hipStream_t st, float2 *img, float2 *mom, const float *xl,
-                const float *yl, const float *zl, const int32_t *vidx,
-                int64_t nvox, const float *ant, const float2 *meas,
-                const float *rxo, double f0c, double dfc, int ny, int nz,
-                int rows, int R, int F, int is_ti) {
-  hipLaunchKernelGGL((k_bp_rec<K, VPT>), dim3(mmn_blocks(nvox, kBlock * VPT)),
-                     dim3(kBlock), 0, st, img, mom, xl, yl, zl, vidx, nvox,
-                     ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti);
+struct RecArgs {
+  float2 *img, *mom;
+  const float *xl, *yl, *zl;
+  const int32_t *vidx;
+  int64_t nvox;
+  const float *ant;
+  const float2 *meas;
+  const float *rxo;
+  double f0c, dfc;
+  int ny, nz, rows, R, F, is_ti;
+};
+
+template <int K, int VPT, int S>
+void launch_rec(hipStream_t st, const RecArgs &a) {
+  hipLaunchKernelGGL((k_bp_rec<K, VPT, S>), dim3(mmn_blocks(a.nvox, kBlock * VPT)),
+                     dim3(kBlock), 0, st, a.img, a.mom, a.xl, a.yl, a.zl, a.vidx,
+                     a.nvox, a.ant, a.meas, a.rxo, a.f0c, a.dfc, a.ny, a.nz,
+                     a.rows, a.R, a.F, a.is_ti);
 }
 
-template <int VPT>
-int dispatch_rec(int K, hipStream_t st, float2 *img, float2 *mom,
-                 const float *xl, const float *yl, const float *zl,
-                 const int32_t *vidx, int64_t nvox, const float *ant,
-                 const float2 *meas, const float *rxo, double f0c, double dfc,
-                 int ny, int nz, int rows, int R, int F, int is_ti) {
+template <int VPT, int S>
+int dispatch_k(int K, hipStream_t st, const RecArgs &a) {
   switch (K) {
-    case 8:  launch_rec<8, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
-    case 16: launch_rec<16, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
-    case 32: launch_rec<32, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
-    case 64: launch_rec<64, VPT>(st, img, mom, xl, yl, zl, vidx, nvox, ant, meas, rxo, f0c, dfc, ny, nz, rows, R, F, is_ti); break;
+    case 8:  launch_rec<8, VPT, S>(st, a); return 0;
+    case 16: launch_rec<16, VPT, S>(st, a); return 0;
+    case 32: launch_rec<32, VPT, S>(st, a); return 0;
+    case 64: launch_rec<64, VPT, S>(st, a); return 0;
     default: return -1;
   }
-  return 0;
+}
+
+// Lane shape: VPT voxels per lane, S-way split Horner.  The defaults (2, 1)
+// were chosen by measurement on MI355X (config 2, K = 64): VPT 2 beats 1 by
+// ~8 %; the even/odd split and two blocks side by side give no gain (the
+// latter spills SGPRs); staging rows in LDS with a double-buffered tile, or
+// streaming chunks with explicit prefetch, measured 15-50 % slower (their
+// extra VGPRs cost occupancy).  MMN_REC_VPT / MMN_REC_SPLIT override them.
+int dispatch_rec(int K, hipStream_t st, const RecArgs &a) {
+  const int vpt = env_int("MMN_REC_VPT", 2);
+  const int split = env_int("MMN_REC_SPLIT", 1);
+  if (vpt == 1 && split == 1) return dispatch_k<1, 1>(K, st, a);
+  if (vpt == 1) return dispatch_k<1, 2>(K, st, a);
+  if (split == 1) return dispatch_k<2, 1>(K, st, a);
+  return dispatch_k<2, 2>(K, st, a);
 }
 
 }  // namespace
@@ -398,7 +462,7 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
               MMN_ERR_ARG, "mmn_bp_forward: sizes must be positive");
   MMN_REQUIRE(variant == MMN_BP_PLAIN || variant == MMN_BP_REC, MMN_ERR_ARG,
               "mmn_bp_forward: unknown variant %d", variant);
-  if (K == 0) K = 32;
+  if (K == 0) K = 64;
   MMN_REQUIRE(variant != MMN_BP_REC || K == 8 || K == 16 || K == 32 || K == 64,
               MMN_ERR_ARG, "mmn_bp_forward: K must be 8, 16, 32 or 64 (got %d)",
               K);
@@ -430,21 +494,10 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
                          num_rows, num_rx, samples, is_ti_radar);
     return mmn_check_launch("mmn_bp_forward(plain)");
   }
-  const double f0c = f0 * inv_c, dfc = df * inv_c;
-  const int vpt = env_int("MMN_REC_VPT", 2);
-  int rc;
-  if (vpt == 1)
-    rc = dispatch_rec<1>(K, st, im, mo, x_locs, y_locs, z_locs, vox_idx, nvox,
-                         ant, ms, rx_off, f0c, dfc, ny, nz, num_rows, num_rx,
-                         samples, is_ti_radar);
-  else if (vpt == 4)
-    rc = dispatch_rec<4>(K, st, im, mo, x_locs, y_locs, z_locs, vox_idx, nvox,
-                         ant, ms, rx_off, f0c, dfc, ny, nz, num_rows, num_rx,
-                         samples, is_ti_radar);
-  else
-    rc = dispatch_rec<2>(K, st, im, mo, x_locs, y_locs, z_locs, vox_idx, nvox,
-                         ant, ms, rx_off, f0c, dfc, ny, nz, num_rows, num_rx,
-                         samples, is_ti_radar);
+  RecArgs ra{im, mo, x_locs, y_locs, z_locs, vox_idx, nvox, ant, ms, rx_off,
+             f0 * inv_c, df * inv_c, ny, nz, num_rows, num_rx,
+             samples, is_ti_radar};
+  const int rc = dispatch_rec(K, st, ra);
   MMN_REQUIRE(rc == 0, MMN_ERR_ARG, "mmn_bp_forward: bad K %d", K);
   return mmn_check_launch("mmn_bp_forward(rec)");
 }

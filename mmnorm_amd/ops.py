@@ -75,7 +75,7 @@ def _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx):
 
 
 def backproject(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
-                variant="rec", K=32, moment=False, vox_idx=None, is_ti=False):
+                variant="rec", K=64, moment=False, vox_idx=None, is_ti=False):
     """Raw full-frequency sum  I_v = sum_ij meas_ij exp(-j 2 pi (f0 + j df) d_iv / c).
 
     Returns ``img`` complex64 (NX, NY, NZ) -- or (n,) with ``vox_idx`` -- and, if
@@ -204,7 +204,7 @@ class Backproject(torch.autograd.Function):
 
 
 def backproject_ad(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df, scale=1.0,
-                   K=32, vox_idx=None, is_ti=False):
+                   K=64, vox_idx=None, is_ti=False):
     """Differentiable ``scale * backproject`` (recurrence kernel forward)."""
     if rx_off is None:
         rx_off = torch.zeros((1, 3), dtype=torch.float32, device=meas.device)

@@ -1,0 +1,191 @@
+"""End-to-end synthetic pipeline: forward with the fused vote, RSDF BFS, then
+isosurface optimisation of the antenna positions through the backward kernel.
+
+    python -m mmnorm_amd.pipeline --config 5 --steps 50 [--out DIR]
+
+1. Scene (``synth``), true antenna positions p*, start positions
+   p0 = p* + N(0, sigma) (sigma = 0.3 mm by default).
+2. Full-grid forward at p0 (recurrence kernel, vote moment fused).
+3. Normals on voxels above 5 % of the image maximum (py_sn_est_gpu.py:326),
+   unit-normalised (image_processor.py:396); 6-connected components and the
+   reference's size rule (sdf.py:105-119); per-component seeds
+   (sdf.py:182-191); GPU RSDF BFS.
+4. Isosurface band B = {reached voxels with |rsdf| <= tau}.  The loss is the
+   negative image energy on the band, L(p) = -sum_{v in B} |I_v(p)|^2 / E0,
+   with I = backprojection / A over the band voxels only, minimised over the
+   antenna positions with Adam; every step is one forward and one backward
+   kernel launch over the band (phases recomputed, nothing stored per
+   voxel x row x sample).
+The reference has no gradient loop (its isosurface stage is a grid search,
+isosurface_optimization.py:462-521); this loss and optimiser are defined here.
+Progress is printed per step; ``--out`` also writes a JSON record and an
+optimiser checkpoint every ``--ckpt-every`` steps (resumed with ``--resume``).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+from scipy import ndimage
+
+from . import ops, sdf, synth
+
+
+def log(msg):
+    print(f"[pipeline] {msg}", flush=True)
+
+
+def valid_components(mask, dx):
+    """Components and the reference's size filter (sdf.py:94-120), labels + 1."""
+    labeled, nlab = ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
+    labeled = labeled + 1
+    labels = np.arange(1, labeled.max() + 2)
+    areas = np.array(ndimage.sum(mask, labeled, labels))
+    for a, b in ((0.03, 0.04), (0.015, 0.02)):
+        lo, lz = int(a / dx), int(b / dx)
+        valid = labels[areas > lo ** 2 * lz]
+        if len(valid):
+            return labeled, valid
+    return labeled, labels[:0]
+
+
+def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-5, K=64, out=None,
+        ckpt_every=10, resume=False, seed=7):
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t0 = time.perf_counter()
+    sc = synth.make_scene(cid, device=dev)
+    cfg = sc.cfg
+    log(f"config {cid}: scene ready in {time.perf_counter() - t0:.1f} s "
+        f"(A={cfg.rows}, F={cfg.n_freq}, grid={cfg.grid})")
+    T = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x, dtype=dt)).to(dev)
+    meas = T(sc.meas, np.complex64)
+    xl, yl, zl = (T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs))
+    rng = np.random.default_rng(seed)
+    p_true = sc.ant.astype(np.float32)
+    p0 = (sc.ant + rng.normal(scale=sigma, size=sc.ant.shape)).astype(np.float32)
+    A = cfg.rows
+    rec = {"config": cid, "sigma_m": sigma, "steps": steps, "K": K}
+
+    # 2. full-grid forward with the fused vote moment
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    img, mom = ops.backproject(meas, T(p0, np.float32), xl, yl, zl, f0=sc.f0, df=sc.df,
+                               K=K, moment=True)
+    torch.cuda.synchronize()
+    rec["forward_full_s"] = time.perf_counter() - t
+    log(f"full-grid forward: {rec['forward_full_s']:.2f} s "
+        f"({cfg.work / rec['forward_full_s'] / 1e9:.0f} G voxel*row*sample/s)")
+
+    # 3. normals, components, seeds, BFS
+    t = time.perf_counter()
+    mag = img.abs().reshape(-1)
+    vidx = torch.nonzero(mag > 0.05 * mag.max()).reshape(-1).to(torch.int32)
+    n = ops.sn_from_moment(img.reshape(-1), mom.reshape(-1, 3)[vidx.long()].contiguous(),
+                           vidx).double()
+    n = n / torch.linalg.norm(n, dim=1, keepdim=True)
+    normals = torch.full((img.numel(), 3), float("nan"), dtype=torch.float64, device=dev)
+    normals[vidx.long()] = n
+    normals = normals.reshape(*img.shape, 3)
+    nrm_np = normals.cpu().numpy()
+    mask = ~np.isnan(nrm_np[..., 0])
+    dx = float(sc.x_locs[1] - sc.x_locs[0])
+    labeled, valid = valid_components(mask, dx)
+    m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
+    coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
+    seeds = sdf.component_seeds(nrm_np, coords, labeled, valid)
+    lab_t = T(labeled, np.int32)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    rsdf, hops = ops.rsdf_bfs(normals, lab_t, T(seeds, np.int32), dx)
+    torch.cuda.synchronize()
+    rec["bfs_s"] = time.perf_counter() - tb
+    rec["prep_s"] = time.perf_counter() - t
+    reached = hops >= 0
+    band = reached & (rsdf.abs() <= tau_vox * dx)
+    bidx = torch.nonzero(band.reshape(-1)).reshape(-1).to(torch.int32)
+    rec.update({"valid_voxels": int(vidx.numel()), "components": int(len(seeds)),
+                "reached": int(reached.sum().item()), "levels": int(hops.max().item()) + 1,
+                "band_voxels": int(bidx.numel())})
+    log(f"vote + BFS: {rec['prep_s']:.2f} s (BFS {rec['bfs_s'] * 1e3:.1f} ms, "
+        f"{rec['components']} components, {rec['reached']} voxels reached, "
+        f"{rec['levels']} levels); band {rec['band_voxels']} voxels")
+    if bidx.numel() == 0:
+        raise RuntimeError("empty isosurface band")
+
+    # 4. optimisation over the band
+    pos = T(p0, np.float32).requires_grad_(True)
+    opt = torch.optim.Adam([pos], lr=lr)
+    start = 0
+    ck = os.path.join(out, f"pipeline_config{cid}.pt") if out else None
+    if resume and ck and os.path.exists(ck):
+        state = torch.load(ck, weights_only=True)
+        with torch.no_grad():
+            pos.copy_(state["pos"])
+        opt.load_state_dict(state["opt"])
+        start = int(state["step"])
+        log(f"resumed at step {start}")
+    with torch.no_grad():
+        e0 = (ops.backproject(meas, pos.detach(), xl, yl, zl, f0=sc.f0, df=sc.df, K=K,
+                              vox_idx=bidx).abs() ** 2).sum().item() / A ** 2
+    hist = []
+    pt = torch.from_numpy(p_true).to(dev)
+
+    def pos_err():
+        d = (pos.detach() - pt)
+        d = d - d.mean(0, keepdim=True)  # a common shift is not observable
+        return float(torch.sqrt((d ** 2).sum(1).mean()).item())
+
+    log(f"start: band energy {e0:.4e}, rms position error {pos_err() * 1e3:.3f} mm")
+    step_times = []
+    for it in range(start, steps):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        opt.zero_grad(set_to_none=True)
+        I = ops.backproject_ad(meas, pos, xl, yl, zl, f0=sc.f0, df=sc.df, scale=1.0 / A,
+                               K=K, vox_idx=bidx)
+        loss = -(I.abs() ** 2).sum() / e0
+        loss.backward()
+        opt.step()
+        torch.cuda.synchronize()
+        step_times.append(time.perf_counter() - ts)
+        hist.append({"step": it + 1, "loss": float(loss.item()), "rms_err_mm": pos_err() * 1e3,
+                     "s": step_times[-1]})
+        log(f"step {it + 1}/{steps}: loss {hist[-1]['loss']:.6f}, rms position error "
+            f"{hist[-1]['rms_err_mm']:.4f} mm, {step_times[-1]:.2f} s")
+        if ck and ((it + 1) % ckpt_every == 0 or it + 1 == steps):
+            torch.save({"pos": pos.detach(), "opt": opt.state_dict(), "step": it + 1}, ck)
+    rec["history"] = hist
+    rec["step_s_mean"] = float(np.mean(step_times)) if step_times else None
+    rec["band_units_per_step"] = int(bidx.numel()) * A * cfg.n_freq
+    rec["total_s"] = time.perf_counter() - t0
+    if out:
+        os.makedirs(out, exist_ok=True)
+        with open(os.path.join(out, f"pipeline_config{cid}.json"), "w") as f:
+            json.dump(rec, f, indent=1)
+    return rec
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--config", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--sigma", type=float, default=3e-4)
+    ap.add_argument("--tau-vox", type=float, default=1.5)
+    ap.add_argument("--lr", type=float, default=2e-5)
+    ap.add_argument("--K", type=int, default=64)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--ckpt-every", type=int, default=10)
+    ap.add_argument("--resume", action="store_true")
+    a = ap.parse_args(argv)
+    rec = run(a.config, a.steps, a.sigma, a.tau_vox, a.lr, a.K, a.out, a.ckpt_every, a.resume)
+    print(json.dumps({k: v for k, v in rec.items() if k != "history"}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

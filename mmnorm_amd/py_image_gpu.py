@@ -93,7 +93,7 @@ def _phase_centers(locs_rep, rx_offsets):
 def image_cuda(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offsets, slope,
                wavelength, bandwidth, num_samples, use_4_rx=False, is_ti_radar=False,
                use_interpolated_processing=False, return_phase_centers=True, *,
-               variant="rec", K=32):
+               variant="rec", K=64):
     """SAR image on the GPU (see module docstring); returns (image, centers) or image."""
     if variant not in ("rec", "plain", "ref"):
         raise ValueError("variant must be 'rec', 'plain' or 'ref'")

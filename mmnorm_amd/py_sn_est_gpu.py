@@ -78,7 +78,7 @@ def _sn_est(sum_image, x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offs
 def normal_estimates_cuda(sum_image, x_locs, y_locs, z_locs, antenna_locs, radar_data,
                           rx_offset, slope, wavelength, bandwidth, num_samples,
                           use_4_rx=False, is_ti_radar=False, initial_filter_percent=0.05,
-                          use_interpolated_processing=True, *, K=32):
+                          use_interpolated_processing=True, *, K=64):
     """Surface-normal votes on the GPU (see module docstring)."""
     if not use_interpolated_processing:
         return _sn_est(sum_image, x_locs, y_locs, z_locs, antenna_locs, radar_data,
