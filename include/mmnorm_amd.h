@@ -101,14 +101,21 @@ int mmn_cuda_sn_est(float *normals, const int32_t *vox_idx, int64_t n_valid,
  * No reference counterpart for the fused moment or the variants; the
  * reference computes the vote only for the interpolated branch
  * (py_sn_est_gpu.py:404-405).
+ * workspace (optional, may be NULL): lets a small voxel set be split over
+ * row segments whose partial sums are reduced in a fixed order, so that the
+ * device fills; mmn_bp_forward_workspace_size() gives the bytes that enable
+ * every split the launch could choose.  Results are deterministic for a
+ * given workspace size and device.
  */
+size_t mmn_bp_forward_workspace_size(int64_t n_vox, int num_rows, int with_moment);
 int mmn_bp_forward(float *img, float *moment, const float *x_locs,
                    const float *y_locs, const float *z_locs,
                    const int32_t *vox_idx, int64_t n_vox, const float *ant,
                    const float *meas, const float *rx_off, double f0,
                    double df, int nx, int ny, int nz, int num_rows,
                    int num_rx, int samples, int is_ti_radar, int variant,
-                   int K, void *stream);
+                   int K, void *workspace, size_t workspace_bytes,
+                   void *stream);
 
 /*
  * mmn_sn_from_moment -- normals[n][3] = Re(conj(I) * moment[n][3]) / |I| with
@@ -130,8 +137,8 @@ int mmn_sn_from_moment(float *normals, const float *img, const float *moment,
  * with Q_iv = sum_j (2 pi f_j / c) meas[i][j] exp(-j 2 pi f_j d_iv / c).
  * Either output may be NULL (not computed).  No reference counterpart:
  * the reference computes no gradients.  Deterministic: partial sums go to
- * the workspace and are reduced in a fixed order.  K (8, 16, 32 or 48;
- * 0 = 32) is the sample block of one lane and its re-anchor interval.
+ * the workspace and are reduced in a fixed order.  K (16, 32 or 48;
+ * 0 = 32) is the sample block of one wave and its re-anchor interval.
  */
 size_t mmn_bp_backward_workspace_size(int num_rows, int samples, int64_t n_vox);
 int mmn_bp_backward(float *grad_meas, float *grad_ant, const float *g,

@@ -9,13 +9,14 @@
 //   dL/dp_i  = scale * sum_v Im( sum_j k_j s_ij conj(z_ijv) ) grad_p d_iv,
 //              z_ijv = g_v conj(phi_ijv),  k_j = 2 pi f_j / c,
 //              grad_p d = -(v-p)/|v-p| - (v-p-o)/|v-p-o|.
-// Both reduce over voxels, so the mapping is transposed with respect to the
-// forward: one lane per (row, block of K samples), all lanes of a wave walk
-// the same voxel (wave-uniform voxel records through scalar loads).  Per
-// (lane, voxel) the K conjugate phasors come from one FP64-reduced anchor and
-// K-1 FP32 rotations; per sample the lane does one rotate (4 FMA), one
-// accumulate (2 add) and the position term (2 FMA).  Each wave owns a voxel
-// sub-range and writes its partial sums to the workspace; a second kernel
+// Both reduce over voxels.  A wave owns one (row, block of K samples, voxel
+// chunk): its K coefficients k_j s_ij are wave-uniform (LDS, broadcast
+// reads), its lanes walk the chunk's voxels with coalesced record loads and
+// keep lane-local sums, and one butterfly across the wave reduces them at the
+// end.  Per (lane, voxel) the K conjugate phasors come from one FP64-reduced
+// anchor and a two-term FP32 recurrence; per sample the lane does the
+// recurrence (2 FMA), one accumulate (2 add) and the position term (2 FMA).
+// Each wave writes its partial sums to the workspace; a second kernel
 // reduces the partials in a fixed order (FP64), so results are bitwise
 // reproducible.
 #include <stdlib.h>
@@ -27,35 +28,30 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kTargetWaves = 8192;
 
-struct VoxRec {  // 32 B, one scalar load per voxel
-  float vx, vy, vz, pad0;
-  float gr, gi, pad1, pad2;
+// Per-voxel record of the backward: position and Re(g) in one 16-byte load,
+// Im(g) in a second array (20 bytes per voxel in all).
+struct VoxRec {
+  float4 p;  // vx, vy, vz, Re g
+  float gi;  // Im g
 };
 
 struct BwdPlan {
   int sub;     // voxel sub-ranges (one per wave)
   int nfb;     // sample blocks of this launch
-  int nfbmax;  // sample blocks the workspace is sized for (K = 8)
+  int nfbmax;  // sample blocks the workspace is sized for (K = 16)
   int64_t chunk;
 };
 
-BwdPlan make_plan(int A, int F, int64_t nvox, int K) {
-  BwdPlan p;
-  const int agroups = (A + kWave - 1) / kWave;
-  const int per_sub = agroups * ((F + 31) / 32);
-  int64_t sub = (kTargetWaves + per_sub - 1) / per_sub;
-  const int64_t by_vox = (nvox + 63) / 64;  // at least ~64 voxels per wave
-  if (sub > by_vox) sub = by_vox;
-  if (sub < 1) sub = 1;
-  if (sub > 1024) sub = 1024;
-  sub = (sub + kWavesPerBlock - 1) / kWavesPerBlock * kWavesPerBlock;
-  p.sub = (int)sub;
-  p.nfb = K > 0 ? (F + K - 1) / K : 0;
-  p.nfbmax = (F + 7) / 8;
-  p.chunk = (nvox + p.sub - 1) / p.sub;
-  return p;
+// Voxel sub-ranges (one per wave) the workspace is sized for: at least ~64
+// voxels per wave, at most 1024 sub-ranges and ~1 GiB of partial sums.
+int sub_max(int A, int F, int64_t nvox) {
+  int64_t m = (nvox + 63) / 64;
+  const int64_t by_ws = ((int64_t)1 << 30) / ((int64_t)A * F * 8);
+  if (m > by_ws) m = by_ws;
+  if (m > 1024) m = 1024;
+  m = m / kWavesPerBlock * kWavesPerBlock;
+  return (int)(m < kWavesPerBlock ? kWavesPerBlock : m);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -66,24 +62,29 @@ int env_int(const char *name, int dflt) {
 }
 
 struct BwdWs {
-  VoxRec *rec;    // [nvox]
+  float4 *rec4;   // [nvox]
+  float *recg;    // [nvox]
   float2 *gs;     // [sub][A][F]
   float *gp;      // [sub][nfbmax][A][3]
 };
 
 size_t bwd_layout(int A, int F, int64_t nvox, BwdWs *w, char *base) {
-  const BwdPlan p = make_plan(A, F, nvox, 8);
+  BwdPlan p;
+  p.sub = sub_max(A, F, nvox);
+  p.nfbmax = (F + 15) / 16;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     char *q = base ? base + off : nullptr;
     off += align256(bytes);
     return q;
   };
-  char *r = take(sizeof(VoxRec) * (size_t)nvox);
+  char *r4 = take(sizeof(float4) * (size_t)nvox);
+  char *rg = take(sizeof(float) * (size_t)nvox);
   char *s = take(sizeof(float2) * (size_t)p.sub * A * F);
   char *g = take(sizeof(float) * (size_t)p.sub * p.nfbmax * A * 3);
   if (w) {
-    w->rec = (VoxRec *)r;
+    w->rec4 = (float4 *)r4;
+    w->recg = (float *)rg;
     w->gs = (float2 *)s;
     w->gp = (float *)g;
   }
@@ -91,20 +92,18 @@ size_t bwd_layout(int A, int F, int64_t nvox, BwdWs *w, char *base) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_pack(
-    VoxRec *__restrict__ rec, const float2 *__restrict__ g,
+    float4 *__restrict__ rec4, float *__restrict__ recg, const float2 *__restrict__ g,
     const float *__restrict__ xl, const float *__restrict__ yl,
     const float *__restrict__ zl, const int32_t *__restrict__ vidx,
     int64_t nvox, int ny, int nz) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= nvox) return;
   const int64_t lin = vidx ? (int64_t)vidx[t] : t;
-  VoxRec r;
-  voxel_coords(xl, yl, zl, ny, nz, lin, r.vx, r.vy, r.vz);
+  float vx, vy, vz;
+  voxel_coords(xl, yl, zl, ny, nz, lin, vx, vy, vz);
   const float2 gv = g[t];
-  r.gr = gv.x;
-  r.gi = gv.y;
-  r.pad0 = r.pad1 = r.pad2 = 0.0f;
-  rec[t] = r;
+  rec4[t] = make_float4(vx, vy, vz, gv.x);
+  recg[t] = gv.y;
 }
 
 // Per (lane row a, voxel v) state of one sample block: the conjugate phasors
@@ -121,38 +120,49 @@ struct VoxStart {
   float gx, gy, gz;              // grad_p d
 };
 
-__device__ __forceinline__ VoxStart vox_start(const VoxRec &r, float px, float py,
-                                              float pz, float ox, float oy, float oz,
+__device__ __forceinline__ VoxStart vox_start(const VoxRec &r, double pxd, double pyd,
+                                              double pzd, float ox, float oy, float oz,
                                               bool mono, int is_ti, double fc,
                                               double dfc, float weight) {
   VoxStart v;
-  double d = roundtrip_dist64(r.vx, r.vy, r.vz, px, py, pz, ox, oy, oz, mono);
+  // round-trip distance in FP64 (as roundtrip_dist64), keeping the first
+  // leg's difference vector and reciprocal length for grad_p d
+  const double dx = (double)r.p.x - pxd, dy = (double)r.p.y - pyd, dz = (double)r.p.z - pzd;
+  const double s1 = fma(dz, dz, fma(dy, dy, dx * dx));
+  const float y1 = __builtin_amdgcn_rsqf((float)s1);
+  const double y1d = (double)y1;
+  const double d0 = s1 * y1d;
+  const double d1 = fma(0.5 * y1d, fma(-d0, d0, s1), d0);
+  const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+  double d;
+  float gx, gy, gz;
+  if (mono) {
+    d = d1 + d1;
+    const float y2 = y1 + y1;
+    gx = fx * y2;
+    gy = fy * y2;
+    gz = fz * y2;
+  } else {
+    const double ex = dx - (double)ox, ey = dy - (double)oy, ez = dz - (double)oz;
+    d = d1 + sqrt_nr64(fma(ez, ez, fma(ey, ey, ex * ex)));
+    const float bx = (float)ex, by = (float)ey, bz = (float)ez;
+    const float n2 = rsqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
+    gx = fmaf(bx, n2, fx * y1);
+    gy = fmaf(by, n2, fy * y1);
+    gz = fmaf(bz, n2, fz * y1);
+  }
   if (is_ti) d += 0.15;
   float ar, ai, rr, ri;
   phasor_hw(d * fc, ar, ai);  // exp(-j ...); conjugated below
   phasor_hw(d * dfc, rr, ri);
   ai = -ai;
   ri = -ri;
-  const float gr = r.gr * weight, gi = r.gi * weight;
+  const float gr = r.p.w * weight, gi = r.gi * weight;
   v.z0r = gr * ar - gi * ai;
   v.z0i = gr * ai + gi * ar;
   v.z1r = v.z0r * rr - v.z0i * ri;
   v.z1i = v.z0r * ri + v.z0i * rr;
   v.c2 = rr + rr;
-  const float ax = r.vx - px, ay = r.vy - py, az = r.vz - pz;
-  const float n1 = rsqrtf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
-  float gx = ax * n1, gy = ay * n1, gz = az * n1;
-  if (mono) {
-    gx += gx;
-    gy += gy;
-    gz += gz;
-  } else {
-    const float bx = ax - ox, by = ay - oy, bz = az - oz;
-    const float n2 = rsqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
-    gx = fmaf(bx, n2, gx);
-    gy = fmaf(by, n2, gy);
-    gz = fmaf(bz, n2, gz);
-  }
   // d(d)/dp = -(unit vectors): the sign is applied with the position term.
   v.gx = gx;
   v.gy = gy;
@@ -160,73 +170,69 @@ __device__ __forceinline__ VoxStart vox_start(const VoxRec &r, float px, float p
   return v;
 }
 
+// Transposed mapping: one wave per (row a, sample block fb, voxel chunk c);
+// its lanes walk different voxels of the chunk (coalesced record loads) and
+// keep lane-local sums, which one butterfly across the wave reduces at the
+// end.  The block's K coefficients k_t s_{a,t} are wave-uniform and sit in
+// LDS (broadcast reads), so a lane holds only its K accumulators.
 template <int K, int VV>
-__global__ __launch_bounds__(kBlock) void k_bwd(
+__global__ __launch_bounds__(kBlock) void k_bwd_t(
     float2 *__restrict__ gs, float *__restrict__ gp,
-    const VoxRec *__restrict__ rec, int64_t nvox, int64_t chunk,
-    const float *__restrict__ ant, const float2 *__restrict__ meas,
-    const float *__restrict__ rxo, double f0c, double dfc, float k0,
-    float dk, int A, int R, int F, int is_ti, int nfbmax) {
+    const float4 *__restrict__ rec4, const float *__restrict__ recg, int64_t nvox,
+    int64_t chunk, const float *__restrict__ ant, const float2 *__restrict__ meas,
+    const float *__restrict__ rxo, double f0c, double dfc, float k0, float dk, int A,
+    int R, int F, int is_ti, int nfbmax, int nfb, int64_t nitems) {
+  __shared__ float2 sp_all[kWavesPerBlock][K];
   const int lane = threadIdx.x % kWave;
-  const int sub = blockIdx.z * kWavesPerBlock + threadIdx.x / kWave;
-  const int fb = blockIdx.y;
-  const int a = min(blockIdx.x * kWave + lane, A - 1);
-  const bool live_row = (blockIdx.x * kWave + lane) < A;
+  const int wv = threadIdx.x / kWave;
+  const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+  const bool live = item < nitems;
+  const int64_t it = live ? item : nitems - 1;
+  const int a = (int)(it % A);
+  const int64_t rest = it / A;
+  const int fb = (int)(rest % nfb);
+  const int64_t c = rest / nfb;
   const int f_lo = fb * K;
   const int nf = min(K, F - f_lo);
+
+  float2 *sp = sp_all[wv];
+  if (lane < K) {
+    float2 sv = make_float2(0.0f, 0.0f);
+    if (lane < nf) sv = meas[(size_t)a * F + f_lo + lane];
+    const float kj = fmaf((float)(f_lo + lane), dk, k0);
+    sp[lane] = make_float2(kj * sv.x, kj * sv.y);
+  }
+  __syncthreads();
 
   const float px = ant[3 * a + 0], py = ant[3 * a + 1], pz = ant[3 * a + 2];
   const int rx = a % R;
   const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
   const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
-
-  // k_j * s_ij for this lane's samples (zero past the end of the row).
-  float spr[K], spi[K], accr[K], acci[K];
-  const float2 *__restrict__ row = meas + (size_t)a * F + f_lo;
-#pragma unroll
-  for (int t = 0; t < K; ++t) {
-    float2 s = make_float2(0.0f, 0.0f);
-    if (t < nf) s = row[t];
-    const float kj = fmaf((float)(f_lo + t), dk, k0);
-    spr[t] = kj * s.x;
-    spi[t] = kj * s.y;
-    accr[t] = acci[t] = 0.0f;
-  }
-  float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
+  const double pxd = px, pyd = py, pzd = pz;
   const double fc = fma((double)f_lo, dfc, f0c);
 
-  const int64_t v0 = (int64_t)sub * chunk;
+  float accr[K], acci[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) accr[t] = acci[t] = 0.0f;
+  float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
+
+  const int64_t v0 = c * chunk;
   const int64_t v1 = min(nvox, v0 + chunk);
-  // VV voxels per iteration: VV independent recurrences per lane.  A short
-  // last group repeats its final voxel with zero weight.
-  // Voxel records are read with vector loads (every lane the same address),
-  // whose in-order completion lets the next group's records be requested one
-  // iteration ahead: the wait for them then falls after a full group of work.
-  // `zero` is a lane value the compiler cannot see through, which keeps these
-  // uniform loads on the vector path instead of scalar loads (whose waits
-  // cover every outstanding scalar load and so cannot be overlapped).
-  int zero;
-  asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
-  const VoxRec *__restrict__ rv = rec + zero;
-  VoxRec nxt[VV];
-  if (v0 < v1) {
-#pragma unroll
-    for (int u = 0; u < VV; ++u) nxt[u] = rv[min(v0 + u, v1 - 1)];
-  }
-  for (int64_t v = v0; v < v1; v += VV) {
-    VoxRec cur[VV];
-#pragma unroll
-    for (int u = 0; u < VV; ++u) cur[u] = nxt[u];
-    if (v + VV < v1) {
-#pragma unroll
-      for (int u = 0; u < VV; ++u) nxt[u] = rv[min(v + VV + u, v1 - 1)];
-    }
+  // lane's voxels: v0 + lane + 64 * (VV * j + u)
+  for (int64_t vb = v0 + lane; vb < v1; vb += (int64_t)kWave * VV) {
+    // keeps the coefficient reads below inside the loop (re-read from LDS each
+    // group) instead of hoisted into K more registers per lane
+    asm volatile("" ::: "memory");
     VoxStart p[VV];
-    float w[VV];
 #pragma unroll
     for (int u = 0; u < VV; ++u) {
-      w[u] = (v + u < v1) ? 1.0f : 0.0f;
-      p[u] = vox_start(cur[u], px, py, pz, ox, oy, oz, mono, is_ti, fc, dfc, w[u]);
+      const int64_t vu = vb + (int64_t)kWave * u;
+      const bool ok = vu < v1;
+      VoxRec r;
+      const int64_t vs = ok ? vu : vb;
+      r.p = rec4[vs];
+      r.gi = recg[vs];
+      p[u] = vox_start(r, pxd, pyd, pzd, ox, oy, oz, mono, is_ti, fc, dfc, ok ? 1.0f : 0.0f);
     }
     float r0[VV], i0[VV], r1[VV], i1[VV], t1[VV], t2[VV];
 #pragma unroll
@@ -236,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
     }
 #pragma unroll
     for (int t = 0; t < K; ++t) {
+      const float2 s2 = sp[t];
       float sr = r0[0], si = i0[0];
 #pragma unroll
       for (int u = 1; u < VV; ++u) {
@@ -246,8 +253,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
       acci[t] += si;
 #pragma unroll
       for (int u = 0; u < VV; ++u) {
-        t1[u] = fmaf(spi[t], r0[u], t1[u]);
-        t2[u] = fmaf(spr[t], i0[u], t2[u]);
+        t1[u] = fmaf(s2.y, r0[u], t1[u]);
+        t2[u] = fmaf(s2.x, i0[u], t2[u]);
         const float r2 = fmaf(p[u].c2, r1[u], -r0[u]);
         const float i2 = fmaf(p[u].c2, i1[u], -i0[u]);
         r0[u] = r1[u]; i0[u] = i1[u]; r1[u] = r2; i1[u] = i2;
@@ -261,15 +268,29 @@ __global__ __launch_bounds__(kBlock) void k_bwd(
       gpz = fmaf(-tt, p[u].gz, gpz);
     }
   }
-  if (live_row) {
-    float2 *__restrict__ out = gs + ((size_t)sub * A + a) * F + f_lo;
+  // butterfly over the wave: every lane ends with the wave's totals
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      accr[t] += __shfl_xor(accr[t], o, kWave);
+      acci[t] += __shfl_xor(acci[t], o, kWave);
+    }
+    gpx += __shfl_xor(gpx, o, kWave);
+    gpy += __shfl_xor(gpy, o, kWave);
+    gpz += __shfl_xor(gpz, o, kWave);
+  }
+  if (live) {
+    float2 *__restrict__ out = gs + ((size_t)c * A + a) * F + f_lo;
 #pragma unroll
     for (int t = 0; t < K; ++t)
-      if (t < nf) out[t] = make_float2(accr[t], acci[t]);
-    float *__restrict__ o = gp + (((size_t)sub * nfbmax + fb) * A + a) * 3;
-    o[0] = gpx;
-    o[1] = gpy;
-    o[2] = gpz;
+      if (lane == t && t < nf) out[t] = make_float2(accr[t], acci[t]);
+    if (lane == 0) {
+      float *__restrict__ o = gp + (((size_t)c * nfbmax + fb) * A + a) * 3;
+      o[0] = gpx;
+      o[1] = gpy;
+      o[2] = gpz;
+    }
   }
 }
 
@@ -322,8 +343,8 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
                   samples > 0,
               MMN_ERR_ARG, "mmn_bp_backward: sizes must be positive");
   if (K == 0) K = 32;
-  MMN_REQUIRE(K == 8 || K == 16 || K == 32 || K == 48, MMN_ERR_ARG,
-              "mmn_bp_backward: K must be 8, 16, 32 or 48 (got %d)", K);
+  MMN_REQUIRE(K == 16 || K == 32 || K == 48, MMN_ERR_ARG,
+              "mmn_bp_backward: K must be 16, 32 or 48 (got %d)", K);
   const int64_t nvox = vox_idx ? n_vox : (int64_t)nx * ny * nz;
   MMN_REQUIRE(nvox >= 0, MMN_ERR_ARG, "mmn_bp_backward: n_vox < 0");
   hipStream_t st = (hipStream_t)stream;
@@ -343,27 +364,47 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
               "mmn_bp_backward: workspace of %zu bytes needed", need);
   BwdWs w;
   bwd_layout(A, F, nvox, &w, (char *)workspace);
-  const BwdPlan p = make_plan(A, F, nvox, K);
-
+  const int vv = env_int("MMN_BWD_VV", 2);
+  // transposed mapping: items = rows x blocks x voxel chunks
+  BwdPlan p;
+  p.nfb = (F + K - 1) / K;
+  p.nfbmax = (F + 15) / 16;
+  void (*kt)(float2 *, float *, const float4 *, const float *, int64_t, int64_t,
+             const float *, const float2 *, const float *, double, double, float, float,
+             int, int, int, int, int, int, int64_t);
+  if (K == 16) kt = vv == 1 ? k_bwd_t<16, 1> : k_bwd_t<16, 2>;
+  else if (K == 48) kt = vv == 2 ? k_bwd_t<48, 2> : k_bwd_t<48, 1>;
+  else kt = vv == 1 ? k_bwd_t<32, 1> : k_bwd_t<32, 2>;
+  int per_cu = 0, dev = 0, cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kt, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipGetLastError();
+  // chunks: enough waves for 8 rounds of resident slots, at least 256
+  // voxels per wave, at most the workspace's sub-range count
+  const int64_t slots_waves = (int64_t)per_cu * cus * kWavesPerBlock;
+  const int64_t pairs = (int64_t)A * p.nfb;
+  int64_t ch = (8 * slots_waves + pairs - 1) / pairs;
+  const int64_t by_vox = (nvox + 255) / 256;
+  if (ch > by_vox) ch = by_vox;
+  const int smax = sub_max(A, F, nvox);
+  if (ch > smax) ch = smax;
+  if (ch < 1) ch = 1;
+  p.sub = (int)ch;
+  p.chunk = (nvox + ch - 1) / ch;
+  const int64_t nitems = pairs * ch;
   hipLaunchKernelGGL(k_pack, dim3(mmn_blocks(nvox, kBlock)), dim3(kBlock), 0,
-                     st, w.rec, (const float2 *)g, x_locs, y_locs, z_locs,
+                     st, w.rec4, w.recg, (const float2 *)g, x_locs, y_locs, z_locs,
                      vox_idx, nvox, ny, nz);
   const double inv_c = 1.0 / MMN_C0;
   const double two_pi = 6.283185307179586476925286766559;
-  const float k0 = (float)(two_pi * f0 * inv_c);
-  const float dk = (float)(two_pi * df * inv_c);
-  const dim3 grid((A + kWave - 1) / kWave, p.nfb, p.sub / kWavesPerBlock);
-#define MMN_BWD_LAUNCH(KK, VV)                                                \
-  hipLaunchKernelGGL((k_bwd<KK, VV>), grid, dim3(kBlock), 0, st, w.gs, w.gp, w.rec, \
-                     nvox, p.chunk, ant, (const float2 *)meas, rx_off,        \
-                     f0 * inv_c, df * inv_c, k0, dk, A, num_rx,               \
-                     F, is_ti_radar, p.nfbmax)
-  const int vv = env_int("MMN_BWD_VV", 2);
-  if (K == 8) MMN_BWD_LAUNCH(8, 2);
-  else if (K == 16) { if (vv == 1) MMN_BWD_LAUNCH(16, 1); else MMN_BWD_LAUNCH(16, 2); }
-  else if (K == 48) { if (vv == 2) MMN_BWD_LAUNCH(48, 2); else MMN_BWD_LAUNCH(48, 1); }
-  else { if (vv == 1) MMN_BWD_LAUNCH(32, 1); else MMN_BWD_LAUNCH(32, 2); }
-#undef MMN_BWD_LAUNCH
+  hipLaunchKernelGGL(kt, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
+                     dim3(kBlock), 0, st, w.gs, w.gp, w.rec4, w.recg, nvox, p.chunk, ant,
+                     (const float2 *)meas, rx_off, f0 * inv_c, df * inv_c,
+                     (float)(two_pi * f0 * inv_c), (float)(two_pi * df * inv_c), A,
+                     num_rx, F, is_ti_radar, p.nfbmax, p.nfb, nitems);
   if (grad_meas)
     hipLaunchKernelGGL(k_reduce_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)),
                        dim3(kBlock), 0, st, (float2 *)grad_meas, w.gs, p.sub,

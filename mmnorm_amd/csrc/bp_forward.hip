@@ -299,6 +299,9 @@ __device__ __forceinline__ void rec_block(const float2 *__restrict__ row, int b0
   }
 }
 
+// Row segments: blockIdx.y takes rows [y*rps, min((y+1)*rps, rows)) and, when
+// there is more than one segment, writes its partial sums to img/mom + y*n
+// (a workspace reduced afterwards in a fixed order).
 template <int K, int VPT, int S>
 __global__ __launch_bounds__(kBlock) void k_bp_rec(
     float2 *__restrict__ img, float2 *__restrict__ mom,
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_rec(
     int64_t nvox, const float *__restrict__ ant,
     const float2 *__restrict__ meas, const float *__restrict__ rxo,
     double f0c, double dfc, int ny, int nz, int rows, int R, int F,
-    int is_ti) {
+    int is_ti, int rps) {
   const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
   float vx[VPT], vy[VPT], vz[VPT];
   load_voxels<VPT>(xl, yl, zl, vidx, nvox, ny, nz, base, vx, vy, vz);
@@ -321,7 +324,9 @@ __global__ __launch_bounds__(kBlock) void k_bp_rec(
   const int nb = F / K;
   const int tail = F - nb * K;
 
-  for (int i = 0; i < rows; ++i) {
+  const int row_lo = blockIdx.y * rps;
+  const int row_hi = min(rows, row_lo + rps);
+  for (int i = row_lo; i < row_hi; ++i) {
     const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
     const int rx = i % R;
     const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
@@ -366,7 +371,36 @@ __global__ __launch_bounds__(kBlock) void k_bp_rec(
       add_row(cr[k], ci[k], vax - vx[k], vay - vy[k], vaz - vz[k], Ir[k],
               Ii[k], Cr[k], Ci[k], M[k]);
   }
-  store_voxels<VPT>(img, mom, nvox, base, Ir, Ii, M);
+  store_voxels<VPT>(img + (size_t)blockIdx.y * nvox,
+                    mom ? mom + (size_t)blockIdx.y * nvox * 3 : nullptr, nvox, base, Ir,
+                    Ii, M);
+}
+
+// Fixed-order FP64 sum of the row segments' partial sums.
+__global__ __launch_bounds__(kBlock) void k_rs_reduce(
+    float2 *__restrict__ img, float2 *__restrict__ mom,
+    const float2 *__restrict__ pimg, const float2 *__restrict__ pmom, int64_t n,
+    int rs) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < rs; ++q) {
+    const float2 v = pimg[(size_t)q * n + t];
+    a += v.x;
+    b += v.y;
+  }
+  img[t] = make_float2((float)a, (float)b);
+  if (mom) {
+    for (int k = 0; k < 3; ++k) {
+      double c = 0.0, e = 0.0;
+      for (int q = 0; q < rs; ++q) {
+        const float2 v = pmom[((size_t)q * n + t) * 3 + k];
+        c += v.x;
+        e += v.y;
+      }
+      mom[3 * t + k] = make_float2((float)c, (float)e);
+    }
+  }
 }
 
 int env_int(const char *name, int dflt) {
@@ -384,14 +418,68 @@ struct RecArgs {
   const float *rxo;
   double f0c, dfc;
   int ny, nz, rows, R, F, is_ti;
+  void *ws;        // row-segment partials (may be null)
+  size_t ws_bytes;
 };
+
+// Most row segments a launch may use for n voxels: enough to fill the device
+// for small voxel sets, never fewer than 32 rows per segment.
+int rs_cap(int64_t n, int rows) {
+  int cap = rows / 32;
+  if (cap > 32) cap = 32;
+  if (cap < 1) cap = 1;
+  (void)n;
+  return cap;
+}
+
+size_t rs_bytes(int rs, int64_t n, bool moment) {
+  return rs > 1 ? (size_t)rs * (size_t)n * sizeof(float2) * (moment ? 4 : 1) : 0;
+}
+
+// Number of row segments: the smallest count that gives the device at least
+// eight full rounds of resident workgroups (occupancy from the runtime), within
+// rs_cap and the workspace the caller provided.  MMN_REC_RS forces a value.
+template <typename KernelT>
+int choose_rs(KernelT kernel, int64_t nblk, const RecArgs &a) {
+  int rs = 1;
+  const int forced = env_int("MMN_REC_RS", 0);
+  if (forced > 0) {
+    rs = forced;
+  } else {
+    int per_cu = 0, dev = 0, cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) !=
+            hipSuccess || per_cu < 1)
+      per_cu = 2;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipGetLastError();
+    const int64_t slots = (int64_t)per_cu * cus;
+    while (nblk * rs < 8 * slots && rs < 64) ++rs;
+  }
+  const int cap = rs_cap(a.nvox, a.rows);
+  if (rs > cap) rs = cap;
+  while (rs > 1 && rs_bytes(rs, a.nvox, a.mom != nullptr) > a.ws_bytes) --rs;
+  if (!a.ws) rs = 1;
+  return rs;
+}
 
 template <int K, int VPT, int S>
 void launch_rec(hipStream_t st, const RecArgs &a) {
-  hipLaunchKernelGGL((k_bp_rec<K, VPT, S>), dim3(mmn_blocks(a.nvox, kBlock * VPT)),
-                     dim3(kBlock), 0, st, a.img, a.mom, a.xl, a.yl, a.zl, a.vidx,
-                     a.nvox, a.ant, a.meas, a.rxo, a.f0c, a.dfc, a.ny, a.nz,
-                     a.rows, a.R, a.F, a.is_ti);
+  const int64_t nblk = mmn_blocks(a.nvox, kBlock * VPT);
+  const int rs = choose_rs(k_bp_rec<K, VPT, S>, nblk, a);
+  const int rps = (a.rows + rs - 1) / rs;
+  const int nseg = (a.rows + rps - 1) / rps;
+  float2 *img = a.img, *mom = a.mom;
+  if (nseg > 1) {
+    img = (float2 *)a.ws;
+    mom = a.mom ? img + (size_t)nseg * a.nvox : nullptr;
+  }
+  hipLaunchKernelGGL((k_bp_rec<K, VPT, S>), dim3((unsigned)nblk, nseg), dim3(kBlock), 0, st,
+                     img, mom, a.xl, a.yl, a.zl, a.vidx, a.nvox, a.ant, a.meas, a.rxo,
+                     a.f0c, a.dfc, a.ny, a.nz, a.rows, a.R, a.F, a.is_ti, rps);
+  if (nseg > 1)
+    hipLaunchKernelGGL(k_rs_reduce, dim3(mmn_blocks(a.nvox, kBlock)), dim3(kBlock), 0, st,
+                       a.img, a.mom, img, mom, a.nvox, nseg);
 }
 
 template <int VPT, int S>
@@ -447,6 +535,12 @@ extern "C" int mmn_cuda_image(float *img, const float *x_locs,
   return mmn_check_launch("mmn_cuda_image");
 }
 
+extern "C" size_t mmn_bp_forward_workspace_size(int64_t n_vox, int num_rows,
+                                                int with_moment) {
+  if (n_vox <= 0 || num_rows <= 0) return 0;
+  return rs_bytes(rs_cap(n_vox, num_rows), n_vox, with_moment != 0);
+}
+
 extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
                               const float *y_locs, const float *z_locs,
                               const int32_t *vox_idx, int64_t n_vox,
@@ -454,7 +548,8 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
                               const float *rx_off, double f0, double df,
                               int nx, int ny, int nz, int num_rows,
                               int num_rx, int samples, int is_ti_radar,
-                              int variant, int K, void *stream) {
+                              int variant, int K, void *workspace,
+                              size_t workspace_bytes, void *stream) {
   MMN_REQUIRE(img && x_locs && y_locs && z_locs && ant && meas && rx_off,
               MMN_ERR_NULL, "mmn_bp_forward: NULL pointer argument");
   MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 &&
@@ -496,7 +591,7 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
   }
   RecArgs ra{im, mo, x_locs, y_locs, z_locs, vox_idx, nvox, ant, ms, rx_off,
              f0 * inv_c, df * inv_c, ny, nz, num_rows, num_rx,
-             samples, is_ti_radar};
+             samples, is_ti_radar, workspace, workspace_bytes};
   const int rc = dispatch_rec(K, st, ra);
   MMN_REQUIRE(rc == 0, MMN_ERR_ARG, "mmn_bp_forward: bad K %d", K);
   return mmn_check_launch("mmn_bp_forward(rec)");

@@ -89,12 +89,15 @@ def backproject(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
     img = torch.empty(n, dtype=torch.complex64, device=dev)
     mom = torch.empty((n, 3), dtype=torch.complex64, device=dev) if moment else None
     if n:
+        lib = _lib.load()
+        ws_bytes = int(lib.mmn_bp_forward_workspace_size(n, meas.shape[0], int(moment)))
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
         _lib.call("mmn_bp_forward", img.data_ptr(), _lib.ptr(mom), x_locs.data_ptr(),
                   y_locs.data_ptr(), z_locs.data_ptr(), _lib.ptr(vox_idx), n,
                   ant.data_ptr(), meas.data_ptr(), rx_off.data_ptr(), float(f0), float(df),
                   x_locs.numel(), y_locs.numel(), z_locs.numel(), meas.shape[0],
                   rx_off.shape[0], meas.shape[1], int(bool(is_ti)), VARIANTS[variant],
-                  int(K), _lib.stream_handle(dev))
+                  int(K), _lib.ptr(ws), ws_bytes, _lib.stream_handle(dev))
     if vox_idx is None:
         img = img.view(x_locs.numel(), y_locs.numel(), z_locs.numel())
         if mom is not None:
@@ -186,7 +189,7 @@ class Backproject(torch.autograd.Function):
         img = backproject(meas, ant, x_locs, y_locs, z_locs, rx_off, f0=f0, df=df,
                           variant="rec", K=K, vox_idx=vox_idx, is_ti=is_ti)
         ctx.save_for_backward(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
-        ctx.args = (f0, df, scale, min(K, 32), is_ti)
+        ctx.args = (f0, df, scale, 32 if K >= 32 else 16, is_ti)
         return img * scale
 
     @staticmethod

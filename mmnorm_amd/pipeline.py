@@ -12,10 +12,13 @@ isosurface optimisation of the antenna positions through the backward kernel.
    (sdf.py:182-191); GPU RSDF BFS.
 4. Isosurface band B = {reached voxels with |rsdf| <= tau}.  The loss is the
    negative image energy on the band, L(p) = -sum_{v in B} |I_v(p)|^2 / E0,
-   with I = backprojection / A over the band voxels only, minimised over the
-   antenna positions with Adam; every step is one forward and one backward
-   kernel launch over the band (phases recomputed, nothing stored per
-   voxel x row x sample).
+   with I = backprojection / A over the band voxels only, minimised with Adam
+   over the range (z) coordinate of every antenna position: for a planar
+   aperture the in-plane coordinates move the round-trip distance by only
+   the sine of the look angle, so an energy objective barely constrains them
+   and Adam's normalised steps would random-walk them.  Every step is one
+   forward and one backward kernel launch over the band (phases recomputed,
+   nothing stored per voxel x row x sample).
 The reference has no gradient loop (its isosurface stage is a grid search,
 isosurface_optimization.py:462-521); this loss and optimiser are defined here.
 Progress is printed per step; ``--out`` also writes a JSON record and an
@@ -54,7 +57,7 @@ def valid_components(mask, dx):
     return labeled, labels[:0]
 
 
-def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-5, K=64, out=None,
+def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=5e-6, K=64, out=None,
         ckpt_every=10, resume=False, seed=7):
     dev = torch.device("cuda", torch.cuda.current_device())
     t0 = time.perf_counter()
@@ -135,12 +138,15 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-5, K=64, out=None,
     hist = []
     pt = torch.from_numpy(p_true).to(dev)
 
-    def pos_err():
+    def pos_err(axis=None):
         d = (pos.detach() - pt)
         d = d - d.mean(0, keepdim=True)  # a common shift is not observable
+        if axis is not None:
+            return float(torch.sqrt((d[:, axis] ** 2).mean()).item())
         return float(torch.sqrt((d ** 2).sum(1).mean()).item())
 
-    log(f"start: band energy {e0:.4e}, rms position error {pos_err() * 1e3:.3f} mm")
+    log(f"start: band energy {e0:.4e}, rms position error {pos_err() * 1e3:.3f} mm "
+        f"(range axis {pos_err(2) * 1e3:.3f} mm)")
     step_times = []
     for it in range(start, steps):
         torch.cuda.synchronize()
@@ -150,13 +156,16 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-5, K=64, out=None,
                                K=K, vox_idx=bidx)
         loss = -(I.abs() ** 2).sum() / e0
         loss.backward()
+        with torch.no_grad():
+            pos.grad[:, :2] = 0  # range coordinate only (see module docstring)
         opt.step()
         torch.cuda.synchronize()
         step_times.append(time.perf_counter() - ts)
         hist.append({"step": it + 1, "loss": float(loss.item()), "rms_err_mm": pos_err() * 1e3,
-                     "s": step_times[-1]})
+                     "rms_z_err_mm": pos_err(2) * 1e3, "s": step_times[-1]})
         log(f"step {it + 1}/{steps}: loss {hist[-1]['loss']:.6f}, rms position error "
-            f"{hist[-1]['rms_err_mm']:.4f} mm, {step_times[-1]:.2f} s")
+            f"{hist[-1]['rms_err_mm']:.4f} mm (range axis {hist[-1]['rms_z_err_mm']:.4f} mm), "
+            f"{step_times[-1]:.2f} s")
         if ck and ((it + 1) % ckpt_every == 0 or it + 1 == steps):
             torch.save({"pos": pos.detach(), "opt": opt.state_dict(), "step": it + 1}, ck)
     rec["history"] = hist
@@ -176,7 +185,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--sigma", type=float, default=3e-4)
     ap.add_argument("--tau-vox", type=float, default=1.5)
-    ap.add_argument("--lr", type=float, default=2e-5)
+    ap.add_argument("--lr", type=float, default=5e-6)
     ap.add_argument("--K", type=int, default=64)
     ap.add_argument("--out", default=None)
     ap.add_argument("--ckpt-every", type=int, default=10)

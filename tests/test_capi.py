@@ -17,7 +17,8 @@ def declared():
 
 def test_header_declares_the_entry_points():
     names = declared()
-    for want in ("mmn_cuda_image", "mmn_cuda_sn_est", "mmn_bp_forward", "mmn_sn_from_moment",
+    for want in ("mmn_cuda_image", "mmn_cuda_sn_est", "mmn_bp_forward",
+                 "mmn_bp_forward_workspace_size", "mmn_sn_from_moment",
                  "mmn_bp_backward", "mmn_bp_backward_workspace_size", "mmn_rsdf_bfs",
                  "mmn_rsdf_bfs_workspace_size", "mmn_abi_version", "mmn_last_error"):
         assert want in names
@@ -42,16 +43,18 @@ def test_workspace_queries_and_argument_errors_without_gpu():
     assert lib.mmn_rsdf_bfs_workspace_size(4, 4, 4) > 4 * 4 * 4 * 12
     assert lib.mmn_rsdf_bfs_workspace_size(0, 4, 4) == 0
     assert lib.mmn_bp_backward_workspace_size(16, 32, 100) > 16 * 32 * 8
+    assert lib.mmn_bp_forward_workspace_size(1000, 3200, 1) == 32 * 1000 * 32
+    assert lib.mmn_bp_forward_workspace_size(1000, 20, 0) == 0
     # argument checks run before any HIP call
     with pytest.raises(ValueError, match="NULL"):
         _lib.call("mmn_bp_forward", None, None, None, None, None, None, 0, None, None, None,
-                  77e9, 1e6, 1, 1, 1, 1, 1, 1, 0, 2, 32, None)
+                  77e9, 1e6, 1, 1, 1, 1, 1, 1, 0, 2, 32, None, 0, None)
     with pytest.raises(ValueError, match="variant"):
         _lib.call("mmn_bp_forward", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 1e6, 1, 1, 1,
-                  1, 1, 1, 0, 7, 32, None)
+                  1, 1, 1, 0, 7, 32, None, 0, None)
     with pytest.raises(ValueError, match="K must be"):
         _lib.call("mmn_bp_forward", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 1e6, 1, 1, 1,
-                  1, 1, 1, 0, 2, 12, None)
+                  1, 1, 1, 0, 2, 12, None, 0, None)
     with pytest.raises(ValueError, match="workspace"):
         _lib.call("mmn_rsdf_bfs", 8, 8, 8, 8, 8, 1, 1e-3, 4, 4, 4, None, 0, None)
     with pytest.raises(ValueError, match="sizes"):
