@@ -10,17 +10,15 @@
 //              z_ijv = g_v conj(phi_ijv),  k_j = 2 pi f_j / c,
 //              grad_p d = -(v-p)/|v-p| - (v-p-o)/|v-p-o|.
 // Both reduce over voxels.  A wave owns one (row, block of K samples, voxel
-// chunk): its K coefficients k_j s_ij are wave-uniform (LDS, broadcast
-// reads), its lanes walk the chunk's voxels with coalesced record loads and
-// keep lane-local sums, and one butterfly across the wave reduces them at the
-// end.  Per (lane, voxel) the K conjugate phasors come from one FP64-reduced
-// anchor and a two-term FP32 recurrence; per sample the lane does the
-// recurrence (2 FMA), one accumulate (2 add) and the position term (2 FMA).
-// Each wave writes its partial sums to the workspace; a second kernel
-// reduces the partials in a fixed order (FP64), so results are bitwise
-// reproducible.
-#include <stdlib.h>
-
+// chunk): its lanes walk the chunk's voxels with coalesced record loads and
+// keep lane-local sums, which one butterfly across the wave reduces at the
+// end.  The block's K position-term coefficients are wave-uniform and come
+// through scalar loads as SGPR operands.  Per (lane, voxel) the K conjugate
+// phasors come from one FP64-reduced anchor and a two-term FP32 recurrence;
+// per sample the lane does the recurrence (2 FMA), one accumulate (2 add)
+// and the position term (2 FMA).  Each wave writes its partial sums to the
+// workspace; a second kernel reduces the partials in a fixed order (FP64),
+// so results are bitwise reproducible.
 #include "mmn_internal.h"
 
 namespace {
@@ -28,65 +26,63 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kVV = 2;                 // voxels per lane per step
+constexpr int kPad = kWave * kVV;      // voxel records are padded to this
+constexpr int kMinK = 16;              // smallest sample block (workspace sizing)
+constexpr int kMaxK = 32;
 
-// Per-voxel record of the backward: position and Re(g) in one 16-byte load,
-// Im(g) in a second array (20 bytes per voxel in all).
+// Per-voxel record: position and Re(g) in one 16-byte load, Im(g) in a
+// second array (20 bytes per voxel in all).  Records past the voxel count
+// (up to a multiple of kPad) repeat the last voxel's position with g = 0.
 struct VoxRec {
   float4 p;  // vx, vy, vz, Re g
   float gi;  // Im g
 };
 
-struct BwdPlan {
-  int sub;     // voxel sub-ranges (one per wave)
-  int nfb;     // sample blocks of this launch
-  int nfbmax;  // sample blocks the workspace is sized for (K = 16)
-  int64_t chunk;
-};
+int64_t pad_vox(int64_t nvox) { return (nvox + kPad - 1) / kPad * kPad; }
 
-// Voxel sub-ranges (one per wave) the workspace is sized for: at least ~64
-// voxels per wave, at most 1024 sub-ranges and ~1 GiB of partial sums.
-int sub_max(int A, int F, int64_t nvox) {
+// Voxel chunks (one per wave along the voxel axis) the workspace is sized
+// for: at least ~64 voxels per chunk, at most 1024 chunks and ~1 GiB of
+// partial sums.
+int chunk_max(int A, int F, int64_t nvox) {
   int64_t m = (nvox + 63) / 64;
   const int64_t by_ws = ((int64_t)1 << 30) / ((int64_t)A * F * 8);
   if (m > by_ws) m = by_ws;
   if (m > 1024) m = 1024;
-  m = m / kWavesPerBlock * kWavesPerBlock;
-  return (int)(m < kWavesPerBlock ? kWavesPerBlock : m);
+  return (int)(m < 1 ? 1 : m);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int env_int(const char *name, int dflt) {
-  const char *v = getenv(name);
-  return (v && *v) ? atoi(v) : dflt;
-}
-
 struct BwdWs {
-  float4 *rec4;   // [nvox]
-  float *recg;    // [nvox]
-  float2 *gs;     // [sub][A][F]
-  float *gp;      // [sub][nfbmax][A][3]
+  float4 *rec4;   // [pad_vox]
+  float *recg;    // [pad_vox]
+  float2 *gs;     // [chunks][A][F]           adjoint partials
+  float *gp;      // [chunks][nfbmax][A][3]   position partials
+  float2 *coef;   // [A][nfb*K]               position-term coefficients
 };
 
 size_t bwd_layout(int A, int F, int64_t nvox, BwdWs *w, char *base) {
-  BwdPlan p;
-  p.sub = sub_max(A, F, nvox);
-  p.nfbmax = (F + 15) / 16;
+  const int64_t nv = pad_vox(nvox);
+  const int sub = chunk_max(A, F, nvox);
+  const int nfbmax = (F + kMinK - 1) / kMinK;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     char *q = base ? base + off : nullptr;
     off += align256(bytes);
     return q;
   };
-  char *r4 = take(sizeof(float4) * (size_t)nvox);
-  char *rg = take(sizeof(float) * (size_t)nvox);
-  char *s = take(sizeof(float2) * (size_t)p.sub * A * F);
-  char *g = take(sizeof(float) * (size_t)p.sub * p.nfbmax * A * 3);
+  char *r4 = take(sizeof(float4) * (size_t)nv);
+  char *rg = take(sizeof(float) * (size_t)nv);
+  char *s = take(sizeof(float2) * (size_t)sub * A * F);
+  char *g = take(sizeof(float) * (size_t)sub * nfbmax * A * 3);
+  char *cf = take(sizeof(float2) * (size_t)A * (F + kMaxK));
   if (w) {
     w->rec4 = (float4 *)r4;
     w->recg = (float *)rg;
     w->gs = (float2 *)s;
     w->gp = (float *)g;
+    w->coef = (float2 *)cf;
   }
   return off;
 }
@@ -95,18 +91,38 @@ __global__ __launch_bounds__(kBlock) void k_pack(
     float4 *__restrict__ rec4, float *__restrict__ recg, const float2 *__restrict__ g,
     const float *__restrict__ xl, const float *__restrict__ yl,
     const float *__restrict__ zl, const int32_t *__restrict__ vidx,
-    int64_t nvox, int ny, int nz) {
+    int64_t nvox, int64_t npad, int ny, int nz) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nvox) return;
-  const int64_t lin = vidx ? (int64_t)vidx[t] : t;
+  if (t >= npad) return;
+  const bool real = t < nvox;
+  const int64_t src = real ? t : nvox - 1;
+  const int64_t lin = vidx ? (int64_t)vidx[src] : src;
   float vx, vy, vz;
   voxel_coords(xl, yl, zl, ny, nz, lin, vx, vy, vz);
-  const float2 gv = g[t];
+  const float2 gv = real ? g[t] : make_float2(0.0f, 0.0f);
   rec4[t] = make_float4(vx, vy, vz, gv.x);
   recg[t] = gv.y;
 }
 
-// Per (lane row a, voxel v) state of one sample block: the conjugate phasors
+// Position-term coefficients (k_j Im s_ij, -k_j Re s_ij) of every (row,
+// sample); rows hold nfb * K samples, zero past F.
+__global__ __launch_bounds__(kBlock) void k_coef(float2 *__restrict__ coef,
+                                                 const float2 *__restrict__ meas,
+                                                 float k0, float dk, int F, int Fs,
+                                                 int64_t AFs) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= AFs) return;
+  const int a = (int)(t / Fs), j = (int)(t % Fs);
+  float2 c = make_float2(0.0f, 0.0f);
+  if (j < F) {
+    const float2 sv = meas[(size_t)a * F + j];
+    const float kj = fmaf((float)j, dk, k0);
+    c = make_float2(kj * sv.y, -(kj * sv.x));
+  }
+  coef[t] = c;
+}
+
+// Per (row a, voxel v) state of one sample block: the conjugate phasors
 // z_t = g_v exp(+j 2 pi f_t d_av / c) for t = 0..K-1 satisfy, because the
 // step exp(+j w) has modulus one, the real-coefficient recurrence
 //     z_{t+1} = 2 cos(w) z_t - z_{t-1}
@@ -117,98 +133,89 @@ __global__ __launch_bounds__(kBlock) void k_pack(
 // 32-sample block.
 struct VoxStart {
   float z0r, z0i, z1r, z1i, c2;  // c2 = 2 cos(w)
-  float gx, gy, gz;              // grad_p d
+  float gx, gy, gz;              // grad_p d (sign applied by the caller)
 };
 
-__device__ __forceinline__ VoxStart vox_start(const VoxRec &r, double pxd, double pyd,
-                                              double pzd, float ox, float oy, float oz,
+__device__ __forceinline__ VoxStart vox_start(const VoxRec &r, float px, float py,
+                                              float pz, float ox, float oy, float oz,
                                               bool mono, int is_ti, double fc,
-                                              double dfc, float weight) {
+                                              float dfc) {
   VoxStart v;
-  // round-trip distance in FP64 (as roundtrip_dist64), keeping the first
-  // leg's difference vector and reciprocal length for grad_p d
-  const double dx = (double)r.p.x - pxd, dy = (double)r.p.y - pyd, dz = (double)r.p.z - pzd;
+  // round-trip distance in FP64 (as roundtrip_dist64)
+  const double dx = (double)r.p.x - (double)px, dy = (double)r.p.y - (double)py,
+               dz = (double)r.p.z - (double)pz;
   const double s1 = fma(dz, dz, fma(dy, dy, dx * dx));
   const float y1 = __builtin_amdgcn_rsqf((float)s1);
   const double y1d = (double)y1;
   const double d0 = s1 * y1d;
   const double d1 = fma(0.5 * y1d, fma(-d0, d0, s1), d0);
-  const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+  // first-leg difference in FP32: the correctly rounded FP32 difference is
+  // the exact (FP64) one rounded once
+  const float fx = r.p.x - px, fy = r.p.y - py, fz = r.p.z - pz;
   double d;
-  float gx, gy, gz;
   if (mono) {
     d = d1 + d1;
     const float y2 = y1 + y1;
-    gx = fx * y2;
-    gy = fy * y2;
-    gz = fz * y2;
+    v.gx = fx * y2;
+    v.gy = fy * y2;
+    v.gz = fz * y2;
   } else {
     const double ex = dx - (double)ox, ey = dy - (double)oy, ez = dz - (double)oz;
     d = d1 + sqrt_nr64(fma(ez, ez, fma(ey, ey, ex * ex)));
     const float bx = (float)ex, by = (float)ey, bz = (float)ez;
     const float n2 = rsqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
-    gx = fmaf(bx, n2, fx * y1);
-    gy = fmaf(by, n2, fy * y1);
-    gz = fmaf(bz, n2, fz * y1);
+    v.gx = fmaf(bx, n2, fx * y1);
+    v.gy = fmaf(by, n2, fy * y1);
+    v.gz = fmaf(bz, n2, fz * y1);
   }
   if (is_ti) d += 0.15;
-  float ar, ai, rr, ri;
+  float ar, ai;
   phasor_hw(d * fc, ar, ai);  // exp(-j ...); conjugated below
-  phasor_hw(d * dfc, rr, ri);
+  // the step phase df d / c is a few hundredths of a turn: FP32 keeps it to
+  // ~1e-9 turns, and a block's K steps to ~1e-7
+  const float ts = (float)d * dfc;
+  const float fr = ts - rintf(ts);
+  const float rr = __builtin_amdgcn_cosf(fr);
+  const float ri = __builtin_amdgcn_sinf(fr);  // exp(+j w)
   ai = -ai;
-  ri = -ri;
-  const float gr = r.p.w * weight, gi = r.gi * weight;
+  const float gr = r.p.w, gi = r.gi;
   v.z0r = gr * ar - gi * ai;
   v.z0i = gr * ai + gi * ar;
   v.z1r = v.z0r * rr - v.z0i * ri;
   v.z1i = v.z0r * ri + v.z0i * rr;
   v.c2 = rr + rr;
-  // d(d)/dp = -(unit vectors): the sign is applied with the position term.
-  v.gx = gx;
-  v.gy = gy;
-  v.gz = gz;
   return v;
 }
 
-// Transposed mapping: one wave per (row a, sample block fb, voxel chunk c);
-// its lanes walk different voxels of the chunk (coalesced record loads) and
-// keep lane-local sums, which one butterfly across the wave reduces at the
-// end.  The block's K coefficients k_t s_{a,t} are wave-uniform and sit in
-// LDS (broadcast reads), so a lane holds only its K accumulators.
-template <int K, int VV>
-__global__ __launch_bounds__(kBlock) void k_bwd_t(
+// One wave per (row a, sample block fb, voxel chunk c).  At 4 waves per SIMD
+// the K = 32 kernel fits in 128 VGPRs (64 of them the adjoint accumulators);
+// the register cap costs a few spills in the epilogue only, and the extra
+// wave per SIMD is worth ~10 % (measured against 3 waves at 148 VGPRs).
+template <int K>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bwd(
     float2 *__restrict__ gs, float *__restrict__ gp,
-    const float4 *__restrict__ rec4, const float *__restrict__ recg, int64_t nvox,
-    int64_t chunk, const float *__restrict__ ant, const float2 *__restrict__ meas,
-    const float *__restrict__ rxo, double f0c, double dfc, float k0, float dk, int A,
-    int R, int F, int is_ti, int nfbmax, int nfb, int64_t nitems) {
-  __shared__ float2 sp_all[kWavesPerBlock][K];
+    const float4 *__restrict__ rec4, const float *__restrict__ recg, int nvp,
+    int chunk, const float *__restrict__ ant, const float2 *__restrict__ coef,
+    const float *__restrict__ rxo, double f0c, float dfc32, double dfc, int A, int R,
+    int F, int is_ti, int nfbmax, int nfb, int64_t nitems) {
+  static_assert(K % 8 == 0, "coefficients are read 8 at a time");
   const int lane = threadIdx.x % kWave;
-  const int wv = threadIdx.x / kWave;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + wv;
   const bool live = item < nitems;
   const int64_t it = live ? item : nitems - 1;
   const int a = (int)(it % A);
   const int64_t rest = it / A;
   const int fb = (int)(rest % nfb);
-  const int64_t c = rest / nfb;
+  const int c = (int)(rest / nfb);
   const int f_lo = fb * K;
   const int nf = min(K, F - f_lo);
-
-  float2 *sp = sp_all[wv];
-  if (lane < K) {
-    float2 sv = make_float2(0.0f, 0.0f);
-    if (lane < nf) sv = meas[(size_t)a * F + f_lo + lane];
-    const float kj = fmaf((float)(f_lo + lane), dk, k0);
-    sp[lane] = make_float2(kj * sv.x, kj * sv.y);
-  }
-  __syncthreads();
+  const float2 *__restrict__ cf = coef + ((size_t)a * nfb + fb) * K;
 
   const float px = ant[3 * a + 0], py = ant[3 * a + 1], pz = ant[3 * a + 2];
   const int rx = a % R;
   const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
   const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
-  const double pxd = px, pyd = py, pzd = pz;
   const double fc = fma((double)f_lo, dfc, f0c);
 
   float accr[K], acci[K];
@@ -216,56 +223,57 @@ __global__ __launch_bounds__(kBlock) void k_bwd_t(
   for (int t = 0; t < K; ++t) accr[t] = acci[t] = 0.0f;
   float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
 
-  const int64_t v0 = c * chunk;
-  const int64_t v1 = min(nvox, v0 + chunk);
-  // lane's voxels: v0 + lane + 64 * (VV * j + u)
-  for (int64_t vb = v0 + lane; vb < v1; vb += (int64_t)kWave * VV) {
-    // keeps the coefficient reads below inside the loop (re-read from LDS each
-    // group) instead of hoisted into K more registers per lane
-    asm volatile("" ::: "memory");
-    VoxStart p[VV];
+  // chunk and the padded voxel count are multiples of kPad, so every voxel
+  // vb + 64 u below is a real or padding record
+  const int v0 = c * chunk;
+  const int v1 = v0 + min(chunk, nvp - v0);
+  for (int vb = v0 + lane; vb < v1; vb += kPad) {
+    VoxStart p[kVV];
 #pragma unroll
-    for (int u = 0; u < VV; ++u) {
-      const int64_t vu = vb + (int64_t)kWave * u;
-      const bool ok = vu < v1;
+    for (int u = 0; u < kVV; ++u) {
       VoxRec r;
-      const int64_t vs = ok ? vu : vb;
-      r.p = rec4[vs];
-      r.gi = recg[vs];
-      p[u] = vox_start(r, pxd, pyd, pzd, ox, oy, oz, mono, is_ti, fc, dfc, ok ? 1.0f : 0.0f);
+      r.p = rec4[vb + kWave * u];
+      r.gi = recg[vb + kWave * u];
+      p[u] = vox_start(r, px, py, pz, ox, oy, oz, mono, is_ti, fc, dfc32);
     }
-    float r0[VV], i0[VV], r1[VV], i1[VV], t1[VV], t2[VV];
+    float r0[kVV], i0[kVV], r1[kVV], i1[kVV], tt[kVV];
 #pragma unroll
-    for (int u = 0; u < VV; ++u) {
+    for (int u = 0; u < kVV; ++u) {
       r0[u] = p[u].z0r; i0[u] = p[u].z0i; r1[u] = p[u].z1r; i1[u] = p[u].z1i;
-      t1[u] = t2[u] = 0.0f;
+      tt[u] = 0.0f;
     }
+    float2 sp[8];
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-      const float2 s2 = sp[t];
+      if (t % 8 == 0) {
+        // 8 coefficients at a time through scalar loads (cache hits), so
+        // that K complex values never sit in scalar registers at once
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sp[q] = cf[t + q];
+      }
       float sr = r0[0], si = i0[0];
 #pragma unroll
-      for (int u = 1; u < VV; ++u) {
+      for (int u = 1; u < kVV; ++u) {
         sr += r0[u];
         si += i0[u];
       }
       accr[t] += sr;
       acci[t] += si;
 #pragma unroll
-      for (int u = 0; u < VV; ++u) {
-        t1[u] = fmaf(s2.y, r0[u], t1[u]);
-        t2[u] = fmaf(s2.x, i0[u], t2[u]);
+      for (int u = 0; u < kVV; ++u) {
+        tt[u] = fmaf(sp[t % 8].x, r0[u], tt[u]);
+        tt[u] = fmaf(sp[t % 8].y, i0[u], tt[u]);
         const float r2 = fmaf(p[u].c2, r1[u], -r0[u]);
         const float i2 = fmaf(p[u].c2, i1[u], -i0[u]);
         r0[u] = r1[u]; i0[u] = i1[u]; r1[u] = r2; i1[u] = i2;
       }
     }
 #pragma unroll
-    for (int u = 0; u < VV; ++u) {
-      const float tt = t1[u] - t2[u];
-      gpx = fmaf(-tt, p[u].gx, gpx);
-      gpy = fmaf(-tt, p[u].gy, gpy);
-      gpz = fmaf(-tt, p[u].gz, gpz);
+    for (int u = 0; u < kVV; ++u) {
+      gpx = fmaf(-tt[u], p[u].gx, gpx);
+      gpy = fmaf(-tt[u], p[u].gy, gpy);
+      gpz = fmaf(-tt[u], p[u].gz, gpz);
     }
   }
   // butterfly over the wave: every lane ends with the wave's totals
@@ -343,10 +351,12 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
                   samples > 0,
               MMN_ERR_ARG, "mmn_bp_backward: sizes must be positive");
   if (K == 0) K = 32;
-  MMN_REQUIRE(K == 16 || K == 32 || K == 48, MMN_ERR_ARG,
-              "mmn_bp_backward: K must be 16, 32 or 48 (got %d)", K);
+  MMN_REQUIRE(K == 16 || K == 32, MMN_ERR_ARG,
+              "mmn_bp_backward: K must be 16 or 32 (got %d)", K);
   const int64_t nvox = vox_idx ? n_vox : (int64_t)nx * ny * nz;
   MMN_REQUIRE(nvox >= 0, MMN_ERR_ARG, "mmn_bp_backward: n_vox < 0");
+  MMN_REQUIRE(pad_vox(nvox) <= INT32_MAX, MMN_ERR_ARG,
+              "mmn_bp_backward: more than 2^31 - 129 voxels");
   hipStream_t st = (hipStream_t)stream;
   const int A = num_rows, F = samples;
   if (nvox == 0 || (!grad_meas && !grad_ant)) {
@@ -364,54 +374,55 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
               "mmn_bp_backward: workspace of %zu bytes needed", need);
   BwdWs w;
   bwd_layout(A, F, nvox, &w, (char *)workspace);
-  const int vv = env_int("MMN_BWD_VV", 2);
-  // transposed mapping: items = rows x blocks x voxel chunks
-  BwdPlan p;
-  p.nfb = (F + K - 1) / K;
-  p.nfbmax = (F + 15) / 16;
-  void (*kt)(float2 *, float *, const float4 *, const float *, int64_t, int64_t,
-             const float *, const float2 *, const float *, double, double, float, float,
-             int, int, int, int, int, int, int64_t);
-  if (K == 16) kt = vv == 1 ? k_bwd_t<16, 1> : k_bwd_t<16, 2>;
-  else if (K == 48) kt = vv == 2 ? k_bwd_t<48, 2> : k_bwd_t<48, 1>;
-  else kt = vv == 1 ? k_bwd_t<32, 1> : k_bwd_t<32, 2>;
+  const int64_t npad = pad_vox(nvox);
+  const int nfb = (F + K - 1) / K;
+  const int nfbmax = (F + kMinK - 1) / kMinK;
+  void (*kern)(float2 *, float *, const float4 *, const float *, int, int, const float *,
+               const float2 *, const float *, double, float, double, int, int, int, int,
+               int, int, int64_t) = K == 16 ? k_bwd<16> : k_bwd<32>;
   int per_cu = 0, dev = 0, cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kt, kBlock, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   (void)hipGetLastError();
-  // chunks: enough waves for 8 rounds of resident slots, at least 256
-  // voxels per wave, at most the workspace's sub-range count
+  // voxel chunks: enough waves for 8 rounds of resident slots, at least 256
+  // voxels per wave, at most the workspace's chunk count
   const int64_t slots_waves = (int64_t)per_cu * cus * kWavesPerBlock;
-  const int64_t pairs = (int64_t)A * p.nfb;
+  const int64_t pairs = (int64_t)A * nfb;
   int64_t ch = (8 * slots_waves + pairs - 1) / pairs;
   const int64_t by_vox = (nvox + 255) / 256;
   if (ch > by_vox) ch = by_vox;
-  const int smax = sub_max(A, F, nvox);
-  if (ch > smax) ch = smax;
+  const int cmax = chunk_max(A, F, nvox);
+  if (ch > cmax) ch = cmax;
   if (ch < 1) ch = 1;
-  p.sub = (int)ch;
-  p.chunk = (nvox + ch - 1) / ch;
+  // chunk length a multiple of kPad, then the chunk count it implies (never
+  // more than before, so the workspace still holds the partials)
+  int64_t chunk = (npad + ch - 1) / ch;
+  chunk = (chunk + kPad - 1) / kPad * kPad;
+  ch = (npad + chunk - 1) / chunk;
   const int64_t nitems = pairs * ch;
-  hipLaunchKernelGGL(k_pack, dim3(mmn_blocks(nvox, kBlock)), dim3(kBlock), 0,
-                     st, w.rec4, w.recg, (const float2 *)g, x_locs, y_locs, z_locs,
-                     vox_idx, nvox, ny, nz);
+  hipLaunchKernelGGL(k_pack, dim3(mmn_blocks(npad, kBlock)), dim3(kBlock), 0, st, w.rec4,
+                     w.recg, (const float2 *)g, x_locs, y_locs, z_locs, vox_idx, nvox, npad,
+                     ny, nz);
   const double inv_c = 1.0 / MMN_C0;
   const double two_pi = 6.283185307179586476925286766559;
-  hipLaunchKernelGGL(kt, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
-                     dim3(kBlock), 0, st, w.gs, w.gp, w.rec4, w.recg, nvox, p.chunk, ant,
-                     (const float2 *)meas, rx_off, f0 * inv_c, df * inv_c,
-                     (float)(two_pi * f0 * inv_c), (float)(two_pi * df * inv_c), A,
-                     num_rx, F, is_ti_radar, p.nfbmax, p.nfb, nitems);
+  const float k0 = (float)(two_pi * f0 * inv_c), dk = (float)(two_pi * df * inv_c);
+  const int Fs = nfb * K;
+  hipLaunchKernelGGL(k_coef, dim3(mmn_blocks((int64_t)A * Fs, kBlock)), dim3(kBlock), 0, st,
+                     w.coef, (const float2 *)meas, k0, dk, F, Fs, (int64_t)A * Fs);
+  hipLaunchKernelGGL(kern, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
+                     dim3(kBlock), 0, st, w.gs, w.gp, w.rec4, w.recg, (int)npad, (int)chunk,
+                     ant, (const float2 *)w.coef, rx_off, f0 * inv_c, (float)(df * inv_c),
+                     df * inv_c, A, num_rx, F, is_ti_radar, nfbmax, nfb, nitems);
   if (grad_meas)
     hipLaunchKernelGGL(k_reduce_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)),
-                       dim3(kBlock), 0, st, (float2 *)grad_meas, w.gs, p.sub,
+                       dim3(kBlock), 0, st, (float2 *)grad_meas, w.gs, (int)ch,
                        (int64_t)A * F, scale);
   if (grad_ant)
     hipLaunchKernelGGL(k_reduce_ant, dim3(mmn_blocks((int64_t)A * 3, kBlock)),
-                       dim3(kBlock), 0, st, grad_ant, w.gp, p.sub, p.nfb,
-                       p.nfbmax, A, scale);
+                       dim3(kBlock), 0, st, grad_ant, w.gp, (int)ch, nfb, nfbmax, A,
+                       scale);
   return mmn_check_launch("mmn_bp_backward");
 }

@@ -55,6 +55,12 @@ def test_workspace_queries_and_argument_errors_without_gpu():
     with pytest.raises(ValueError, match="K must be"):
         _lib.call("mmn_bp_forward", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 1e6, 1, 1, 1,
                   1, 1, 1, 0, 2, 12, None, 0, None)
+    with pytest.raises(ValueError, match="K must be 16 or 32"):
+        _lib.call("mmn_bp_backward", 8, 8, 8, 1.0, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 1e6,
+                  1, 1, 1, 1, 1, 1, 0, 48, None, 0, None)
+    with pytest.raises(ValueError, match="workspace"):
+        _lib.call("mmn_bp_backward", 8, 8, 8, 1.0, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 1e6,
+                  1, 1, 1, 1, 1, 1, 0, 32, None, 0, None)
     with pytest.raises(ValueError, match="workspace"):
         _lib.call("mmn_rsdf_bfs", 8, 8, 8, 8, 8, 1, 1e-3, 4, 4, 4, None, 0, None)
     with pytest.raises(ValueError, match="sizes"):

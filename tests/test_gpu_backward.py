@@ -163,3 +163,58 @@ def test_backward_deterministic():
     r1 = ops.backproject_backward(g, meas, a, xl, yl, zl, f0=sc.f0, df=sc.df)
     r2 = ops.backproject_backward(g, meas, a, xl, yl, zl, f0=sc.f0, df=sc.df)
     assert torch.equal(r1[0], r2[0]) and torch.equal(r1[1], r2[1])
+
+
+def _backward64(g, meas, ant, rxo, v, f0, df, scale, is_ti):
+    """FP64 NumPy backward of I = scale * sum_ij s_ij exp(-j k_j d_iv), bistatic:
+    d_iv = |v - p_i| + |v - p_i - o_i| (+ 0.15 m), k_j = 2 pi (f0 + j df) / c."""
+    A, F = meas.shape
+    k = 2 * np.pi * (f0 + df * np.arange(F)) / C0
+    o = rxo[np.arange(A) % len(rxo)]
+    e1 = v[None, :, :] - ant[:, None, :]                     # (A, V, 3)
+    e2 = e1 - o[:, None, :]
+    n1 = np.linalg.norm(e1, axis=2)
+    n2 = np.linalg.norm(e2, axis=2)
+    d = n1 + n2 + (0.15 if is_ti else 0.0)
+    ph = np.exp(-1j * d[..., None] * k)                      # (A, V, F)
+    gm = scale * np.einsum("v,avf->af", g, np.conj(ph))
+    Q = np.einsum("af,avf->av", k[None, :] * meas, ph)
+    w = np.imag(np.conj(g)[None, :] * Q)                     # (A, V)
+    grad_d = -(e1 / n1[..., None] + e2 / n2[..., None])      # d(d)/dp
+    ga = scale * np.einsum("av,avk->ak", w, grad_d)
+    return gm, ga
+
+
+@pytest.mark.parametrize("n_list", [1, 77, 129, None])
+def test_backward_ragged_bistatic(n_list):
+    """Voxel counts that are not multiples of the kernel's 128-voxel padding,
+    a sample count (40) that leaves a short last block, two RX offsets and the
+    TI range offset, against an FP64 NumPy backward."""
+    sc = small_case(seed=5, n_ap=6, F=40, grid=(9, 7, 5))
+    rng = np.random.default_rng(11)
+    rxo = np.array([[0.0, 0.0, 0.0], [4e-3, -1e-3, 0.0]])
+    meas = np.repeat(sc.meas, 2, axis=0)[: 2 * len(sc.ant)]
+    meas = meas * np.exp(1j * rng.uniform(0, 2 * np.pi, size=meas.shape))
+    ant = np.repeat(sc.ant, 2, axis=0).astype(np.float32).astype(np.float64)
+    X, Y, Z = np.meshgrid(sc.x_locs.astype(np.float32).astype(np.float64),
+                          sc.y_locs.astype(np.float32).astype(np.float64),
+                          sc.z_locs.astype(np.float32).astype(np.float64), indexing="ij")
+    vall = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    if n_list is None:
+        idx, v = None, vall
+    else:
+        sel = np.sort(rng.choice(len(vall), size=n_list, replace=False)).astype(np.int32)
+        idx, v = T(sel, np.int32), vall[sel]
+    g = rng.normal(size=len(v)) + 1j * rng.normal(size=len(v))
+    scale = 1.0 / len(ant)
+    xl, yl, zl = (T(q, np.float32) for q in (sc.x_locs, sc.y_locs, sc.z_locs))
+    gm, ga = ops.backproject_backward(T(g, np.complex64), T(meas, np.complex64),
+                                      T(ant, np.float32), xl, yl, zl, T(rxo, np.float32),
+                                      f0=sc.f0, df=sc.df, scale=scale, vox_idx=idx,
+                                      is_ti=True)
+    gm64, ga64 = _backward64(g, meas.astype(np.complex64).astype(np.complex128), ant,
+                             rxo.astype(np.float32).astype(np.float64), v, sc.f0, sc.df,
+                             scale, True)
+    gm, ga = gm.cpu().numpy().astype(np.complex128), ga.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(gm - gm64) <= 1e-4 * np.linalg.norm(gm64)
+    assert np.linalg.norm(ga - ga64) <= 1e-3 * np.linalg.norm(ga64)
