@@ -19,6 +19,12 @@ isosurface optimisation of the antenna positions through the backward kernel.
    and Adam's normalised steps would random-walk them.  Every step is one
    forward and one backward kernel launch over the band (phases recomputed,
    nothing stored per voxel x row x sample).
+   ``objective="sharpness"`` maximises sum |I|^4 / (sum |I|^2)^2 instead.
+   With one free range per antenna both objectives can be pushed past their
+   value at the true positions (noise is focused too), so the step size is
+   kept small: at config 5 the energy objective with lr = 2e-6 lowers the
+   range error steadily (0.301 -> 0.286 mm in 50 steps) while larger steps
+   or the sharpness objective overshoot.
 The reference has no gradient loop (its isosurface stage is a grid search,
 isosurface_optimization.py:462-521); this loss and optimiser are defined here.
 Progress is printed per step; ``--out`` also writes a JSON record and an
@@ -57,8 +63,17 @@ def valid_components(mask, dx):
     return labeled, labels[:0]
 
 
-def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=5e-6, K=64, out=None,
-        ckpt_every=10, resume=False, seed=7):
+def _objective(I, kind):
+    p = I.abs() ** 2
+    if kind == "energy":
+        return p.sum()
+    if kind == "sharpness":  # sum |I|^4 / (sum |I|^2)^2: scale-free image sharpness
+        return (p * p).sum() / p.sum() ** 2
+    raise ValueError(f"unknown objective {kind!r}")
+
+
+def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
+        ckpt_every=10, resume=False, seed=7, objective="energy"):
     dev = torch.device("cuda", torch.cuda.current_device())
     t0 = time.perf_counter()
     sc = synth.make_scene(cid, device=dev)
@@ -132,9 +147,12 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=5e-6, K=64, out=None,
         opt.load_state_dict(state["opt"])
         start = int(state["step"])
         log(f"resumed at step {start}")
-    with torch.no_grad():
-        e0 = (ops.backproject(meas, pos.detach(), xl, yl, zl, f0=sc.f0, df=sc.df, K=K,
-                              vox_idx=bidx).abs() ** 2).sum().item() / A ** 2
+    def objective_at(p):
+        with torch.no_grad():
+            return _objective(ops.backproject(meas, p, xl, yl, zl, f0=sc.f0, df=sc.df, K=K,
+                                              vox_idx=bidx) / A, objective).item()
+
+    e0 = objective_at(pos.detach())
     hist = []
     pt = torch.from_numpy(p_true).to(dev)
 
@@ -145,8 +163,23 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=5e-6, K=64, out=None,
             return float(torch.sqrt((d[:, axis] ** 2).mean()).item())
         return float(torch.sqrt((d ** 2).sum(1).mean()).item())
 
-    log(f"start: band energy {e0:.4e}, rms position error {pos_err() * 1e3:.3f} mm "
-        f"(range axis {pos_err(2) * 1e3:.3f} mm)")
+    # a range error that is affine over the aperture plane only shifts the
+    # image, which an energy objective cannot see: report the residual too
+    Xa = torch.stack([pt[:, 0], pt[:, 1], torch.ones_like(pt[:, 0])], 1).double()
+
+    def range_err_affine_removed():
+        d = (pos.detach() - pt)[:, 2].double()
+        fit = torch.linalg.lstsq(Xa, d[:, None]).solution
+        return float(torch.sqrt(((d - (Xa @ fit)[:, 0]) ** 2).mean()).item())
+
+    e_true = objective_at(pt)
+    rec["objective"] = objective
+    rec["objective_true_over_start"] = e_true / e0
+    rec["start_rms_z_err_mm"] = pos_err(2) * 1e3
+    rec["start_rms_z_err_affine_removed_mm"] = range_err_affine_removed() * 1e3
+    log(f"start: band {objective} {e0:.4e} (true positions: {e_true / e0:.4f} x), rms position "
+        f"error {pos_err() * 1e3:.3f} mm (range axis {pos_err(2) * 1e3:.3f} mm, "
+        f"{rec['start_rms_z_err_affine_removed_mm']:.3f} mm without its affine part)")
     step_times = []
     for it in range(start, steps):
         torch.cuda.synchronize()
@@ -154,7 +187,7 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=5e-6, K=64, out=None,
         opt.zero_grad(set_to_none=True)
         I = ops.backproject_ad(meas, pos, xl, yl, zl, f0=sc.f0, df=sc.df, scale=1.0 / A,
                                K=K, vox_idx=bidx)
-        loss = -(I.abs() ** 2).sum() / e0
+        loss = -_objective(I, objective) / e0
         loss.backward()
         with torch.no_grad():
             pos.grad[:, :2] = 0  # range coordinate only (see module docstring)
@@ -162,9 +195,12 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=5e-6, K=64, out=None,
         torch.cuda.synchronize()
         step_times.append(time.perf_counter() - ts)
         hist.append({"step": it + 1, "loss": float(loss.item()), "rms_err_mm": pos_err() * 1e3,
-                     "rms_z_err_mm": pos_err(2) * 1e3, "s": step_times[-1]})
+                     "rms_z_err_mm": pos_err(2) * 1e3,
+                     "rms_z_err_affine_removed_mm": range_err_affine_removed() * 1e3,
+                     "s": step_times[-1]})
         log(f"step {it + 1}/{steps}: loss {hist[-1]['loss']:.6f}, rms position error "
-            f"{hist[-1]['rms_err_mm']:.4f} mm (range axis {hist[-1]['rms_z_err_mm']:.4f} mm), "
+            f"{hist[-1]['rms_err_mm']:.4f} mm (range axis {hist[-1]['rms_z_err_mm']:.4f} mm, "
+            f"{hist[-1]['rms_z_err_affine_removed_mm']:.4f} mm non-affine), "
             f"{step_times[-1]:.2f} s")
         if ck and ((it + 1) % ckpt_every == 0 or it + 1 == steps):
             torch.save({"pos": pos.detach(), "opt": opt.state_dict(), "step": it + 1}, ck)
@@ -185,13 +221,15 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--sigma", type=float, default=3e-4)
     ap.add_argument("--tau-vox", type=float, default=1.5)
-    ap.add_argument("--lr", type=float, default=5e-6)
+    ap.add_argument("--lr", type=float, default=2e-6)
+    ap.add_argument("--objective", choices=("energy", "sharpness"), default="energy")
     ap.add_argument("--K", type=int, default=64)
     ap.add_argument("--out", default=None)
     ap.add_argument("--ckpt-every", type=int, default=10)
     ap.add_argument("--resume", action="store_true")
     a = ap.parse_args(argv)
-    rec = run(a.config, a.steps, a.sigma, a.tau_vox, a.lr, a.K, a.out, a.ckpt_every, a.resume)
+    rec = run(a.config, a.steps, a.sigma, a.tau_vox, a.lr, a.K, a.out, a.ckpt_every, a.resume,
+              objective=a.objective)
     print(json.dumps({k: v for k, v in rec.items() if k != "history"}), flush=True)
     return 0
 
