@@ -169,6 +169,50 @@ int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
                  double dx, int nx, int ny, int nz, void *workspace,
                  size_t workspace_bytes, void *stream);
 
+/*
+ * mmn_li_threshold -- replaces skimage.filters.threshold_li as called by
+ * _separate_binary_image (sdf.py:46-70) on the image |img|^exponent.
+ * img: complex [n] (finite values).  With a = |img|^exponent (FP64) and
+ * x = a - min(a), iterates t <- (m_b - m_f) / (ln m_b - ln m_f) (m_f, m_b the
+ * means of x above / not above t) from t = mean(x), stopping when t stops
+ * changing, when m_b = 0, or after max_iter updates of t.  Writes the
+ * threshold in the units of a (t + min a), the number of updates made (the
+ * reference's iteration callbacks after the first) and whether the stopping
+ * rule was met (host pointers).  A constant image
+ * returns its value after 0 iterations.  If below != NULL it receives the
+ * binary image of sdf.py:66, below[i] = a[i] < threshold (uint8 [n], device),
+ * formed from the same a.  All sums are FP64 in a fixed order
+ * (deterministic).  Synchronises `stream` once per 16 iterations.
+ */
+size_t mmn_li_threshold_workspace_size(int64_t n);
+int mmn_li_threshold(double *threshold, int *iterations, int *converged,
+                     uint8_t *below, const float *img, int64_t n,
+                     double exponent, int max_iter, void *workspace,
+                     size_t workspace_bytes, void *stream);
+
+/*
+ * mmn_ccl6 -- replaces cc3d.connected_components(mask, connectivity=6) of
+ * _find_connected_components (sdf.py:89).  mask: uint8 [NX*NY*NZ] (non-zero
+ * = foreground, z fastest).  labels: int32 [V], 0 on background and 1..n on
+ * the n 6-connected components, numbered in the order a C-order scan first
+ * meets them (scipy.ndimage.label's numbering; the component partition is
+ * cc3d's).  If n_components != NULL the count is written there (host
+ * pointer; the call then synchronises `stream`).
+ */
+size_t mmn_ccl6_workspace_size(int nx, int ny, int nz);
+int mmn_ccl6(int32_t *labels, int32_t *n_components, const uint8_t *mask,
+             int nx, int ny, int nz, void *workspace, size_t workspace_bytes,
+             void *stream);
+
+/*
+ * mmn_label_sizes -- sizes[l] = number of the n_vox entries of labels equal
+ * to l, for l = 1..n_labels, and sizes[0] = the number of all other entries
+ * (int64 [n_labels + 1]).  The per-label areas of sdf.py:100 (ndimage.sum
+ * of the mask over the labels).
+ */
+int mmn_label_sizes(int64_t *sizes, const int32_t *labels, int64_t n_vox,
+                    int n_labels, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -502,6 +502,7 @@ int dispatch_k(int K, hipStream_t st, const RecArgs &a) {
 int dispatch_rec(int K, hipStream_t st, const RecArgs &a) {
   const int vpt = env_int("MMN_REC_VPT", 2);
   const int split = env_int("MMN_REC_SPLIT", 1);
+  if (vpt == 4) return dispatch_k<4, 1>(K, st, a);
   if (vpt == 1 && split == 1) return dispatch_k<1, 1>(K, st, a);
   if (vpt == 1) return dispatch_k<1, 2>(K, st, a);
   if (split == 1) return dispatch_k<2, 1>(K, st, a);

@@ -244,3 +244,69 @@ def rsdf_bfs(normals, labels, seeds, dx):
               labels.data_ptr(), seeds.data_ptr(), seeds.numel(), float(dx), nx, ny, nz,
               ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
     return rsdf, hops
+
+
+def li_threshold(img, exponent, max_iter=1000, below=False):
+    """Li threshold of |img|**exponent on the GPU.
+
+    img complex64 (any shape).  Returns (threshold, iterations, converged):
+    the threshold in the units of |img|**exponent, the number of Li
+    iterations run (at most ``max_iter``) and whether the stopping rule was
+    met; with ``below=True`` also the bool tensor |img|**exponent < threshold
+    (shaped like img).  Synchronises the stream once per 16 iterations.
+    """
+    _lib.require_gpu()
+    _check(img, "img", torch.complex64)
+    n = img.numel()
+    if n == 0:
+        raise ValueError("img is empty")
+    import ctypes
+    dev = img.device
+    lib = _lib.load()
+    ws_bytes = int(lib.mmn_li_threshold_workspace_size(n))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    thr, it, conv = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+    mask = torch.empty(img.shape, dtype=torch.uint8, device=dev) if below else None
+    _lib.call("mmn_li_threshold", ctypes.addressof(thr), ctypes.addressof(it),
+              ctypes.addressof(conv), _lib.ptr(mask), img.data_ptr(), n, float(exponent),
+              int(max_iter), ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
+    if below:
+        return thr.value, it.value, bool(conv.value), mask.view(torch.bool)
+    return thr.value, it.value, bool(conv.value)
+
+
+def ccl6(mask):
+    """6-connected components of a 3-D mask on the GPU.
+
+    mask bool or uint8 (NX,NY,NZ).  Returns (labels int32 (NX,NY,NZ), n):
+    0 on background, 1..n numbered in the order a C-order scan first meets
+    each component.
+    """
+    _lib.require_gpu()
+    if not isinstance(mask, torch.Tensor):
+        raise TypeError("mask must be a torch.Tensor")
+    if mask.dtype == torch.bool:
+        mask = mask.view(torch.uint8)
+    _check(mask, "mask", torch.uint8, 3)
+    import ctypes
+    nx, ny, nz = mask.shape
+    dev = mask.device
+    labels = torch.empty((nx, ny, nz), dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    ws_bytes = int(lib.mmn_ccl6_workspace_size(nx, ny, nz))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    n = ctypes.c_int32()
+    _lib.call("mmn_ccl6", labels.data_ptr(), ctypes.addressof(n), mask.data_ptr(), nx, ny, nz,
+              ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
+    return labels, int(n.value)
+
+
+def label_sizes(labels, n_labels):
+    """int64 (n_labels + 1,): entries of ``labels`` equal to l for l = 1..n_labels;
+    index 0 counts all other entries."""
+    _lib.require_gpu()
+    _check(labels, "labels", torch.int32)
+    sizes = torch.empty(n_labels + 1, dtype=torch.int64, device=labels.device)
+    _lib.call("mmn_label_sizes", sizes.data_ptr(), labels.data_ptr(), labels.numel(),
+              int(n_labels), _lib.stream_handle(labels.device))
+    return sizes

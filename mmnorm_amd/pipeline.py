@@ -7,8 +7,8 @@ isosurface optimisation of the antenna positions through the backward kernel.
    p0 = p* + N(0, sigma) (sigma = 0.3 mm by default).
 2. Full-grid forward at p0 (recurrence kernel, vote moment fused).
 3. Normals on voxels above 5 % of the image maximum (py_sn_est_gpu.py:326),
-   unit-normalised (image_processor.py:396); 6-connected components and the
-   reference's size rule (sdf.py:105-119); per-component seeds
+   unit-normalised (image_processor.py:396); 6-connected components on the
+   GPU and the reference's size rule (sdf.py:88-119); per-component seeds
    (sdf.py:182-191); GPU RSDF BFS.
 4. Isosurface band B = {reached voxels with |rsdf| <= tau}.  The loss is the
    negative image energy on the band, L(p) = -sum_{v in B} |I_v(p)|^2 / E0,
@@ -40,27 +40,12 @@ import time
 
 import numpy as np
 import torch
-from scipy import ndimage
 
 from . import ops, sdf, synth
 
 
 def log(msg):
     print(f"[pipeline] {msg}", flush=True)
-
-
-def valid_components(mask, dx):
-    """Components and the reference's size filter (sdf.py:94-120), labels + 1."""
-    labeled, nlab = ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
-    labeled = labeled + 1
-    labels = np.arange(1, labeled.max() + 2)
-    areas = np.array(ndimage.sum(mask, labeled, labels))
-    for a, b in ((0.03, 0.04), (0.015, 0.02)):
-        lo, lz = int(a / dx), int(b / dx)
-        valid = labels[areas > lo ** 2 * lz]
-        if len(valid):
-            return labeled, valid
-    return labeled, labels[:0]
 
 
 def _objective(I, kind):
@@ -109,14 +94,18 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
     normals = torch.full((img.numel(), 3), float("nan"), dtype=torch.float64, device=dev)
     normals[vidx.long()] = n
     normals = normals.reshape(*img.shape, 3)
-    nrm_np = normals.cpu().numpy()
-    mask = ~np.isnan(nrm_np[..., 0])
     dx = float(sc.x_locs[1] - sc.x_locs[0])
-    labeled, valid = valid_components(mask, dx)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    lab_t, areas = sdf.find_components_gpu(normals)   # GPU 6-connected labels + 1
+    valid = sdf.size_rule(areas, dx)
+    torch.cuda.synchronize()
+    rec["ccl_s"] = time.perf_counter() - tc
+    nrm_np = normals.cpu().numpy()
+    labeled = lab_t.cpu().numpy()
     m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
     coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
     seeds = sdf.component_seeds(nrm_np, coords, labeled, valid)
-    lab_t = T(labeled, np.int32)
     torch.cuda.synchronize()
     tb = time.perf_counter()
     rsdf, hops = ops.rsdf_bfs(normals, lab_t, T(seeds, np.int32), dx)
@@ -129,7 +118,8 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
     rec.update({"valid_voxels": int(vidx.numel()), "components": int(len(seeds)),
                 "reached": int(reached.sum().item()), "levels": int(hops.max().item()) + 1,
                 "band_voxels": int(bidx.numel())})
-    log(f"vote + BFS: {rec['prep_s']:.2f} s (BFS {rec['bfs_s'] * 1e3:.1f} ms, "
+    log(f"vote + CCL + BFS: {rec['prep_s']:.2f} s (CCL {rec['ccl_s'] * 1e3:.1f} ms, "
+        f"BFS {rec['bfs_s'] * 1e3:.1f} ms, "
         f"{rec['components']} components, {rec['reached']} voxels reached, "
         f"{rec['levels']} levels); band {rec['band_voxels']} voxels")
     if bidx.numel() == 0:

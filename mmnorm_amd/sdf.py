@@ -1,20 +1,123 @@
-"""Drop-in for ``sdf._construct_rsdf_components`` (sdf.py:122-224).
+"""Drop-ins for the RSDF stage of ``sdf.py``: ``_separate_binary_image``
+(sdf.py:33-86), ``_find_connected_components`` (sdf.py:88-120) and
+``_construct_rsdf_components`` (sdf.py:122-224).  Same signatures, NumPy in
+and out.
 
-Same signature and result: a float64 array shaped like the image holding
-the RSDF of every valid connected component and NaN elsewhere.  The
-breadth-first search itself (sdf.py:195-222, the interpreter-bound loop)
-runs on the GPU (``mmnorm_amd.ops.rsdf_bfs``), bit-exact with the
-reference's FIFO order.  The start voxel of each component is picked on the
-host with the reference's own NumPy expressions (sdf.py:182-191: mean of the
-component's coordinates, Euclidean distance, first minimum), so ties resolve
-identically.
+* The Li threshold of each exponent (sdf.py:46-70) runs on the GPU
+  (``ops.li_threshold``); the binary-image volume test (sdf.py:77-84) uses
+  ``volume_fn`` (see there).
+* The 6-connected labelling (cc3d, sdf.py:89) and the per-label areas
+  (sdf.py:100) run on the GPU (``ops.ccl6``, ``ops.label_sizes``); the
+  reference's size rule (sdf.py:92-119) is applied to the areas as written.
+* The breadth-first search (sdf.py:195-222, the interpreter-bound loop) runs
+  on the GPU (``ops.rsdf_bfs``), bit-exact with the reference's FIFO order.
+  The start voxel of each component is picked on the host with the
+  reference's own NumPy expressions (sdf.py:182-191: mean of the component's
+  coordinates, Euclidean distance, first minimum), so ties resolve
+  identically.
 """
 from __future__ import annotations
+
+import sys
 
 import numpy as np
 import torch
 
 from . import _lib, ops
+
+LI_EXPONENTS = (2, 2.5, 3.5, 4)     # sdf.py:73
+LI_MAX_ITER = 1000                  # sdf.py:56-58
+VOLUME_THRESHOLD = 0.03             # sdf.py:74 (m^3)
+
+
+def _device():
+    _lib.require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def pca_box_volume(points):
+    """Volume of the box aligned with the principal axes of ``points``.
+
+    Stands in for open3d's minimal oriented bounding box (sdf.py:79-81),
+    which is not available here: it is an upper bound of that volume (equal
+    for box-like point sets), so the volume test can only be stricter.
+    """
+    p = np.asarray(points, dtype=np.float64)
+    if p.shape[0] < 2:
+        return 0.0
+    c = p - p.mean(axis=0)
+    _, _, vt = np.linalg.svd(c, full_matrices=False)
+    q = c @ vt.T
+    ext = q.max(axis=0) - q.min(axis=0)
+    return float(np.prod(ext)) if ext.size == 3 else 0.0
+
+
+def _separate_binary_image(sum_img, coords_full, volume_fn=pca_box_volume):
+    """Binary image of the SAR image (sdf.py:33-86): True where |sum_img|**exp is
+    below the Li threshold, for the first exponent of (2, 2.5, 3.5, 4) whose Li
+    iteration ends within 1000 callbacks and whose kept voxels span at most
+    0.03 m^3 (``volume_fn`` of their coordinates); if no exponent passes the
+    volume test, the last exponent's result if its threshold succeeded, else
+    ``sys.exit(0)`` as the reference does."""
+    dev = _device()
+    z = torch.from_numpy(np.ascontiguousarray(sum_img, dtype=np.complex64)).to(dev)
+    filter_idx, success = None, False
+    for ex in LI_EXPONENTS:
+        _, iters, _, below = ops.li_threshold(z, ex, max_iter=LI_MAX_ITER, below=True)
+        # the reference's callback raises on its 1001st call: the initial
+        # guess plus 1000 iterations
+        success = iters < LI_MAX_ITER
+        if success:
+            filter_idx = below.cpu().numpy()
+        points = coords_full[np.logical_not(filter_idx)] if filter_idx is not None else \
+            coords_full.reshape(-1, 3)
+        volume = volume_fn(points)
+        if success and volume <= VOLUME_THRESHOLD:
+            return filter_idx
+    if success:
+        return filter_idx
+    sys.exit(0)
+
+
+def size_rule(areas, dx):
+    """Valid labels by the reference's size filter (sdf.py:92-119).
+
+    ``areas[l - 1]`` is the voxel count of label l for l = 1..max+1 (the
+    reference's ``np.arange(1, labeled.max() + 2)``).
+    """
+    labels = np.arange(1, len(areas) + 1)
+    lo, lz = int(0.03 / dx), int(0.04 / dx)
+    valid = labels[areas > lo ** 2 * lz]
+    if len(valid) == 0:
+        lo, lz = int(0.015 / dx), int(0.02 / dx)
+        valid = labels[areas > lo ** 2 * lz]
+    return valid
+
+
+def find_components_gpu(normals):
+    """GPU labelling of the non-NaN voxels of a device normal field.
+
+    normals float64 device tensor (NX,NY,NZ,3).  Returns (labeled int32
+    device tensor = component label + 1, background 1, as sdf.py:89;
+    areas int64 NumPy array of labels 1..max+1).
+    """
+    mask = ~torch.isnan(normals[..., 0])
+    lab, n = ops.ccl6(mask.contiguous())
+    sizes = ops.label_sizes(lab, n).cpu().numpy()   # [0] background, [1..n]
+    # labels 1..n+2 of the reference's +1 numbering: background (label 1)
+    # has mask-sum 0, components 2..n+1 their sizes, label n+2 is empty
+    areas = np.concatenate(([0], sizes[1:], [0])).astype(np.float64)
+    return lab + 1, areas
+
+
+def _find_connected_components(all_normals, x_locs):
+    """6-connected components of the non-NaN normals and the valid labels
+    (sdf.py:88-120); labels numbered in C-order first appearance, + 1."""
+    dev = _device()
+    nrm = torch.from_numpy(np.ascontiguousarray(all_normals, dtype=np.float64)).to(dev)
+    labeled, areas = find_components_gpu(nrm)
+    valid = size_rule(areas, x_locs[1] - x_locs[0])
+    return labeled.cpu().numpy().astype(np.uint32), valid
 
 
 def component_seeds(all_normals, coords_full, labeled, valid_labels):

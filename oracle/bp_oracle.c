@@ -21,8 +21,20 @@
  *   or_bfs_fifo    restates the FIFO breadth-first search of
  *                  _construct_rsdf_components (sdf.py:180-222) for given
  *                  seeds.
+ *   or_li_threshold  restates the Li iteration of skimage.filters.threshold_li
+ *                  (Li & Tam's minimum cross-entropy threshold, called at
+ *                  sdf.py:60-70) in FP64 with sequential sums.
+ *   or_ccl6        6-connected component labelling (the partition of
+ *                  cc3d.connected_components(mask, connectivity=6),
+ *                  sdf.py:89) by flood fill in C-order scan, numbered in
+ *                  order of first appearance.
  *
- * Parity pins for each function are listed in oracle/README.md.
+ * Pins (tests/test_oracle.py): or_bp_ref and or_sn_ref equal their
+ * pure-Python restatements bit for bit on the golden inputs; or_bp_f64
+ * equals a NumPy brute force and the point-scatterer closed form;
+ * or_bfs_fifo equals the deque restatement and the constant-normal closed
+ * form; or_li_threshold meets its fixed-point equation and the constant and
+ * two-level closed forms; or_ccl6 equals scipy.ndimage.label.
  */
 #include <complex.h>
 #include <math.h>
@@ -286,4 +298,88 @@ int or_bfs_fifo(double *rsdf, int32_t *hops, const double *nrm,
   free(visited);
   free(q);
   return 0;
+}
+
+/* Li threshold on a = |img|^ex (img complex, interleaved), FP64 throughout:
+ * x = a - min a; t = mean x; repeat { t_c = t; m_f = mean{x > t_c},
+ * m_b = mean{x <= t_c}; stop if m_b == 0; t = (m_b - m_f)/(ln m_b - ln m_f) }
+ * while t != t_c and fewer than max_iter updates of t were made.  Writes
+ * t + min a and the number of updates; a constant image gives its value and
+ * 0 updates.
+ * Returns 0, or -1 on an allocation failure. */
+int or_li_threshold(double *thr, int *iters, const float *img, int64_t n,
+                    double ex, int max_iter) {
+  double *a = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  if (!a) return -1;
+  double lo = INFINITY, hi = -INFINITY, sum = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    a[i] = pow(hypot((double)img[2 * i], (double)img[2 * i + 1]), ex);
+    if (a[i] < lo) lo = a[i];
+    if (a[i] > hi) hi = a[i];
+    sum += a[i];
+  }
+  *iters = 0;
+  if (hi == lo) {
+    *thr = lo;
+    free(a);
+    return 0;
+  }
+  double t_next = sum / (double)n - lo, t_curr = -1.0;
+  int k = 0;
+  while (fabs(t_next - t_curr) > 0.0 && k < max_iter) {
+    t_curr = t_next;
+    double sf = 0.0, nf = 0.0, sb = 0.0, nb = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double x = a[i] - lo;
+      if (x > t_curr) {
+        sf += x;
+        nf += 1.0;
+      } else {
+        sb += x;
+        nb += 1.0;
+      }
+    }
+    const double mf = sf / nf, mb = sb / nb;
+    if (mb == 0.0) break;
+    t_next = (mb - mf) / (log(mb) - log(mf));
+    ++k;
+  }
+  *thr = t_next + lo;
+  *iters = k;
+  free(a);
+  return 0;
+}
+
+/* 6-connected labels: 0 background, 1..n by first appearance in a C-order
+ * scan; each new component is flood-filled with a FIFO queue.  Returns n,
+ * or -1 on an allocation failure. */
+int or_ccl6(int32_t *labels, const uint8_t *mask, int nx, int ny, int nz) {
+  const int64_t V = (int64_t)nx * ny * nz;
+  int64_t *q = malloc(sizeof(int64_t) * (size_t)(V > 0 ? V : 1));
+  if (!q) return -1;
+  for (int64_t i = 0; i < V; ++i) labels[i] = 0;
+  int n = 0;
+  for (int64_t s = 0; s < V; ++s) {
+    if (!mask[s] || labels[s]) continue;
+    ++n;
+    int64_t head = 0, tail = 0;
+    q[tail++] = s;
+    labels[s] = n;
+    while (head < tail) {
+      const int64_t u = q[head++];
+      const int x = (int)(u / ((int64_t)ny * nz));
+      const int y = (int)((u / nz) % ny);
+      const int z = (int)(u % nz);
+      for (int d = 0; d < 6; ++d) {
+        const int ax = x + DXS[d], ay = y + DYS[d], az = z + DZS[d];
+        if (ax < 0 || ay < 0 || az < 0 || ax >= nx || ay >= ny || az >= nz) continue;
+        const int64_t c = ((int64_t)ax * ny + ay) * nz + az;
+        if (!mask[c] || labels[c]) continue;
+        labels[c] = n;
+        q[tail++] = c;
+      }
+    }
+  }
+  free(q);
+  return n;
 }

@@ -61,6 +61,10 @@ def lib() -> ctypes.CDLL:
                                         i32, i32, i32, i32, p, i64]
         L.or_bfs_fifo.argtypes = [p, p, p, p, p, i32, f64, i32, i32, i32]
         L.or_bfs_fifo.restype = i32
+        L.or_li_threshold.argtypes = [p, p, p, i64, f64, i32]
+        L.or_li_threshold.restype = i32
+        L.or_ccl6.argtypes = [p, p, i32, i32, i32]
+        L.or_ccl6.restype = i32
         for fn in (L.or_bp_ref, L.or_sn_ref, L.or_bp_f64, L.or_bp_cpu_imaging):
             fn.restype = None
         _lib = L
@@ -172,6 +176,48 @@ def bfs_fifo(normals, labels, seeds, dx):
     if rc != 0:
         raise MemoryError("or_bfs_fifo: allocation failed")
     return rsdf.reshape(nx, ny, nz), hops.reshape(nx, ny, nz)
+
+
+def li_threshold(img, exponent, max_iter=1000):
+    """Li threshold of |img|**exponent (FP64, sequential sums): (threshold, iterations)."""
+    z = np.ascontiguousarray(np.asarray(img, np.complex64).ravel())
+    thr = ctypes.c_double()
+    it = ctypes.c_int()
+    rc = lib().or_li_threshold(ctypes.addressof(thr), ctypes.addressof(it), _ptr(z), z.size,
+                               float(exponent), int(max_iter))
+    if rc != 0:
+        raise MemoryError("or_li_threshold: allocation failed")
+    return thr.value, it.value
+
+
+def ccl6(mask):
+    """6-connected labels (0 background, 1..n by first appearance in C order), n."""
+    m = np.ascontiguousarray(np.asarray(mask, dtype=bool).astype(np.uint8))
+    nx, ny, nz = m.shape
+    lab = np.empty(m.shape, np.int32)
+    n = lib().or_ccl6(_ptr(lab), _ptr(m), nx, ny, nz)
+    if n < 0:
+        raise MemoryError("or_ccl6: allocation failed")
+    return lab, n
+
+
+def py_li_threshold(img, exponent, max_iter=1000):
+    """The same iteration with NumPy float64 sums (pairwise): tiny inputs."""
+    a = np.abs(np.asarray(img, np.complex64).astype(np.complex128).ravel()) ** exponent
+    lo = a.min()
+    if a.max() == lo:
+        return float(lo), 0
+    x = a - lo
+    t_next, t_curr, k = x.mean(), -1.0, 0
+    while abs(t_next - t_curr) > 0 and k < max_iter:
+        t_curr = t_next
+        fg = x > t_curr
+        mf, mb = x[fg].mean(), x[~fg].mean()
+        if mb == 0:
+            break
+        t_next = (mb - mf) / (np.log(mb) - np.log(mf))
+        k += 1
+    return float(t_next + lo), k
 
 
 # ----------------------------------------------------------------------------

@@ -20,7 +20,9 @@ def test_header_declares_the_entry_points():
     for want in ("mmn_cuda_image", "mmn_cuda_sn_est", "mmn_bp_forward",
                  "mmn_bp_forward_workspace_size", "mmn_sn_from_moment",
                  "mmn_bp_backward", "mmn_bp_backward_workspace_size", "mmn_rsdf_bfs",
-                 "mmn_rsdf_bfs_workspace_size", "mmn_abi_version", "mmn_last_error"):
+                 "mmn_rsdf_bfs_workspace_size", "mmn_li_threshold",
+                 "mmn_li_threshold_workspace_size", "mmn_ccl6", "mmn_ccl6_workspace_size",
+                 "mmn_label_sizes", "mmn_abi_version", "mmn_last_error"):
         assert want in names
 
 
@@ -63,6 +65,14 @@ def test_workspace_queries_and_argument_errors_without_gpu():
                   1, 1, 1, 1, 1, 1, 0, 32, None, 0, None)
     with pytest.raises(ValueError, match="workspace"):
         _lib.call("mmn_rsdf_bfs", 8, 8, 8, 8, 8, 1, 1e-3, 4, 4, 4, None, 0, None)
+    assert lib.mmn_ccl6_workspace_size(4, 4, 4) >= 4 * 4 * 4 * 4
+    assert lib.mmn_li_threshold_workspace_size(100) >= 800
+    with pytest.raises(ValueError, match="workspace"):
+        _lib.call("mmn_ccl6", 8, None, 8, 4, 4, 4, None, 0, None)
+    with pytest.raises(ValueError, match="exponent"):
+        _lib.call("mmn_li_threshold", 8, None, None, None, 8, 10, -1.0, 10, None, 0, None)
+    with pytest.raises(ValueError, match="NULL"):
+        _lib.call("mmn_label_sizes", None, 8, 10, 3, None)
     with pytest.raises(ValueError, match="sizes"):
         _lib.call("mmn_cuda_image", 8, 8, 8, 8, 8, 8, 8, 1.0, 1.0, 1.0, 0, 1, 1, 1, 1, 1, 0,
                   0, None)

@@ -1,0 +1,153 @@
+"""Segmentation before the RSDF search on the GPU: Li threshold, 6-connected
+labelling and label sizes against the oracle and scipy.ndimage.label, and
+the sdf.py drop-ins built on them."""
+import numpy as np
+import pytest
+import torch
+from scipy import ndimage
+
+from mmnorm_amd import ops, sdf
+
+pytestmark = pytest.mark.gpu
+
+
+def dev():
+    return torch.device("cuda", 0)
+
+
+def scipy_ccl6(mask):
+    return ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
+
+
+def smooth_mask(shape, frac, seed, sigma=2.0):
+    """Blobs: a smoothed random field above its (1 - frac) quantile."""
+    f = ndimage.gaussian_filter(np.random.default_rng(seed).normal(size=shape), sigma)
+    return f > np.quantile(f, 1.0 - frac)
+
+
+@pytest.mark.parametrize("shape,density,seed", [
+    ((1, 1, 1), 1.0, 0), ((1, 1, 1), 0.0, 1), ((5, 7, 9), 0.3, 2), ((33, 17, 65), 0.25, 3),
+    ((64, 64, 32), 0.3116, 4), ((64, 64, 32), 0.5, 5), ((40, 30, 20), 1.0, 6),
+    ((40, 30, 20), 0.0, 7)])
+def test_ccl6_equals_scipy_random(shape, density, seed):
+    """Random site percolation, including densities near the 3-D threshold
+    (~0.3116) where components are long and branched."""
+    mask = np.random.default_rng(seed).uniform(size=shape) < density
+    lab, n = ops.ccl6(torch.from_numpy(mask).to(dev()))
+    ref, nref = scipy_ccl6(mask)
+    assert n == nref
+    assert np.array_equal(lab.cpu().numpy(), ref)
+
+
+def test_ccl6_equals_scipy_blobs_large():
+    """A 256x256x128 blob field (config-5 grid size)."""
+    mask = smooth_mask((256, 256, 128), 0.2, 11)
+    lab, n = ops.ccl6(torch.from_numpy(mask).to(dev()))
+    ref, nref = scipy_ccl6(mask)
+    assert n == nref
+    assert np.array_equal(lab.cpu().numpy(), ref)
+
+
+def test_ccl6_snake_spanning_the_grid():
+    """One path that zigzags through every row of every slab, so unions chain
+    across the whole grid."""
+    nx, ny, nz = 8, 33, 70
+    mask = np.zeros((nx, ny, nz), bool)
+    for x in range(nx):
+        for y in range(0, ny, 2):
+            mask[x, y, :] = True
+            if y + 1 < ny:
+                mask[x, y + 1, (nz - 1) if (y // 2) % 2 == 0 else 0] = True
+        if x + 1 < nx:
+            mask[x, ny - 1, :] = True
+    lab, n = ops.ccl6(torch.from_numpy(mask).to(dev()))
+    ref, nref = scipy_ccl6(mask)
+    assert n == nref
+    assert np.array_equal(lab.cpu().numpy(), ref)
+
+
+def test_ccl6_deterministic():
+    mask = torch.from_numpy(np.random.default_rng(3).uniform(size=(48, 40, 36)) < 0.33).to(dev())
+    a, na = ops.ccl6(mask)
+    b, nb = ops.ccl6(mask)
+    assert na == nb and torch.equal(a, b)
+
+
+def test_label_sizes_equal_bincount():
+    mask = smooth_mask((64, 48, 40), 0.3, 5)
+    lab, n = ops.ccl6(torch.from_numpy(mask).to(dev()))
+    sizes = ops.label_sizes(lab, n).cpu().numpy()
+    ref = np.bincount(lab.cpu().numpy().ravel(), minlength=n + 1)
+    assert np.array_equal(sizes, ref)
+    # labels beyond n_labels count with the background in sizes[0]
+    sizes2 = ops.label_sizes(lab, 1).cpu().numpy()
+    assert sizes2[1] == ref[1] and sizes2[0] == lab.numel() - ref[1]
+
+
+@pytest.mark.parametrize("ex", [2, 2.5, 3.5, 4])
+def test_li_threshold_matches_oracle(oracle, ex):
+    rng = np.random.default_rng(int(ex * 100))
+    n = 200_000
+    img = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64)
+    img[: n // 50] *= 6
+    thr, it, conv, below = ops.li_threshold(torch.from_numpy(img).to(dev()), ex, below=True)
+    thr_o, it_o = oracle.li_threshold(img, ex)
+    assert conv and it == it_o
+    assert abs(thr - thr_o) <= 1e-12 * thr_o
+    a = np.abs(img.astype(np.complex128)) ** ex
+    assert np.array_equal(below.cpu().numpy(), a < thr)
+
+
+def test_li_threshold_closed_forms():
+    z = torch.full((1000,), 3.0 + 0j, dtype=torch.complex64, device=dev())
+    thr, it, conv = ops.li_threshold(z, 1.0)
+    # the device pow may be an ulp off, even for exponent 1
+    assert abs(thr - 3.0) <= 2e-15 and it == 0 and conv
+    img = np.array([2.0, 3.0, 12.0] * 1000, np.complex64)
+    thr, it, _ = ops.li_threshold(torch.from_numpy(img).to(dev()), 1.0)
+    assert it == 2 and abs(thr - (2.0 + 9.5 / np.log(20.0))) < 1e-13
+    # iteration cap: one update allowed
+    rng = np.random.default_rng(9)
+    img = (rng.normal(size=5000) + 1j * rng.normal(size=5000)).astype(np.complex64)
+    _, it, conv = ops.li_threshold(torch.from_numpy(img).to(dev()), 2.0, max_iter=1)
+    assert it == 1 and not conv
+
+
+def test_find_connected_components_dropin():
+    """_find_connected_components: labels + 1 and the size rule, against the
+    same steps on scipy's labels."""
+    shape = (96, 96, 64)
+    mask = smooth_mask(shape, 0.15, 21, sigma=3.0)
+    normals = np.full(shape + (3,), np.nan)
+    normals[mask] = 1.0
+    x_locs = np.arange(shape[0]) * 1e-3
+    labeled, valid = sdf._find_connected_components(normals, x_locs)
+    ref, _ = scipy_ccl6(mask)
+    ref = ref + 1
+    labels = np.arange(1, ref.max() + 2)
+    areas = np.array(ndimage.sum(mask, ref, labels))
+    lo, lz = int(0.03 / 1e-3), int(0.04 / 1e-3)
+    vref = labels[areas > lo ** 2 * lz]
+    if len(vref) == 0:
+        lo, lz = int(0.015 / 1e-3), int(0.02 / 1e-3)
+        vref = labels[areas > lo ** 2 * lz]
+    assert np.array_equal(labeled, ref)
+    assert np.array_equal(valid, vref)
+    assert len(valid) > 0
+
+
+def test_separate_binary_image_dropin():
+    rng = np.random.default_rng(4)
+    shape = (32, 24, 16)
+    img = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
+    img[10:20, 8:16, 4:12] *= 10
+    m1, m2, m3 = np.meshgrid(np.arange(24) * 1e-3, np.arange(32) * 1e-3, np.arange(16) * 1e-3)
+    coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
+    # volume test passes: the first exponent's binary image
+    got = sdf._separate_binary_image(img, coords, volume_fn=lambda p: 0.0)
+    _, _, _, below = ops.li_threshold(torch.from_numpy(img).to(dev()), 2, below=True)
+    assert np.array_equal(got, below.cpu().numpy())
+    # volume test never passes: the last exponent's
+    got = sdf._separate_binary_image(img, coords, volume_fn=lambda p: 1.0)
+    _, _, _, below = ops.li_threshold(torch.from_numpy(img).to(dev()), 4, below=True)
+    assert np.array_equal(got, below.cpu().numpy())

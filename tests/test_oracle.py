@@ -233,3 +233,76 @@ def test_seed_ties_take_the_first_voxel(oracle):
     X, Y, Z = np.meshgrid(*(np.arange(4) * 0.25 for _ in range(3)), indexing="ij")
     seeds = oracle.rsdf_seeds(n, np.stack([X, Y, Z], -1), lab, [2])
     assert seeds[0] == np.ravel_multi_index((1, 1, 1), shape)
+
+
+# ---------------------------------------------------------------- CCL / Li --
+def _scipy_ccl6(mask):
+    return ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
+
+
+@pytest.mark.parametrize("shape,density,seed", [((1, 1, 1), 1.0, 0), ((5, 7, 9), 0.3, 1),
+                                                ((16, 12, 10), 0.5, 2), ((24, 20, 18), 0.31, 3),
+                                                ((9, 9, 9), 0.0, 4), ((6, 5, 4), 1.0, 5)])
+def test_ccl6_oracle_equals_scipy(oracle, shape, density, seed):
+    """The flood-fill restatement gives scipy.ndimage.label's labels exactly
+    (the same partition and the same first-appearance numbering)."""
+    mask = np.random.default_rng(seed).uniform(size=shape) < density
+    lab, n = oracle.ccl6(mask)
+    ref, nref = _scipy_ccl6(mask)
+    assert n == nref
+    assert np.array_equal(lab, ref)
+
+
+def test_ccl6_oracle_snake(oracle):
+    """One component threading every row of a slab, back and forth."""
+    nx, ny, nz = 3, 9, 11
+    mask = np.zeros((nx, ny, nz), bool)
+    for x in range(nx):
+        for y in range(0, ny, 2):
+            mask[x, y, :] = True
+            if y + 1 < ny:
+                mask[x, y + 1, (nz - 1) if (y // 2) % 2 == 0 else 0] = True
+    lab, n = oracle.ccl6(mask)
+    ref, nref = _scipy_ccl6(mask)
+    assert n == nref and np.array_equal(lab, ref)
+
+
+def _li_fixed_point_gap(img, ex, thr):
+    a = np.abs(np.asarray(img, np.complex64).astype(np.complex128)) ** ex
+    x = a - a.min()
+    t = thr - a.min()
+    mf, mb = x[x > t].mean(), x[x <= t].mean()
+    return abs((mb - mf) / (np.log(mb) - np.log(mf)) - t) / t
+
+
+@pytest.mark.parametrize("ex", [2, 2.5, 3.5, 4])
+def test_li_oracle_fixed_point_and_python(oracle, ex):
+    """At the returned threshold the Li update maps t to itself, and the C
+    restatement agrees with the NumPy one."""
+    rng = np.random.default_rng(int(ex * 10))
+    img = (rng.normal(size=4000) + 1j * rng.normal(size=4000)).astype(np.complex64)
+    img[:40] *= 8  # a bright minority, as a SAR image has
+    thr, it = oracle.li_threshold(img, ex)
+    assert 0 < it < 1000
+    assert _li_fixed_point_gap(img, ex, thr) < 1e-12
+    thr_py, it_py = oracle.py_li_threshold(img, ex)
+    assert it_py == it
+    assert abs(thr_py - thr) <= 1e-12 * thr
+
+
+def test_li_oracle_closed_forms(oracle):
+    # constant image: its value, no update
+    thr, it = oracle.li_threshold(np.full(50, 3.0 + 0j), 1.0)
+    assert thr == 3.0 and it == 0
+    # two levels: after the min shift the background mean is 0, so the
+    # iteration stops at the initial guess, the mean
+    img = np.array([2.0] * 7 + [5.0] * 3, np.complex64)
+    thr, it = oracle.li_threshold(img, 1.0)
+    assert it == 0 and abs(thr - np.mean(np.abs(img))) < 1e-15
+    # three levels 2, 3, 12 (shifted 0, 1, 10): the mean 11/3 puts 12 alone in
+    # the foreground, m_b = 1/2, m_f = 10, t = 9.5 / ln 20 separates the same
+    # way, so it is the answer
+    img = np.array([2.0, 3.0, 12.0], np.complex64)
+    thr, it = oracle.li_threshold(img, 1.0)
+    assert it == 2
+    assert abs(thr - (2.0 + 9.5 / np.log(20.0))) < 1e-14
