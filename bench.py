@@ -14,10 +14,11 @@ in HBM.  Prints ONE JSON line:
              126.7 TFLOP/s an FMA-only microbenchmark reaches on the same part
              (tests/probes/valu_peak.hip); traffic from the committed
              rocprofv3 --pmc summary when present
-* plain      the direct-sincos FP32 kernel on the same inputs (fewer steps)
+* plain      the direct-sincos FP32 kernel on the same inputs (fewer steps),
+             and plain_fast, the same with the hardware sine/cosine
 * backward   one backward (dL/dmeas, dL/dant) of L = sum w|I|^2
-* rsdf_bfs   GPU BFS over the 5%-filtered vote field (components by 6-connected
-             labelling on the host)
+* rsdf_bfs   GPU 6-connected labelling and GPU BFS over the 5%-filtered vote
+             field
 * cpu_baseline  the CPU restatement of the reference's imaging loop
              (imaging.cpp:60-139, OpenMP) on a bounded voxel sample
 """
@@ -178,7 +179,15 @@ def main(argv=None):
             log(f"plain forward: {p_s * 1e3:.2f} ms")
             rec["plain"] = {"value": W / p_s / 1e9, "kernel_ms": p_s * 1e3,
                             "tflops": flops_fwd(V, A, F) / p_s / 1e12,
-                            "rec_over_plain": p_s / k_avg}
+                            "rec_over_plain": p_s / k_avg,
+                            "kind": "library sincosf of the FP32 phase (radians)"}
+            fw, fk, _ = timed(lambda: fwd("plain_fast"), max(a.plain_steps, 2), 1)
+            f_s = float(np.mean(fk)) * 1e-3
+            log(f"plain_fast forward: {f_s * 1e3:.2f} ms")
+            rec["plain_fast"] = {"value": W / f_s / 1e9, "kernel_ms": f_s * 1e3,
+                                 "rec_over_plain_fast": f_s / k_avg,
+                                 "kind": "hardware sin/cos of the FP32 phase fraction "
+                                         "(revolutions)"}
         # backward of L = sum w |I|^2 (I = img / A)
         w = torch.rand(V, device=dev, generator=torch.Generator(dev).manual_seed(1))
         g = (2.0 * w * img.reshape(-1) / A / A).to(torch.complex64).contiguous()
@@ -207,9 +216,9 @@ def main(argv=None):
 
 
 def bfs_stage(img, mom, sc, dev):
-    """5% filter -> votes -> 6-connected components (host) -> GPU FIFO BFS, timed."""
+    """5% filter -> votes -> 6-connected components and the size rule (GPU) ->
+    GPU FIFO BFS, timed."""
     import torch
-    from scipy import ndimage
     from mmnorm_amd import ops, sdf
     mag = img.abs().reshape(-1)
     vidx = torch.nonzero(mag > 0.05 * mag.max()).reshape(-1).to(torch.int32)
@@ -220,25 +229,27 @@ def bfs_stage(img, mom, sc, dev):
     normals = torch.full((img.numel(), 3), float("nan"), dtype=torch.float64, device=dev)
     normals[vidx.long()] = nrm
     normals = normals.reshape(*shape, 3)
-    mask = (~torch.isnan(normals[..., 0])).cpu().numpy()
-    labeled, nlab = ndimage.label(mask, structure=ndimage.generate_binary_structure(3, 1))
-    labeled = labeled + 1
-    sizes = ndimage.sum(mask, labeled, np.arange(2, nlab + 2))
-    valid = np.arange(2, nlab + 2)[sizes >= 27]
+    dx = sc.x_locs[1] - sc.x_locs[0]
+    sdf.find_components_gpu(normals)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    lab_t, areas = sdf.find_components_gpu(normals)   # GPU 6-connected labels + 1
+    torch.cuda.synchronize()
+    ccl_ms = (time.perf_counter() - tc) * 1e3
+    valid = sdf.size_rule(areas, dx)
+    labeled = lab_t.cpu().numpy()
     nrm_np = normals.cpu().numpy()
     m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
     coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
     seeds = sdf.component_seeds(nrm_np, coords, labeled, valid)
-    lab_t = torch.from_numpy(labeled.astype(np.int32)).to(dev)
     seeds_t = torch.from_numpy(seeds.astype(np.int32)).to(dev)
-    dx = sc.x_locs[1] - sc.x_locs[0]
     ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     rsdf, hops = ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return {"ms": dt * 1e3, "voxels_reached": int((hops >= 0).sum().item()),
+    return {"ms": dt * 1e3, "ccl_ms": ccl_ms, "voxels_reached": int((hops >= 0).sum().item()),
             "components": int(len(seeds)), "levels": int(hops.max().item()) + 1,
             "valid_voxels": int(vidx.numel())}
 

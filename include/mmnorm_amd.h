@@ -40,6 +40,8 @@ extern "C" {
 #define MMN_BP_PLAIN 1 /* direct FP32 sincosf per (voxel, row, sample)          */
 #define MMN_BP_REC   2 /* FP32 rotation (Horner) over samples, FP64 re-anchor
                           every K samples, compensated FP32 accumulation       */
+#define MMN_BP_PLAIN_FAST 3 /* direct FP32: phase in revolutions, hardware sine
+                               and cosine of its fraction                     */
 
 /* Status codes below zero. */
 #define MMN_ERR_ARG      (-1) /* bad size, count or enum value                */
@@ -96,7 +98,8 @@ int mmn_cuda_sn_est(float *normals, const int32_t *vox_idx, int64_t n_valid,
  *     moment[v][k] = sum_i (p_i + o_i/2 - v)_k * c_iv      (complex, [n][3])
  * where c_iv is row i's contribution to img[v]; the un-normalised vote of
  * surface_normal_est.cu:128-144 is then Re(conj(I_v) moment[v]) / |I_v|
- * (mmn_sn_from_moment).  variant: MMN_BP_PLAIN or MMN_BP_REC; K (REC only)
+ * (mmn_sn_from_moment).  variant: MMN_BP_PLAIN, MMN_BP_PLAIN_FAST or
+ * MMN_BP_REC; K (REC only)
  * is the re-anchor interval in samples, one of 8, 16, 32, 64 (or 0 = 64).
  * No reference counterpart for the fused moment or the variants; the
  * reference computes the vote only for the interpolated branch

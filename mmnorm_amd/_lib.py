@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 MMN_BP_PLAIN = 1
 MMN_BP_REC = 2
+MMN_BP_PLAIN_FAST = 3
 
 _c_int = ctypes.c_int
 _c_i64 = ctypes.c_int64

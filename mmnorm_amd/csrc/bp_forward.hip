@@ -3,8 +3,10 @@
 //   k_image_mirror  the reference kernel's arithmetic (imaging_gpu.cu:48-122):
 //                   FP32 distance, FP64 phase and cos/sin, FP32 sums in row
 //                   order; both the interpolated and the full branch.
-//   k_bp_plain      full matched filter with one FP32 sincosf per
-//                   (voxel, row, sample): the direct baseline.
+//   k_bp_plain      full matched filter with one FP32 sine/cosine pair per
+//                   (voxel, row, sample): the direct baselines, with the
+//                   library sincosf of the phase in radians, or (FAST) the
+//                   hardware sine/cosine of its fraction in revolutions.
 //   k_bp_rec        full matched filter evaluated per (voxel, row) as a
 //                   polynomial in the per-sample phase step r = exp(-j 2pi df d/c)
 //                   (Horner: one complex rotate-and-add, 4 FMAs, per sample),
@@ -142,14 +144,14 @@ __device__ __forceinline__ void store_voxels(
 // --------------------------------------------------------------------------
 // Plain: FP32 distance, FP32 phase theta = d * 2 pi f_j / c, sincosf.
 // --------------------------------------------------------------------------
-template <int VPT>
+template <int VPT, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_bp_plain(
     float2 *__restrict__ img, float2 *__restrict__ mom,
     const float *__restrict__ xl, const float *__restrict__ yl,
     const float *__restrict__ zl, const int32_t *__restrict__ vidx,
     int64_t nvox, const float *__restrict__ ant,
     const float2 *__restrict__ meas, const float *__restrict__ rxo, float k0,
-    float dk, int ny, int nz, int rows, int R, int F, int is_ti) {
+    float dk, float u0, float du, int ny, int nz, int rows, int R, int F, int is_ti) {
   const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
   float vx[VPT], vy[VPT], vz[VPT];
   load_voxels<VPT>(xl, yl, zl, vidx, nvox, ny, nz, base, vx, vy, vz);
@@ -177,10 +179,20 @@ __global__ __launch_bounds__(kBlock) void k_bp_plain(
     for (int j = 0; j < F; ++j) {
       const float2 m = row[j];
       const float kj = fmaf((float)j, dk, k0);
+      const float uj = fmaf((float)j, du, u0);
 #pragma unroll
       for (int k = 0; k < VPT; ++k) {
         float s, c;
-        sincosf(d[k] * kj, &s, &c);
+        if (FAST) {
+          // phase in revolutions, FP32; its fraction to the hardware sine
+          // and cosine (which take revolutions)
+          const float t = d[k] * uj;
+          const float fr = t - rintf(t);
+          s = __builtin_amdgcn_sinf(fr);
+          c = __builtin_amdgcn_cosf(fr);
+        } else {
+          sincosf(d[k] * kj, &s, &c);
+        }
         // m * exp(-j theta) = m * (c - j s)
         cr[k] = fmaf(m.x, c, fmaf(m.y, s, cr[k]));
         ci[k] = fmaf(m.y, c, fmaf(-m.x, s, ci[k]));
@@ -556,7 +568,9 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
   MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 &&
                   samples > 0,
               MMN_ERR_ARG, "mmn_bp_forward: sizes must be positive");
-  MMN_REQUIRE(variant == MMN_BP_PLAIN || variant == MMN_BP_REC, MMN_ERR_ARG,
+  MMN_REQUIRE(variant == MMN_BP_PLAIN || variant == MMN_BP_REC ||
+                  variant == MMN_BP_PLAIN_FAST,
+              MMN_ERR_ARG,
               "mmn_bp_forward: unknown variant %d", variant);
   if (K == 0) K = 64;
   MMN_REQUIRE(variant != MMN_BP_REC || K == 8 || K == 16 || K == 32 || K == 64,
@@ -574,20 +588,16 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
   float2 *mo = (float2 *)moment;
   const float2 *ms = (const float2 *)meas;
   const double inv_c = 1.0 / MMN_C0;
-  if (variant == MMN_BP_PLAIN) {
+  if (variant == MMN_BP_PLAIN || variant == MMN_BP_PLAIN_FAST) {
     const float k0 = (float)(2.0 * kPi * f0 * inv_c);
     const float dk = (float)(2.0 * kPi * df * inv_c);
-    const int vpt = env_int("MMN_PLAIN_VPT", 1);
-    if (vpt == 2)
-      hipLaunchKernelGGL(k_bp_plain<2>, dim3(mmn_blocks(nvox, kBlock * 2)),
-                         dim3(kBlock), 0, st, im, mo, x_locs, y_locs, z_locs,
-                         vox_idx, nvox, ant, ms, rx_off, k0, dk, ny, nz,
-                         num_rows, num_rx, samples, is_ti_radar);
-    else
-      hipLaunchKernelGGL(k_bp_plain<1>, dim3(mmn_blocks(nvox, kBlock)),
-                         dim3(kBlock), 0, st, im, mo, x_locs, y_locs, z_locs,
-                         vox_idx, nvox, ant, ms, rx_off, k0, dk, ny, nz,
-                         num_rows, num_rx, samples, is_ti_radar);
+    const float u0 = (float)(f0 * inv_c), du = (float)(df * inv_c);
+    const bool fast = variant == MMN_BP_PLAIN_FAST;
+    // VPT 1: the transcendental-bound loop gains nothing from a second voxel
+    auto kern = fast ? k_bp_plain<1, true> : k_bp_plain<1, false>;
+    hipLaunchKernelGGL(kern, dim3(mmn_blocks(nvox, kBlock)), dim3(kBlock), 0, st, im, mo, x_locs,
+                       y_locs, z_locs, vox_idx, nvox, ant, ms, rx_off, k0, dk, u0, du, ny,
+                       nz, num_rows, num_rx, samples, is_ti_radar);
     return mmn_check_launch("mmn_bp_forward(plain)");
   }
   RecArgs ra{im, mo, x_locs, y_locs, z_locs, vox_idx, nvox, ant, ms, rx_off,

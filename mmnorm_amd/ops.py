@@ -20,7 +20,8 @@ import torch
 from . import _lib
 
 C0 = 2.99792458e8
-VARIANTS = {"plain": _lib.MMN_BP_PLAIN, "rec": _lib.MMN_BP_REC}
+VARIANTS = {"plain": _lib.MMN_BP_PLAIN, "plain_fast": _lib.MMN_BP_PLAIN_FAST,
+            "rec": _lib.MMN_BP_REC}
 
 
 def _check(t, name, dtype, ndim=None, last=None):

@@ -1,8 +1,8 @@
 """Precision of the recurrence against the FP64 oracle (config 3 = config-1 inputs),
 and full-size (config 2) parity on a seeded voxel sample.
 
-Reported per K in {8, 16, 32, 64} and for the plain and reference-arithmetic
-kernels: relative L2 image error over the grid, max relative |I| error and max
+Reported per K in {8, 16, 32, 64} and for the plain (library sincosf and
+hardware sine/cosine) and reference-arithmetic kernels: relative L2 image error over the grid, max relative |I| error and max
 phase error (rad) on voxels above 5 % of the max, and the max normal-angle
 error (deg) on those voxels.  Set MMN_REPORT_DIR to also write the table as
 JSON there.
@@ -65,6 +65,9 @@ def test_precision_report_config3(cfg3):
     img, mom = ops.backproject(m, a, x, y, z, f0=sc.f0, df=sc.df, variant="plain",
                                moment=True)
     table["plain"] = metrics(img.cpu().numpy(), mom.cpu().numpy(), ref, rmom)
+    img, mom = ops.backproject(m, a, x, y, z, f0=sc.f0, df=sc.df, variant="plain_fast",
+                               moment=True)
+    table["plain_fast"] = metrics(img.cpu().numpy(), mom.cpu().numpy(), ref, rmom)
     img = ops.cuda_image(m, a, x, y, z, None, slope=np.float32(sc.slope),
                          wavelength=np.float32(sc.wavelength))
     table["reference_arithmetic"] = metrics(img.cpu().numpy(), None, ref, None)
@@ -100,3 +103,8 @@ def test_config2_full_size_sampled_parity(oracle):
                           f0=sc.f0, df=sc.df, variant="plain",
                           vox_idx=T(vidx, np.int32)).cpu().numpy()
     assert np.linalg.norm(sub - ref) / np.linalg.norm(ref) <= 2e-3
+    fast = ops.backproject(T(sc.meas, np.complex64), T(sc.ant, np.float32),
+                           *(T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs)),
+                           f0=sc.f0, df=sc.df, variant="plain_fast",
+                           vox_idx=T(vidx, np.int32)).cpu().numpy()
+    assert np.linalg.norm(fast - ref) / np.linalg.norm(ref) <= 2e-3
