@@ -216,6 +216,57 @@ int mmn_ccl6(int32_t *labels, int32_t *n_components, const uint8_t *mask,
 int mmn_label_sizes(int64_t *sizes, const int32_t *labels, int64_t n_vox,
                     int n_labels, void *stream);
 
+/*
+ * Ray marching and channel simulation on an RSDF isosurface (the consumer of
+ * the RSDF).  Common arguments: `grid` float [NX*NY*NZ] (z fastest), the
+ * SDF with undefined voxels at -50; bounding_box float[6] = (min x, y, z,
+ * max x, y, z), the coordinates of the grid's corner voxels; grid_res
+ * int[3]; eye float[3] the antenna position; rx_dir float[3] its boresight
+ * (rays leave along -rx_dir through a width x height pinhole grid spanning
+ * +-width_fov / +-height_fov degrees).  Host pointers for the small arrays.
+ *
+ * mmn_ray_matching -- replaces ray_tracing.ray_matching (ray_matching_cuda,
+ * ray_tracing_kernel.cu:425-494): writes, per ray in pixel order
+ * (x fastest), the rough hit, the index of its voxel's minimum corner (both
+ * -1 for no hit) and the unit direction, each float [width*height][3].
+ */
+int mmn_ray_matching(float *intersection_pos, float *voxel_position,
+                     float *directions, const float *grid, int width,
+                     int height, float width_fov, float height_fov,
+                     const float *bounding_box, const int *grid_res,
+                     const float *eye, const float *rx_dir, void *stream);
+
+/*
+ * mmn_simulate_channel -- the channel of one antenna from the isosurface
+ * grid - tau (_simulate_channels for one component,
+ * isosurface_optimization.py:137-245): march every ray, refine the hit by
+ * the interpolated value along the ray, d = 2 |rx_loc - hit| (+ 0.15 m if
+ * is_ti_radar), and channel[j] = sum over hits of exp(+j 2 pi (f0 + j df) d / c)
+ * (complex [samples]).  voxel_size is the refine step's voxel size (the
+ * reference's z_locs[1] - z_locs[0]); rx_loc double[3] (host).  Workspace
+ * from mmn_simulate_channel_workspace_size.
+ */
+size_t mmn_simulate_channel_workspace_size(int width, int height, int samples);
+int mmn_simulate_channel(float *channel, const float *grid, float tau, int width,
+                         int height, float width_fov, float height_fov,
+                         const float *bounding_box, const int *grid_res,
+                         const float *eye, const float *rx_dir,
+                         const double *rx_loc, float voxel_size, double f0,
+                         double df, int samples, int is_ti_radar,
+                         void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * mmn_channel_loss -- loss_fn of isosurface_optimization.py:248-280 for
+ * n_pairs (simulated, measured) channel pairs (complex [n_pairs][samples]
+ * each): the DFT of each zero-padded to n_fft, bins [bin_lo, bin_hi) (the
+ * measured profile zero from bin_hi_target on), each profile scaled to unit
+ * L2 norm, loss[p] = sum over the bins of | |sim| - |measured| |.  At most
+ * 1024 bins.
+ */
+int mmn_channel_loss(float *loss, const float *sim, const float *target,
+                     int n_pairs, int samples, int n_fft, int bin_lo, int bin_hi,
+                     int bin_hi_target, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,14 @@ SIGNATURES = {
     "mmn_ccl6_workspace_size": (_sz, [_c_int, _c_int, _c_int]),
     "mmn_ccl6": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_int, _p, _sz, _p]),
     "mmn_label_sizes": (_c_int, [_p, _p, _c_i64, _c_int, _p]),
+    "mmn_ray_matching": (_c_int, [_p, _p, _p, _p, _c_int, _c_int, _c_f, _c_f, _p, _p, _p,
+                                  _p, _p]),
+    "mmn_simulate_channel_workspace_size": (_sz, [_c_int, _c_int, _c_int]),
+    "mmn_simulate_channel": (_c_int, [_p, _p, _c_f, _c_int, _c_int, _c_f, _c_f, _p, _p, _p,
+                                      _p, _p, _c_f, _c_d, _c_d, _c_int, _c_int, _p, _sz,
+                                      _p]),
+    "mmn_channel_loss": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                  _c_int, _p]),
 }
 
 _lock = threading.Lock()

@@ -23,7 +23,7 @@ OUT = os.path.join(PKG, "libmmnorm_hip.so")
 ARCH = "gfx950"
 
 SOURCES = ["capi.hip", "bp_forward.hip", "sn_vote.hip", "bp_backward.hip",
-           "rsdf_bfs.hip", "segment.hip"]
+           "rsdf_bfs.hip", "segment.hip", "raymarch.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1,0 +1,517 @@
+// Ray marching through an RSDF isosurface and the mmWave channel it returns:
+// the consumer of the RSDF (ray_tracing_kernel.cu:60-421 and
+// isosurface_optimization.py:111-246).
+//
+// Per antenna, width x height rays leave the antenna through a pinhole
+// camera looking along -rx_dir (ray_tracing_kernel.cu:60-127); each ray is
+// sphere-traced through the trilinearly interpolated SDF grid minus the
+// isosurface offset tau, stepping by the SDF value and taking MIN_STEP where
+// a corner is undefined, until the value drops below one voxel
+// (ray_tracing_kernel.cu:206-421).  The rough hit is refined by one step of
+// the interpolated value along the ray (isosurface_optimization.py:111-135),
+// and the channel is the sum over hits of exp(+j 2 pi f d / c) with d twice
+// the antenna-to-hit distance (+0.15 m for the TI radar)
+// (isosurface_optimization.py:220-245).
+//
+// Arithmetic follows the reference expression by expression (FP32, no FMA
+// contraction; the pixel angle in FP64 as its mixed-precision expression
+// evaluates; distances in FP64 from FP32 hits), except where noted:
+//   * grid reads clamp their indices into the grid (the reference reads out
+//     of bounds for a hit on the far faces);
+//   * the channel sum uses the FP64-anchored phase recurrence of the
+//     backward (anchor every K = 32 samples) instead of one FP64 exp per
+//     (ray, sample), and FP32 lane sums reduced in FP64 in a fixed order.
+#include "mmn_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kMaxSteps = 1000;        // ray_tracing_kernel.cu:342
+constexpr float kUndef = -50.0f;       // UNDEF_VAL
+constexpr float kMinStep = 0.005f;     // MIN_STEP
+constexpr int kK = 32;                 // samples per wave of the synthesis
+
+struct RayGeom {
+  float fov_w, fov_h;       // degrees
+  float bmin[3], bmax[3];   // grid bounding box (voxel centres of the corners)
+  int res[3];
+  float eye[3], rxd[3];
+  int width, height;
+  float tau;                // isosurface offset subtracted from the grid
+};
+
+__device__ __forceinline__ float len3(float x, float y, float z) {
+#pragma clang fp contract(off)
+  return sqrtf(x * x + y * y + z * z);
+}
+
+// GenerateRay (ray_tracing_kernel.cu:60-127): the unit direction of ray p.
+__device__ void ray_dir(const RayGeom &g, int p, float &dx, float &dy, float &dz) {
+#pragma clang fp contract(off)
+  const float pi = 3.14159265358979323846f;
+  const float vfov = g.fov_h * (pi / 180.f);
+  const float hfov = g.fov_w * (pi / 180.f);
+  const float lw = len3(g.rxd[0], g.rxd[1], g.rxd[2]);
+  const float wx = g.rxd[0] / lw, wy = g.rxd[1] / lw, wz = g.rxd[2] / lw;
+  // up = (1, 0, 0): cross(up, w) = (0, -wz, wy)
+  const float cx = 0.0f * wz - 0.0f * wy, cy = 0.0f * wx - 1.0f * wz, cz = 1.0f * wy - 0.0f * wx;
+  const float lc = len3(cx, cy, cz);
+  const float ux = cx / lc, uy = cy / lc, uz = cz / lc;
+  const float vx = wy * uz - wz * uy, vy = wz * ux - wx * uz, vz = wx * uy - wy * ux;
+  const int x = p % g.width, y = p / g.width;
+  // (float)x + 0.5 is a double expression in the reference
+  const float ha = (float)(-(double)hfov + (double)(hfov * 2) * ((double)(float)x + 0.5) /
+                                               (double)(float)g.width);
+  const float va = (float)(-(double)vfov + (double)(vfov * 2) * ((double)(float)y + 0.5) /
+                                               (double)(float)g.height);
+  const float px = tanf(ha), py = tanf(va);
+  const float sx = px * ux + py * vx - wx;
+  const float sy = px * uy + py * vy - wy;
+  const float sz = px * uz + py * vz - wz;
+  const float ls = len3(sx, sy, sz);
+  dx = sx / ls;
+  dy = sy / ls;
+  dz = sz / ls;
+}
+
+__device__ __forceinline__ bool inside(const RayGeom &g, float x, float y, float z) {
+  return x >= g.bmin[0] && x <= g.bmax[0] && y >= g.bmin[1] && y <= g.bmax[1] &&
+         z >= g.bmin[2] && z <= g.bmax[2];
+}
+
+// Distance (ray_tracing_kernel.cu:146-196): along the ray to the box, from
+// outside; -1 if no face is ahead.  The direction is renormalised one
+// component at a time, each with the already updated components.
+__device__ float box_distance(const RayGeom &g, float px, float py, float pz, float dx,
+                              float dy, float dz) {
+#pragma clang fp contract(off)
+  float dist = -1.f;
+  dx = dx / len3(dx, dy, dz);
+  dy = dy / len3(dx, dy, dz);
+  dz = dz / len3(dx, dy, dz);
+  const float v[3] = {px, py, pz}, d[3] = {dx, dy, dz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (dist == -1) {
+      if ((v[a] < g.bmin[a]) && (d[a] > 0)) dist = (g.bmin[a] - v[a]) / d[a];
+      if ((v[a] > g.bmax[a]) && (d[a] < 0)) dist = (g.bmax[a] - v[a]) / d[a];
+    } else {
+      if ((v[a] < g.bmin[a]) && (d[a] > 0)) dist = fmaxf(dist, (g.bmin[a] - v[a]) / d[a]);
+      if ((v[a] > g.bmax[a]) && (d[a] < 0)) dist = fmaxf(dist, (g.bmax[a] - v[a]) / d[a]);
+    }
+  }
+  return dist;
+}
+
+// flat() of the reference: the index is formed in FP32 from truncated
+// coordinates (exact below 2^24 voxels); clamped into the grid here.
+__device__ __forceinline__ int64_t flat_idx(const RayGeom &g, float x, float y, float z) {
+#pragma clang fp contract(off)
+  const float fz = (float)(int)z, fy = (float)(int)y, fx = (float)(int)x;
+  const float f = fz + fy * (float)g.res[2] + fx * (float)g.res[2] * (float)g.res[1];
+  int64_t i = (int64_t)(int)f;
+  const int64_t V = (int64_t)g.res[0] * g.res[1] * g.res[2];
+  return i < 0 ? 0 : (i >= V ? V - 1 : i);
+}
+
+__device__ __forceinline__ float grid_at(const float *grid, const RayGeom &g, float x, float y,
+                                         float z) {
+#pragma clang fp contract(off)
+  return grid[flat_idx(g, x, y, z)] - g.tau;
+}
+
+// ValueAt (ray_tracing_kernel.cu:206-301) on grid - tau.
+__device__ float value_at(const float *grid, const RayGeom &g, float px, float py, float pz,
+                          float dx, float dy, float dz, bool first) {
+#pragma clang fp contract(off)
+  if (!inside(g, px, py, pz)) {
+    if (first) return box_distance(g, px, py, pz, dx, dy, dz) + 0.00001f;
+    return -1;
+  }
+  const float vs = (g.bmax[1] - g.bmin[1]) / (float)(g.res[1] - 1);
+  float ix = floorf((px - g.bmin[0]) / vs);
+  float iy = floorf((py - g.bmin[1]) / vs);
+  float iz = floorf((pz - g.bmin[2]) / vs);
+  if (ix == (g.bmax[0] - g.bmin[0]) / vs) ix = (g.bmax[0] - g.bmin[0]) / vs - 1;
+  if (iy == (g.bmax[1] - g.bmin[1]) / vs) iy = (g.bmax[1] - g.bmin[1]) / vs - 1;
+  if (iz == (g.bmax[2] - g.bmin[2]) / vs) iz = (g.bmax[2] - g.bmin[2]) / vs - 1;
+  const float lim = kUndef + 10;
+  const float g000 = grid_at(grid, g, ix, iy, iz), g010 = grid_at(grid, g, ix, iy + 1, iz);
+  const float g001 = grid_at(grid, g, ix, iy, iz + 1), g011 = grid_at(grid, g, ix, iy + 1, iz + 1);
+  const float g100 = grid_at(grid, g, ix + 1, iy, iz), g110 = grid_at(grid, g, ix + 1, iy + 1, iz);
+  const float g101 = grid_at(grid, g, ix + 1, iy, iz + 1);
+  const float g111 = grid_at(grid, g, ix + 1, iy + 1, iz + 1);
+  if (g000 < lim || g010 < lim || g001 < lim || g011 < lim || g100 < lim || g110 < lim ||
+      g101 < lim || g111 < lim)
+    return kMinStep;
+  const float tx = (px - (g.bmin[0] + ix * vs)) / vs;
+  const float c01 = (1.f - tx) * g000 + tx * g100;
+  const float c23 = (1.f - tx) * g010 + tx * g110;
+  const float c45 = (1.f - tx) * g001 + tx * g101;
+  const float c67 = (1.f - tx) * g011 + tx * g111;
+  const float ty = (py - (g.bmin[1] + iy * vs)) / vs;
+  const float c0 = (1.f - ty) * c01 + ty * c23;
+  const float c1 = (1.f - ty) * c45 + ty * c67;
+  const float tz = (pz - (g.bmin[2] + iz * vs)) / vs;
+  return (1.f - tz) * c0 + tz * c1;
+}
+
+// Intersect (ray_tracing_kernel.cu:305-421) for one ray: the rough hit and
+// the voxel index of its minimum corner, or all -1.
+__device__ void intersect(const float *grid, const RayGeom &g, float dx, float dy, float dz,
+                          float (&hit)[3], float (&vox)[3]) {
+#pragma clang fp contract(off)
+  const float vs = (g.bmax[1] - g.bmin[1]) / (float)(g.res[1] - 1);
+  float depth = 0;
+  bool first = true;
+  for (int s = 0; s < kMaxSteps; ++s) {
+    const float rx = g.eye[0] + depth * dx;
+    const float ry = g.eye[1] + depth * dy;
+    const float rz = g.eye[2] + depth * dz;
+    const float dist = value_at(grid, g, rx, ry, rz, dx, dy, dz, first);
+    first = false;
+    if (dist == -1) break;
+    if (dist < vs && dist != 0.0f) {
+      const float r[3] = {rx, ry, rz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float v = floorf((r[a] - g.bmin[a]) / vs);
+        if (v == (float)(g.res[a] - 1)) v = v - 1;
+        vox[a] = v;
+        hit[a] = r[a];
+      }
+      return;
+    }
+    depth += dist;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) hit[a] = vox[a] = -1.0f;
+}
+
+// ray_matching drop-in: rays, march, outputs as the reference's three
+// (width, height, 3) tensors in pixel order.
+__global__ __launch_bounds__(kBlock) void k_ray_match(float *__restrict__ hit_out,
+                                                      float *__restrict__ vox_out,
+                                                      float *__restrict__ dir_out,
+                                                      const float *__restrict__ grid,
+                                                      RayGeom g) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= g.width * g.height) return;
+  float dx, dy, dz, hit[3], vox[3];
+  ray_dir(g, p, dx, dy, dz);
+  intersect(grid, g, dx, dy, dz, hit, vox);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    hit_out[3 * (size_t)p + a] = hit[a];
+    vox_out[3 * (size_t)p + a] = vox[a];
+  }
+  dir_out[3 * (size_t)p + 0] = dx;
+  dir_out[3 * (size_t)p + 1] = dy;
+  dir_out[3 * (size_t)p + 2] = dz;
+}
+
+// Grid index of the refine step as torch indexing reads it: -1 was replaced
+// by 0 (isosurface_optimization.py:194-196); other negatives wrap.
+__device__ __forceinline__ int64_t torch_index(float v, int res) {
+  int64_t i = (int64_t)v;
+  if (i == -1) i = 0;
+  if (i < 0) i += res;
+  return i < 0 ? 0 : (i >= res ? res - 1 : i);
+}
+
+// Per ray: march, refine (isosurface_optimization.py:111-135,190-205) and
+// the round-trip distance in FP64 (isosurface_optimization.py:208-211);
+// NaN marks a ray without a hit.
+__global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
+                                                     const float *__restrict__ grid, RayGeom g,
+                                                     float vsz, double rx0, double rx1,
+                                                     double rx2, int is_ti) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= g.width * g.height) return;
+  float d[3], hit[3], vox[3];
+  ray_dir(g, p, d[0], d[1], d[2]);
+  intersect(grid, g, d[0], d[1], d[2], hit, vox);
+  if (vox[0] == -1.0f) {  // mask = voxel index x != -1
+    dist[p] = NAN;
+    return;
+  }
+  int64_t ii[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) ii[a] = torch_index(vox[a], g.res[a]);
+  const int64_t ny = g.res[1], nz = g.res[2];
+  auto at = [&](int64_t x, int64_t y, int64_t z) {
+    x = x >= g.res[0] ? g.res[0] - 1 : x;
+    y = y >= g.res[1] ? g.res[1] - 1 : y;
+    z = z >= g.res[2] ? g.res[2] - 1 : z;
+    return grid[(x * ny + y) * nz + z] - g.tau;
+  };
+  float t[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float vmin = g.bmin[a] + vox[a] * vsz;  // bounding_box_lower + index * voxel_size
+    t[a] = (hit[a] - vmin) / vsz;
+  }
+  const int64_t x = ii[0], y = ii[1], z = ii[2];
+  const float c01 = (1 - t[0]) * at(x, y, z) + t[0] * at(x + 1, y, z);
+  const float c23 = (1 - t[0]) * at(x, y + 1, z) + t[0] * at(x + 1, y + 1, z);
+  const float c45 = (1 - t[0]) * at(x, y, z + 1) + t[0] * at(x + 1, y, z + 1);
+  const float c67 = (1 - t[0]) * at(x, y + 1, z + 1) + t[0] * at(x + 1, y + 1, z + 1);
+  const float c0 = (1 - t[1]) * c01 + t[1] * c23;
+  const float c1 = (1 - t[1]) * c45 + t[1] * c67;
+  const float sv = (1 - t[2]) * c0 + t[2] * c1;
+  const double rx[3] = {rx0, rx1, rx2};
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float q = hit[a] + d[a] * sv;  // refined hit (mask = 1)
+    const double e = rx[a] - (double)q;
+    s += e * e;
+  }
+  double dd = sqrt(s) * 2;
+  if (is_ti) dd += 0.15;
+  dist[p] = dd;
+}
+
+// Channel synthesis: out[f] = sum over hit rays of exp(+j 2 pi (f0 + f df) d / c).
+// One wave per (block of kK samples, ray chunk); lanes walk the chunk's rays
+// and keep kK complex sums, from one FP64-reduced anchor and the two-term
+// recurrence z_{t+1} = 2 cos(w) z_t - z_{t-1} per ray; a butterfly reduces
+// the wave and the partials go to the workspace.
+__global__ __launch_bounds__(kBlock) void k_ray_synth(float2 *__restrict__ part,
+                                                      const double *__restrict__ dist,
+                                                      int nrays, int chunk, double f0c,
+                                                      double dfc, int F, int nfb,
+                                                      int64_t nitems) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const bool live = item < nitems;
+  const int64_t it = live ? item : nitems - 1;
+  const int fb = (int)(it % nfb);
+  const int c = (int)(it / nfb);
+  const int f_lo = fb * kK;
+  const int nf = min(kK, F - f_lo);
+  const double fc = fma((double)f_lo, dfc, f0c);
+  float accr[kK], acci[kK];
+#pragma unroll
+  for (int t = 0; t < kK; ++t) accr[t] = acci[t] = 0.0f;
+  const int r0 = c * chunk;
+  const int r1 = r0 + min(chunk, nrays - r0);
+  for (int r = r0 + lane; r < r1; r += kWave) {
+    const double d = dist[r];
+    const bool ok = d == d;  // hit
+    const double dd = ok ? d : 0.0;
+    float ar, ai, rr, ri;
+    phasor_hw(dd * fc, ar, ai);
+    phasor_hw(dd * dfc, rr, ri);
+    const float w = ok ? 1.0f : 0.0f;
+    float zr = w * ar, zi = -w * ai;          // exp(+j ...)
+    const float sr = rr, si = -ri;
+    float pr = zr * sr - zi * si, pi_ = zr * si + zi * sr;
+    const float c2 = sr + sr;
+#pragma unroll
+    for (int t = 0; t < kK; ++t) {
+      accr[t] += zr;
+      acci[t] += zi;
+      const float nr = fmaf(c2, pr, -zr), ni = fmaf(c2, pi_, -zi);
+      zr = pr;
+      zi = pi_;
+      pr = nr;
+      pi_ = ni;
+    }
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+#pragma unroll
+    for (int t = 0; t < kK; ++t) {
+      accr[t] += __shfl_xor(accr[t], o, kWave);
+      acci[t] += __shfl_xor(acci[t], o, kWave);
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int t = 0; t < kK; ++t)
+      if (lane == t && t < nf) part[(size_t)c * F + f_lo + t] = make_float2(accr[t], acci[t]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_synth_reduce(float2 *__restrict__ out,
+                                                         const float2 *__restrict__ part,
+                                                         int nchunks, int F) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= F) return;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    const float2 v = part[(size_t)c * F + f];
+    sr += v.x;
+    si += v.y;
+  }
+  out[f] = make_float2((float)sr, (float)si);
+}
+
+// Range-profile loss of isosurface_optimization.py:248-280 for B channel
+// pairs: DFT of each channel zero-padded to n_fft, bins [lo, hi) (target
+// bins >= hi_target zeroed first), each profile normalised to unit L2 norm,
+// loss = sum | |sim| - |target| |.  One block per pair; twiddles from the
+// exact integer phase k n mod n_fft.
+__global__ __launch_bounds__(kBlock) void k_channel_loss(float *__restrict__ loss,
+                                                         const float2 *__restrict__ sim,
+                                                         const float2 *__restrict__ tgt,
+                                                         int F, int n_fft, int lo, int hi,
+                                                         int hi_target) {
+  __shared__ double sh[3][kBlock];
+  __shared__ float mag[2][1024];
+  const float2 *a = sim + (size_t)blockIdx.x * F;
+  const float2 *b = tgt + (size_t)blockIdx.x * F;
+  const int nb = hi - lo;
+  for (int k = lo + threadIdx.x; k < hi; k += kBlock) {
+    double ar = 0, ai = 0, br = 0, bi = 0;
+    for (int n = 0; n < F; ++n) {
+      const int m = (int)(((int64_t)k * n) % n_fft);
+      double s, cth;
+      sincospi(2.0 * m / n_fft, &s, &cth);  // exp(-j 2 pi k n / N) = cth - j s
+      const float2 x = a[n], y = b[n];
+      ar += x.x * cth + x.y * s;
+      ai += x.y * cth - x.x * s;
+      br += y.x * cth + y.y * s;
+      bi += y.y * cth - y.x * s;
+    }
+    mag[0][k - lo] = (float)sqrt(ar * ar + ai * ai);
+    mag[1][k - lo] = k < hi_target ? (float)sqrt(br * br + bi * bi) : 0.0f;
+  }
+  __syncthreads();
+  double na = 0, nbv = 0;
+  for (int k = threadIdx.x; k < nb; k += kBlock) {
+    na += (double)mag[0][k] * mag[0][k];
+    nbv += (double)mag[1][k] * mag[1][k];
+  }
+  sh[0][threadIdx.x] = na;
+  sh[1][threadIdx.x] = nbv;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  const double ia = 1.0 / sqrt(sh[0][0]), ib = 1.0 / sqrt(sh[1][0]);
+  __syncthreads();
+  double l = 0;
+  for (int k = threadIdx.x; k < nb; k += kBlock) l += fabs(mag[0][k] * ia - mag[1][k] * ib);
+  sh[2][threadIdx.x] = l;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[blockIdx.x] = (float)sh[2][0];
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int synth_chunks(int nrays) {
+  // ~256 rays per wave, at most 1024 chunks
+  int c = (nrays + 255) / 256;
+  return c < 1 ? 1 : (c > 1024 ? 1024 : c);
+}
+
+bool geom_from(RayGeom &g, int width, int height, float fov_w, float fov_h,
+               const float *bbox, const int *res, const float *eye, const float *rx_dir,
+               float tau) {
+  if (width <= 0 || height <= 0 || (int64_t)width * height > INT32_MAX) return false;
+  g.fov_w = fov_w;
+  g.fov_h = fov_h;
+  for (int a = 0; a < 3; ++a) {
+    g.bmin[a] = bbox[a];
+    g.bmax[a] = bbox[3 + a];
+    g.res[a] = res[a];
+    g.eye[a] = eye[a];
+    g.rxd[a] = rx_dir[a];
+    if (res[a] < 2) return false;
+  }
+  g.width = width;
+  g.height = height;
+  g.tau = tau;
+  return true;
+}
+
+}  // namespace
+
+extern "C" int mmn_ray_matching(float *intersection_pos, float *voxel_position,
+                                float *directions, const float *grid, int width,
+                                int height, float width_fov, float height_fov,
+                                const float *bounding_box, const int *grid_res,
+                                const float *eye, const float *rx_dir, void *stream) {
+  MMN_REQUIRE(intersection_pos && voxel_position && directions && grid && bounding_box &&
+                  grid_res && eye && rx_dir,
+              MMN_ERR_NULL, "mmn_ray_matching: NULL pointer argument");
+  RayGeom g;
+  MMN_REQUIRE(geom_from(g, width, height, width_fov, height_fov, bounding_box, grid_res, eye,
+                        rx_dir, 0.0f),
+              MMN_ERR_ARG, "mmn_ray_matching: bad sizes");
+  const int n = width * height;
+  hipLaunchKernelGGL(k_ray_match, dim3(mmn_blocks(n, kBlock)), dim3(kBlock), 0,
+                     (hipStream_t)stream, intersection_pos, voxel_position, directions, grid,
+                     g);
+  return mmn_check_launch("mmn_ray_matching");
+}
+
+extern "C" size_t mmn_simulate_channel_workspace_size(int width, int height, int samples) {
+  if (width <= 0 || height <= 0 || samples <= 0) return 0;
+  const int n = width * height;
+  return align256(sizeof(double) * (size_t)n) +
+         align256(sizeof(float2) * (size_t)synth_chunks(n) * samples);
+}
+
+extern "C" int mmn_simulate_channel(float *channel, const float *grid, float tau, int width,
+                                    int height, float width_fov, float height_fov,
+                                    const float *bounding_box, const int *grid_res,
+                                    const float *eye, const float *rx_dir,
+                                    const double *rx_loc, float voxel_size, double f0,
+                                    double df, int samples, int is_ti_radar,
+                                    void *workspace, size_t workspace_bytes, void *stream) {
+  MMN_REQUIRE(channel && grid && bounding_box && grid_res && eye && rx_dir && rx_loc,
+              MMN_ERR_NULL, "mmn_simulate_channel: NULL pointer argument");
+  RayGeom g;
+  MMN_REQUIRE(geom_from(g, width, height, width_fov, height_fov, bounding_box, grid_res, eye,
+                        rx_dir, tau) &&
+                  samples > 0,
+              MMN_ERR_ARG, "mmn_simulate_channel: bad sizes");
+  const size_t need = mmn_simulate_channel_workspace_size(width, height, samples);
+  MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
+              "mmn_simulate_channel: workspace of %zu bytes needed", need);
+  const int n = width * height;
+  double *dist = (double *)workspace;
+  float2 *part = (float2 *)((char *)workspace + align256(sizeof(double) * (size_t)n));
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_ray_dist, dim3(mmn_blocks(n, kBlock)), dim3(kBlock), 0, st, dist, grid,
+                     g, voxel_size, rx_loc[0], rx_loc[1], rx_loc[2], is_ti_radar);
+  const int nch = synth_chunks(n);
+  const int chunk = (n + nch - 1) / nch;
+  const int nfb = (samples + kK - 1) / kK;
+  const int64_t nitems = (int64_t)nfb * nch;
+  const double inv_c = 1.0 / MMN_C0;
+  hipLaunchKernelGGL(k_ray_synth, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
+                     dim3(kBlock), 0, st, part, dist, n, chunk, f0 * inv_c, df * inv_c,
+                     samples, nfb, nitems);
+  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock)), dim3(kBlock), 0, st,
+                     (float2 *)channel, part, nch, samples);
+  return mmn_check_launch("mmn_simulate_channel");
+}
+
+extern "C" int mmn_channel_loss(float *loss, const float *sim, const float *target,
+                                int n_pairs, int samples, int n_fft, int bin_lo, int bin_hi,
+                                int bin_hi_target, void *stream) {
+  MMN_REQUIRE(loss && sim && target, MMN_ERR_NULL, "mmn_channel_loss: NULL pointer argument");
+  MMN_REQUIRE(n_pairs >= 0 && samples > 0 && n_fft >= samples && bin_lo >= 0 &&
+                  bin_lo < bin_hi && bin_hi <= n_fft && bin_hi - bin_lo <= 1024,
+              MMN_ERR_ARG, "mmn_channel_loss: bad sizes or bins");
+  if (n_pairs == 0) return 0;
+  hipLaunchKernelGGL(k_channel_loss, dim3(n_pairs), dim3(kBlock), 0, (hipStream_t)stream, loss,
+                     (const float2 *)sim, (const float2 *)target, samples, n_fft, bin_lo,
+                     bin_hi, bin_hi_target);
+  return mmn_check_launch("mmn_channel_loss");
+}

@@ -311,3 +311,76 @@ def label_sizes(labels, n_labels):
     _lib.call("mmn_label_sizes", sizes.data_ptr(), labels.data_ptr(), labels.numel(),
               int(n_labels), _lib.stream_handle(labels.device))
     return sizes
+
+
+def _f32s(vals, n):
+    import ctypes
+    vals = [float(v) for v in vals]
+    if len(vals) != n:
+        raise ValueError(f"expected {n} values, got {len(vals)}")
+    return (ctypes.c_float * n)(*vals)
+
+
+def ray_matching(grid, width, height, width_fov, height_fov, bounding_box, eye, rx_dir):
+    """Sphere-trace width x height rays from ``eye`` through the SDF ``grid``.
+
+    grid float32 (NX,NY,NZ); bounding_box (min x, y, z, max x, y, z); eye and
+    rx_dir 3 values.  Returns (intersection_pos, voxel_position, directions),
+    each float32 (width, height, 3) in the reference's pixel order; -1 where a
+    ray has no hit.
+    """
+    import ctypes
+    _lib.require_gpu()
+    _check(grid, "grid", torch.float32, 3)
+    dev = grid.device
+    n = int(width) * int(height)
+    hit, vox, dirs = (torch.empty((width, height, 3), dtype=torch.float32, device=dev)
+                      for _ in range(3))
+    res = (ctypes.c_int * 3)(*grid.shape)
+    _lib.call("mmn_ray_matching", hit.data_ptr(), vox.data_ptr(), dirs.data_ptr(),
+              grid.data_ptr(), int(width), int(height), float(width_fov), float(height_fov),
+              _f32s(bounding_box, 6), res, _f32s(eye, 3), _f32s(rx_dir, 3),
+              _lib.stream_handle(dev))
+    assert n == hit.shape[0] * hit.shape[1]
+    return hit, vox, dirs
+
+
+def simulate_channel(grid, tau, width, height, width_fov, height_fov, bounding_box, eye,
+                     rx_dir, rx_loc, voxel_size, f0, df, samples, is_ti=False):
+    """Channel (complex64, samples) that one antenna receives from the isosurface
+    grid - tau: rays marched, hits refined, sum of exp(+j 2 pi (f0 + j df) d / c)
+    over hits with d the FP64 round trip (+0.15 m if is_ti)."""
+    import ctypes
+    _lib.require_gpu()
+    _check(grid, "grid", torch.float32, 3)
+    dev = grid.device
+    ch = torch.empty(int(samples), dtype=torch.complex64, device=dev)
+    lib = _lib.load()
+    ws_bytes = int(lib.mmn_simulate_channel_workspace_size(int(width), int(height),
+                                                           int(samples)))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    res = (ctypes.c_int * 3)(*grid.shape)
+    rx = (ctypes.c_double * 3)(*[float(v) for v in rx_loc])
+    _lib.call("mmn_simulate_channel", ch.data_ptr(), grid.data_ptr(), float(tau), int(width),
+              int(height), float(width_fov), float(height_fov), _f32s(bounding_box, 6), res,
+              _f32s(eye, 3), _f32s(rx_dir, 3), rx, float(voxel_size), float(f0), float(df),
+              int(samples), int(bool(is_ti)), ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
+    return ch
+
+
+def channel_loss(sim, target, n_fft, bin_lo, bin_hi, bin_hi_target=None):
+    """Range-profile L1 loss of each (sim, target) channel pair: complex64 (B, F)
+    each -> float32 (B,)."""
+    _lib.require_gpu()
+    sim = sim.reshape(-1, sim.shape[-1]).contiguous()
+    target = target.reshape(-1, target.shape[-1]).contiguous()
+    _check(sim, "sim", torch.complex64, 2)
+    _check(target, "target", torch.complex64, 2)
+    if sim.shape != target.shape:
+        raise ValueError("sim and target must have the same shape")
+    B, F = sim.shape
+    out = torch.empty(B, dtype=torch.float32, device=sim.device)
+    hi_t = bin_hi if bin_hi_target is None else int(bin_hi_target)
+    _lib.call("mmn_channel_loss", out.data_ptr(), sim.data_ptr(), target.data_ptr(), B, F,
+              int(n_fft), int(bin_lo), int(bin_hi), hi_t, _lib.stream_handle(sim.device))
+    return out

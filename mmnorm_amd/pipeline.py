@@ -7,7 +7,8 @@ isosurface optimisation of the antenna positions through the backward kernel.
    p0 = p* + N(0, sigma) (sigma = 0.3 mm by default).
 2. Full-grid forward at p0 (recurrence kernel, vote moment fused).
 3. Normals on voxels above 5 % of the image maximum (py_sn_est_gpu.py:326),
-   unit-normalised (image_processor.py:396); 6-connected components on the
+   unit-normalised (image_processor.py:396); the Li binary image of |I|^2
+   removes the rest (sdf.py:276-279); 6-connected components on the
    GPU and the reference's size rule (sdf.py:88-119); per-component seeds
    (sdf.py:182-191); GPU RSDF BFS.
 4. Isosurface band B = {reached voxels with |rsdf| <= tau}.  The loss is the
@@ -44,6 +45,9 @@ import torch
 from . import ops, sdf, synth
 
 
+C0_M = 2.99792458e8
+
+
 def log(msg):
     print(f"[pipeline] {msg}", flush=True)
 
@@ -57,8 +61,59 @@ def _objective(I, kind):
     raise ValueError(f"unknown objective {kind!r}")
 
 
+def tau_search_stage(sc, p0, rsdf, labeled, valid, coords, rays, fov, num_rx, seed, rec):
+    """The RSDF consumer: the isosurface offset search of
+    isosurface_optimization.py:454-521 on the synthetic scene, with every
+    antenna's measured row as its channel, then the distance from the chosen
+    isosurface's points to the true surface (the scatterers)."""
+    from scipy.spatial import cKDTree
+    from . import isosurface
+    dev = rsdf.device
+    comps, bounds = isosurface.prepare_components(rsdf.cpu().numpy(), None, labeled, valid)
+    F = sc.cfg.n_freq
+    wl = C0_M / (sc.f0 + sc.df * np.arange(F))
+    bbox = [sc.x_locs[0], sc.y_locs[0], sc.z_locs[0], sc.x_locs[-1], sc.y_locs[-1],
+            sc.z_locs[-1]]
+    rx_locs = torch.from_numpy(np.asarray(p0, np.float64))
+    rx_dirs = torch.zeros(len(p0), 3)
+    rx_dirs[:, 2] = 1.0
+    chans = torch.from_numpy(np.ascontiguousarray(sc.meas)).to(dev)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    taus, grid, losses = isosurface.search_taus(
+        comps, rx_locs, rx_dirs, chans, bbox, wl, float(sc.z_locs[1] - sc.z_locs[0]), bounds, F,
+        is_real_data=False, width=rays, height=rays, width_fov=fov, height_fov=fov,
+        num_rx=num_rx, generator=torch.Generator().manual_seed(seed), skip=0, min_dist=0.2,
+        max_dist=0.4)
+    torch.cuda.synchronize()
+    rec["tau_search_s"] = time.perf_counter() - t
+    n_sim = int(np.prod(losses.shape)) * num_rx * comps.shape[-1]
+    rec["tau_search_sims"] = n_sim
+    tree = cKDTree(sc.pos)
+    sdf_np = comps.cpu().numpy()
+
+    def surf_err(tv):
+        pts = isosurface._get_final_point_cloud(sdf_np, np.atleast_1d(tv), coords)
+        return (float(np.median(tree.query(pts)[0])) if len(pts) else float("nan")), len(pts)
+
+    flat = grid.reshape(-1, grid.shape[-1]).numpy()
+    errs = [surf_err(tv)[0] for tv in flat]
+    chosen_err, npts = surf_err(taus)
+    rec.update({"taus": np.atleast_1d(taus).tolist(), "tau_grid_size": len(flat),
+                "chosen_surface_err_mm": chosen_err * 1e3, "chosen_points": npts,
+                "best_grid_surface_err_mm": float(np.nanmin(errs)) * 1e3,
+                "tau0_surface_err_mm": surf_err(np.zeros(grid.shape[-1]))[0] * 1e3})
+    log(f"tau search: {len(flat)} offsets x {num_rx} antennas x {rays}^2 rays in "
+        f"{rec['tau_search_s']:.2f} s ({rec['tau_search_s'] / n_sim * 1e3:.2f} ms per "
+        f"simulated channel); tau = {rec['taus']}, median distance of its {npts} points to the "
+        f"true surface {rec['chosen_surface_err_mm']:.2f} mm (tau = 0: "
+        f"{rec['tau0_surface_err_mm']:.2f} mm, best offset on the grid: "
+        f"{rec['best_grid_surface_err_mm']:.2f} mm)")
+
+
 def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
-        ckpt_every=10, resume=False, seed=7, objective="energy"):
+        ckpt_every=10, resume=False, seed=7, objective="energy", tau_search=False, rays=512,
+        fov=20.0, num_rx=50):
     dev = torch.device("cuda", torch.cuda.current_device())
     t0 = time.perf_counter()
     sc = synth.make_scene(cid, device=dev)
@@ -94,6 +149,12 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
     normals = torch.full((img.numel(), 3), float("nan"), dtype=torch.float64, device=dev)
     normals[vidx.long()] = n
     normals = normals.reshape(*img.shape, 3)
+    # binary image (sdf.py:276-279): voxels below the Li threshold of |I|^2
+    # lose their normals (the first exponent passes the volume test here:
+    # the whole grid spans less than 0.03 m^3)
+    thr, li_iters, _, below = ops.li_threshold(img.contiguous(), 2, below=True)
+    normals[below] = float("nan")
+    rec["li_threshold_iters"] = li_iters
     dx = float(sc.x_locs[1] - sc.x_locs[0])
     torch.cuda.synchronize()
     tc = time.perf_counter()
@@ -124,6 +185,8 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
         f"{rec['levels']} levels); band {rec['band_voxels']} voxels")
     if bidx.numel() == 0:
         raise RuntimeError("empty isosurface band")
+    if tau_search:
+        tau_search_stage(sc, p0, rsdf, labeled, valid, coords, rays, fov, num_rx, seed, rec)
 
     # 4. optimisation over the band
     pos = T(p0, np.float32).requires_grad_(True)
@@ -217,9 +280,15 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--ckpt-every", type=int, default=10)
     ap.add_argument("--resume", action="store_true")
+    ap.add_argument("--tau-search", action="store_true",
+                    help="run the isosurface offset search (ray-marched channels) after the BFS")
+    ap.add_argument("--rays", type=int, default=512, help="rays per side per antenna")
+    ap.add_argument("--fov", type=float, default=20.0, help="ray half-angle, degrees")
+    ap.add_argument("--num-rx", type=int, default=50)
     a = ap.parse_args(argv)
     rec = run(a.config, a.steps, a.sigma, a.tau_vox, a.lr, a.K, a.out, a.ckpt_every, a.resume,
-              objective=a.objective)
+              objective=a.objective, tau_search=a.tau_search, rays=a.rays, fov=a.fov,
+              num_rx=a.num_rx)
     print(json.dumps({k: v for k, v in rec.items() if k != "history"}), flush=True)
     return 0
 

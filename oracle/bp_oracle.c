@@ -24,6 +24,10 @@
  *   or_li_threshold  restates the Li iteration of skimage.filters.threshold_li
  *                  (Li & Tam's minimum cross-entropy threshold, called at
  *                  sdf.py:60-70) in FP64 with sequential sums.
+ *   or_ray_matching, or_simulate_channel  restate the ray marcher
+ *                  (ray_tracing_kernel.cu:60-421) and the channel simulation
+ *                  of isosurface_optimization.py:111-245 (refine, FP64
+ *                  distance, one FP64 exp per hit and sample).
  *   or_ccl6        6-connected component labelling (the partition of
  *                  cc3d.connected_components(mask, connectivity=6),
  *                  sdf.py:89) by flood fill in C-order scan, numbered in
@@ -34,7 +38,10 @@
  * equals a NumPy brute force and the point-scatterer closed form;
  * or_bfs_fifo equals the deque restatement and the constant-normal closed
  * form; or_li_threshold meets its fixed-point equation and the constant and
- * two-level closed forms; or_ccl6 equals scipy.ndimage.label.
+ * two-level closed forms; or_ccl6 equals scipy.ndimage.label; the ray
+ * marcher equals its pure-Python restatement and hits a plane SDF where the
+ * geometry puts it (a single vertical ray exactly), and a single hit's
+ * channel is exp(+j 2 pi f d / c).
  */
 #include <complex.h>
 #include <math.h>
@@ -382,4 +389,211 @@ int or_ccl6(int32_t *labels, const uint8_t *mask, int nx, int ny, int nz) {
   }
   free(q);
   return n;
+}
+
+/* ------------------------------------------------------------------ rays -- */
+typedef struct {
+  float fov_w, fov_h, bmin[3], bmax[3];
+  int res[3];
+  float eye[3], rxd[3];
+  int width, height;
+  float tau;
+} RayGeom;
+
+static float len3f(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+static void ray_dir(const RayGeom *g, int p, float *d) {
+  const float pi = 3.14159265358979323846f;
+  const float vfov = g->fov_h * (pi / 180.f), hfov = g->fov_w * (pi / 180.f);
+  const float lw = len3f(g->rxd[0], g->rxd[1], g->rxd[2]);
+  const float wx = g->rxd[0] / lw, wy = g->rxd[1] / lw, wz = g->rxd[2] / lw;
+  const float cx = 0.0f * wz - 0.0f * wy, cy = 0.0f * wx - 1.0f * wz, cz = 1.0f * wy - 0.0f * wx;
+  const float lc = len3f(cx, cy, cz);
+  const float ux = cx / lc, uy = cy / lc, uz = cz / lc;
+  const float vx = wy * uz - wz * uy, vy = wz * ux - wx * uz, vz = wx * uy - wy * ux;
+  const int x = p % g->width, y = p / g->width;
+  const float ha = (float)(-(double)hfov + (double)(hfov * 2) * ((double)(float)x + 0.5) /
+                                               (double)(float)g->width);
+  const float va = (float)(-(double)vfov + (double)(vfov * 2) * ((double)(float)y + 0.5) /
+                                               (double)(float)g->height);
+  const float px = tanf(ha), py = tanf(va);
+  const float sx = px * ux + py * vx - wx, sy = px * uy + py * vy - wy, sz = px * uz + py * vz - wz;
+  const float ls = len3f(sx, sy, sz);
+  d[0] = sx / ls;
+  d[1] = sy / ls;
+  d[2] = sz / ls;
+}
+
+static float box_distance(const RayGeom *g, const float *v, float dx, float dy, float dz) {
+  float dist = -1.f;
+  dx = dx / len3f(dx, dy, dz);
+  dy = dy / len3f(dx, dy, dz);
+  dz = dz / len3f(dx, dy, dz);
+  const float d[3] = {dx, dy, dz};
+  for (int a = 0; a < 3; ++a) {
+    if (dist == -1) {
+      if ((v[a] < g->bmin[a]) && (d[a] > 0)) dist = (g->bmin[a] - v[a]) / d[a];
+      if ((v[a] > g->bmax[a]) && (d[a] < 0)) dist = (g->bmax[a] - v[a]) / d[a];
+    } else {
+      if ((v[a] < g->bmin[a]) && (d[a] > 0)) dist = fmaxf(dist, (g->bmin[a] - v[a]) / d[a]);
+      if ((v[a] > g->bmax[a]) && (d[a] < 0)) dist = fmaxf(dist, (g->bmax[a] - v[a]) / d[a]);
+    }
+  }
+  return dist;
+}
+
+static float grid_at(const float *grid, const RayGeom *g, float x, float y, float z) {
+  const float fz = (float)(int)z, fy = (float)(int)y, fx = (float)(int)x;
+  const float f = fz + fy * (float)g->res[2] + fx * (float)g->res[2] * (float)g->res[1];
+  int64_t i = (int64_t)(int)f;
+  const int64_t V = (int64_t)g->res[0] * g->res[1] * g->res[2];
+  if (i < 0) i = 0;
+  if (i >= V) i = V - 1;
+  return grid[i] - g->tau;
+}
+
+static float value_at(const float *grid, const RayGeom *g, const float *p, const float *d,
+                      int first) {
+  if (!(p[0] >= g->bmin[0] && p[0] <= g->bmax[0] && p[1] >= g->bmin[1] &&
+        p[1] <= g->bmax[1] && p[2] >= g->bmin[2] && p[2] <= g->bmax[2])) {
+    if (first) return box_distance(g, p, d[0], d[1], d[2]) + 0.00001f;
+    return -1;
+  }
+  const float vs = (g->bmax[1] - g->bmin[1]) / (float)(g->res[1] - 1);
+  float ix = floorf((p[0] - g->bmin[0]) / vs);
+  float iy = floorf((p[1] - g->bmin[1]) / vs);
+  float iz = floorf((p[2] - g->bmin[2]) / vs);
+  if (ix == (g->bmax[0] - g->bmin[0]) / vs) ix = (g->bmax[0] - g->bmin[0]) / vs - 1;
+  if (iy == (g->bmax[1] - g->bmin[1]) / vs) iy = (g->bmax[1] - g->bmin[1]) / vs - 1;
+  if (iz == (g->bmax[2] - g->bmin[2]) / vs) iz = (g->bmax[2] - g->bmin[2]) / vs - 1;
+  const float lim = -50.0f + 10;
+  const float g000 = grid_at(grid, g, ix, iy, iz), g010 = grid_at(grid, g, ix, iy + 1, iz);
+  const float g001 = grid_at(grid, g, ix, iy, iz + 1), g011 = grid_at(grid, g, ix, iy + 1, iz + 1);
+  const float g100 = grid_at(grid, g, ix + 1, iy, iz), g110 = grid_at(grid, g, ix + 1, iy + 1, iz);
+  const float g101 = grid_at(grid, g, ix + 1, iy, iz + 1);
+  const float g111 = grid_at(grid, g, ix + 1, iy + 1, iz + 1);
+  if (g000 < lim || g010 < lim || g001 < lim || g011 < lim || g100 < lim || g110 < lim ||
+      g101 < lim || g111 < lim)
+    return 0.005f;
+  const float tx = (p[0] - (g->bmin[0] + ix * vs)) / vs;
+  const float c01 = (1.f - tx) * g000 + tx * g100, c23 = (1.f - tx) * g010 + tx * g110;
+  const float c45 = (1.f - tx) * g001 + tx * g101, c67 = (1.f - tx) * g011 + tx * g111;
+  const float ty = (p[1] - (g->bmin[1] + iy * vs)) / vs;
+  const float c0 = (1.f - ty) * c01 + ty * c23, c1 = (1.f - ty) * c45 + ty * c67;
+  const float tz = (p[2] - (g->bmin[2] + iz * vs)) / vs;
+  return (1.f - tz) * c0 + tz * c1;
+}
+
+static void intersect(const float *grid, const RayGeom *g, const float *d, float *hit,
+                      float *vox) {
+  const float vs = (g->bmax[1] - g->bmin[1]) / (float)(g->res[1] - 1);
+  float depth = 0;
+  int first = 1;
+  for (int s = 0; s < 1000; ++s) {
+    const float r[3] = {g->eye[0] + depth * d[0], g->eye[1] + depth * d[1],
+                        g->eye[2] + depth * d[2]};
+    const float dist = value_at(grid, g, r, d, first);
+    first = 0;
+    if (dist == -1) break;
+    if (dist < vs && dist != 0.0f) {
+      for (int a = 0; a < 3; ++a) {
+        float v = floorf((r[a] - g->bmin[a]) / vs);
+        if (v == (float)(g->res[a] - 1)) v = v - 1;
+        vox[a] = v;
+        hit[a] = r[a];
+      }
+      return;
+    }
+    depth += dist;
+  }
+  for (int a = 0; a < 3; ++a) hit[a] = vox[a] = -1.0f;
+}
+
+static void geom(RayGeom *g, int width, int height, float fov_w, float fov_h,
+                 const float *bbox, const int *res, const float *eye, const float *rxd,
+                 float tau) {
+  g->fov_w = fov_w;
+  g->fov_h = fov_h;
+  for (int a = 0; a < 3; ++a) {
+    g->bmin[a] = bbox[a];
+    g->bmax[a] = bbox[3 + a];
+    g->res[a] = res[a];
+    g->eye[a] = eye[a];
+    g->rxd[a] = rxd[a];
+  }
+  g->width = width;
+  g->height = height;
+  g->tau = tau;
+}
+
+void or_ray_matching(float *hit, float *vox, float *dir, const float *grid, int width,
+                     int height, float fov_w, float fov_h, const float *bbox, const int *res,
+                     const float *eye, const float *rxd, float tau) {
+  RayGeom g;
+  geom(&g, width, height, fov_w, fov_h, bbox, res, eye, rxd, tau);
+  for (int p = 0; p < width * height; ++p) {
+    ray_dir(&g, p, dir + 3 * (int64_t)p);
+    intersect(grid, &g, dir + 3 * (int64_t)p, hit + 3 * (int64_t)p, vox + 3 * (int64_t)p);
+  }
+}
+
+static int64_t torch_index(float v, int res) {
+  int64_t i = (int64_t)v;
+  if (i == -1) i = 0;
+  if (i < 0) i += res;
+  return i < 0 ? 0 : (i >= res ? res - 1 : i);
+}
+
+/* Per-ray FP64 round-trip distances (NaN: no hit) and the FP64 channel
+ * sum_hits exp(+j 2 pi (f0 + j df) d / c) as interleaved doubles. */
+void or_simulate_channel(double *chan, double *dist, const float *grid, float tau, int width,
+                         int height, float fov_w, float fov_h, const float *bbox,
+                         const int *res, const float *eye, const float *rxd,
+                         const double *rx, float vsz, double f0, double df, int F,
+                         int is_ti) {
+  RayGeom g;
+  geom(&g, width, height, fov_w, fov_h, bbox, res, eye, rxd, tau);
+  const int64_t ny = res[1], nz = res[2];
+  for (int j = 0; j < 2 * F; ++j) chan[j] = 0.0;
+  for (int p = 0; p < width * height; ++p) {
+    float d[3], hit[3], vox[3];
+    ray_dir(&g, p, d);
+    intersect(grid, &g, d, hit, vox);
+    dist[p] = NAN;
+    if (vox[0] == -1.0f) continue;
+    int64_t ii[3];
+    for (int a = 0; a < 3; ++a) ii[a] = torch_index(vox[a], res[a]);
+#define AT(X, Y, Z)                                                                    \
+  (grid[(((X) >= res[0] ? res[0] - 1 : (X)) * ny + ((Y) >= res[1] ? res[1] - 1 : (Y))) * nz + \
+        ((Z) >= res[2] ? res[2] - 1 : (Z))] -                                          \
+   tau)
+    float t[3];
+    for (int a = 0; a < 3; ++a) {
+      const float vmin = g.bmin[a] + vox[a] * vsz;
+      t[a] = (hit[a] - vmin) / vsz;
+    }
+    const int64_t x = ii[0], y = ii[1], z = ii[2];
+    const float c01 = (1 - t[0]) * AT(x, y, z) + t[0] * AT(x + 1, y, z);
+    const float c23 = (1 - t[0]) * AT(x, y + 1, z) + t[0] * AT(x + 1, y + 1, z);
+    const float c45 = (1 - t[0]) * AT(x, y, z + 1) + t[0] * AT(x + 1, y, z + 1);
+    const float c67 = (1 - t[0]) * AT(x, y + 1, z + 1) + t[0] * AT(x + 1, y + 1, z + 1);
+#undef AT
+    const float c0 = (1 - t[1]) * c01 + t[1] * c23;
+    const float c1 = (1 - t[1]) * c45 + t[1] * c67;
+    const float sv = (1 - t[2]) * c0 + t[2] * c1;
+    double s = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const float q = hit[a] + d[a] * sv;
+      const double e = rx[a] - (double)q;
+      s += e * e;
+    }
+    double dd = sqrt(s) * 2;
+    if (is_ti) dd += 0.15;
+    dist[p] = dd;
+    for (int j = 0; j < F; ++j) {
+      const double ph = 2.0 * PI_D * dd * (f0 + j * df) / C0;
+      chan[2 * j] += cos(ph);
+      chan[2 * j + 1] += sin(ph);
+    }
+  }
 }

@@ -65,6 +65,11 @@ def lib() -> ctypes.CDLL:
         L.or_li_threshold.restype = i32
         L.or_ccl6.argtypes = [p, p, i32, i32, i32]
         L.or_ccl6.restype = i32
+        L.or_ray_matching.argtypes = [p, p, p, p, i32, i32, f32, f32, p, p, p, p, f32]
+        L.or_ray_matching.restype = None
+        L.or_simulate_channel.argtypes = [p, p, p, f32, i32, i32, f32, f32, p, p, p, p, p,
+                                          f32, f64, f64, i32, i32]
+        L.or_simulate_channel.restype = None
         for fn in (L.or_bp_ref, L.or_sn_ref, L.or_bp_f64, L.or_bp_cpu_imaging):
             fn.restype = None
         _lib = L
@@ -199,6 +204,65 @@ def ccl6(mask):
     if n < 0:
         raise MemoryError("or_ccl6: allocation failed")
     return lab, n
+
+
+def _ray_args(grid, bbox, eye, rx_dir):
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    bb = np.ascontiguousarray(bbox, dtype=np.float32)
+    res = np.ascontiguousarray(g.shape, dtype=np.int32)
+    e = np.ascontiguousarray(eye, dtype=np.float32)
+    r = np.ascontiguousarray(rx_dir, dtype=np.float32)
+    return g, bb, res, e, r
+
+
+def ray_matching(grid, width, height, width_fov, height_fov, bbox, eye, rx_dir, tau=0.0):
+    """ray_tracing_kernel.cu:60-421: (hit, voxel, direction), each (width*height, 3)."""
+    g, bb, res, e, r = _ray_args(grid, bbox, eye, rx_dir)
+    n = width * height
+    hit, vox, d = (np.empty((n, 3), np.float32) for _ in range(3))
+    lib().or_ray_matching(_ptr(hit), _ptr(vox), _ptr(d), _ptr(g), width, height,
+                          float(width_fov), float(height_fov), _ptr(bb), _ptr(res), _ptr(e),
+                          _ptr(r), float(tau))
+    return hit, vox, d
+
+
+def simulate_channel(grid, tau, width, height, width_fov, height_fov, bbox, eye, rx_dir,
+                     rx_loc, voxel_size, f0, df, F, is_ti=False):
+    """isosurface_optimization.py:137-245 for one component, FP64 exp per hit and
+    sample: (channel complex128 (F,), distances float64 (width*height,), NaN = no hit)."""
+    g, bb, res, e, r = _ray_args(grid, bbox, eye, rx_dir)
+    rx = np.ascontiguousarray(rx_loc, dtype=np.float64)
+    ch = np.empty(2 * F, np.float64)
+    dist = np.empty(width * height, np.float64)
+    lib().or_simulate_channel(_ptr(ch), _ptr(dist), _ptr(g), float(tau), width, height,
+                              float(width_fov), float(height_fov), _ptr(bb), _ptr(res), _ptr(e),
+                              _ptr(r), _ptr(rx), float(np.float32(voxel_size)), float(f0),
+                              float(df), int(F), int(bool(is_ti)))
+    return ch[0::2] + 1j * ch[1::2], dist
+
+
+def py_ray_dir(width, height, width_fov, height_fov, rx_dir, p):
+    """GenerateRay (ray_tracing_kernel.cu:60-127) for ray p with NumPy float32 scalars."""
+    f = np.float32
+    pi = f(3.14159265358979323846)
+    vfov, hfov = f(height_fov) * (pi / f(180)), f(width_fov) * (pi / f(180))
+    rd = [f(v) for v in rx_dir]
+
+    def ln(x, y, z):
+        return np.sqrt(x * x + y * y + z * z, dtype=np.float32)
+    lw = ln(*rd)
+    w = [rd[0] / lw, rd[1] / lw, rd[2] / lw]
+    c = [f(0) * w[2] - f(0) * w[1], f(0) * w[0] - f(1) * w[2], f(1) * w[1] - f(0) * w[0]]
+    lc = ln(*c)
+    u = [c[0] / lc, c[1] / lc, c[2] / lc]
+    v = [w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]]
+    x, y = p % width, p // width
+    ha = f(-float(hfov) + float(hfov * f(2)) * (float(f(x)) + 0.5) / float(f(width)))
+    va = f(-float(vfov) + float(vfov * f(2)) * (float(f(y)) + 0.5) / float(f(height)))
+    px, py = np.tan(ha, dtype=np.float32), np.tan(va, dtype=np.float32)
+    s = [px * u[a] + py * v[a] - w[a] for a in range(3)]
+    ls = ln(*s)
+    return np.array([s[0] / ls, s[1] / ls, s[2] / ls], np.float32)
 
 
 def py_li_threshold(img, exponent, max_iter=1000):

@@ -26,3 +26,26 @@ def test_size_rule_matches_the_reference_rule():
     areas = np.array([0, 4501, 4500, 10], float)
     assert list(sdf.size_rule(areas, dx)) == [2]
     assert len(sdf.size_rule(np.array([0, 10], float), dx)) == 0
+
+
+def test_select_tau_one_component():
+    from mmnorm_amd import isosurface
+    taus = np.linspace(-0.02, 0.02, 41)[:, None]
+    # two dips; the deeper one first, a nearly as deep one later (closer to
+    # the top): the later peak of the flipped loss is chosen
+    x = np.arange(41)
+    loss = 50 - 30 * np.exp(-((x - 10) / 2.0) ** 2) - 28 * np.exp(-((x - 30) / 2.0) ** 2)
+    assert isosurface._select_tau(taus, loss)[0] == pytest.approx(taus[30, 0])
+    # a single dip
+    loss = 50 - 30 * np.exp(-((x - 12) / 2.0) ** 2)
+    assert isosurface._select_tau(taus, loss)[0] == pytest.approx(taus[12, 0])
+
+
+def test_range_window_bins():
+    from mmnorm_amd import isosurface
+    f = 77e9 + np.arange(512) * (4e9 / 512)
+    isosurface.set_range_window(512, 2.99792458e8 / f)
+    d = isosurface.DISTANCES
+    assert isosurface.MIN_DIST_IDX == np.count_nonzero(d < 0.35)
+    assert isosurface.MAX_DIST_IDX == np.count_nonzero(d < 0.55)
+    assert 0 < isosurface.MIN_DIST_IDX < isosurface.MAX_DIST_IDX < 512

@@ -306,3 +306,79 @@ def test_li_oracle_closed_forms(oracle):
     thr, it = oracle.li_threshold(img, 1.0)
     assert it == 2
     assert abs(thr - (2.0 + 9.5 / np.log(20.0))) < 1e-14
+
+
+# ------------------------------------------------------------- ray marching --
+def _plane_grid(res, vs, z0, bmin_z):
+    """SDF of the plane z = z0 on a grid whose z axis starts at bmin_z."""
+    z = bmin_z + np.arange(res[2]) * vs
+    return np.broadcast_to((z - z0).astype(np.float32), res).copy()
+
+
+@pytest.mark.parametrize("rx_dir", [(0.0, 0.0, 1.0), (0.1, -0.2, 1.0)])
+def test_ray_dirs_c_equals_python(oracle, rx_dir):
+    W, H = 7, 5
+    grid = np.zeros((4, 4, 4), np.float32)
+    _, _, d = oracle.ray_matching(grid, W, H, 40.0, 30.0, [0, 0, 0, 0.006, 0.006, 0.006],
+                                  [1.0, 1.0, 1.0], rx_dir)
+    for p in range(W * H):
+        assert np.array_equal(d[p], oracle.py_ray_dir(W, H, 40.0, 30.0, rx_dir, p))
+
+
+def test_ray_dirs_geometry(oracle):
+    W, H = 9, 7
+    grid = np.zeros((4, 4, 4), np.float32)
+    _, _, d = oracle.ray_matching(grid, W, H, 50.0, 35.0, [0, 0, 0, 0.006, 0.006, 0.006],
+                                  [1.0, 1.0, 1.0], (0.0, 0.0, 1.0))
+    assert np.max(np.abs(np.linalg.norm(d, axis=1) - 1)) < 1e-6
+    # the centre pixel looks along -rx_dir
+    c = (H // 2) * W + W // 2
+    assert np.allclose(d[c], [0, 0, -1], atol=1e-7)
+    # pixels mirrored in x (the u = (0, -1, 0) axis for this boresight) mirror y
+    img = d.reshape(H, W, 3)
+    assert np.allclose(img[:, ::-1, 1], -img[:, :, 1], atol=1e-6)
+    assert np.allclose(img[:, ::-1, 0], img[:, :, 0], atol=1e-6)
+
+
+def test_single_vertical_ray_hits_plane(oracle):
+    """One ray straight down onto the plane z = z0 (an exact SDF, which the
+    trilinear interpolation reproduces): the refined hit lies on the plane,
+    d = 2 (eye_z - z0), and the channel is exp(+j 2 pi f d / c)."""
+    vs, res = 2e-3, (16, 16, 32)
+    bmin = np.array([-0.015, -0.015, 0.0])
+    bbox = list(bmin) + list(bmin + vs * (np.array(res) - 1))
+    z0 = 0.0311
+    grid = _plane_grid(res, vs, z0, 0.0)
+    eye = [0.0013, -0.0021, 0.25]
+    f0, df, F = 77e9, 4e9 / 63, 64
+    ch, dist = oracle.simulate_channel(grid, 0.0, 1, 1, 40.0, 40.0, bbox, eye, (0, 0, 1), eye,
+                                       vs, f0, df, F)
+    assert abs(dist[0] - 2 * (eye[2] - z0)) < 2e-6
+    f = f0 + df * np.arange(F)
+    assert np.allclose(ch, np.exp(2j * np.pi * dist[0] * f / synth.C0), atol=1e-12)
+    hit, vox, d = oracle.ray_matching(grid, 1, 1, 40.0, 40.0, bbox, eye, (0, 0, 1))
+    assert np.array_equal(d[0], np.float32([0, 0, -1]))
+    assert abs(hit[0, 2] - z0) < vs and vox[0, 2] == np.floor((hit[0, 2] - 0.0) / np.float32(vs))
+
+
+def test_tau_offsets_the_isosurface(oracle):
+    """grid - tau moves the hit plane by tau.  (A march that lands exactly on a
+    zero of the SDF never counts it as a hit and stalls there, as in the
+    reference, ray_tracing_kernel.cu:391 -- so tau is not a round number.)"""
+    vs, res = 2e-3, (8, 8, 32)
+    bbox = [-0.007, -0.007, 0.0] + [-0.007 + 7 * vs, -0.007 + 7 * vs, 31 * vs]
+    grid = _plane_grid(res, vs, 0.02, 0.0)
+    eye = [0.0, 0.0, 0.2]
+    _, d0 = oracle.simulate_channel(grid, 0.0, 1, 1, 30, 30, bbox, eye, (0, 0, 1), eye, vs,
+                                    77e9, 1e7, 4)
+    _, d1 = oracle.simulate_channel(grid, 0.0103, 1, 1, 30, 30, bbox, eye, (0, 0, 1), eye, vs,
+                                    77e9, 1e7, 4)
+    assert abs((d0[0] - d1[0]) - 2 * 0.0103) < 4e-6
+
+
+def test_rays_leaving_the_box_have_no_hit(oracle):
+    vs, res = 2e-3, (8, 8, 8)
+    bbox = [0, 0, 0, 7 * vs, 7 * vs, 7 * vs]
+    grid = np.full(res, 1.0, np.float32)   # no surface inside
+    hit, vox, _ = oracle.ray_matching(grid, 4, 4, 20, 20, bbox, [0.007, 0.007, 0.3], (0, 0, 1))
+    assert np.all(hit == -1) and np.all(vox == -1)
