@@ -185,6 +185,13 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
         f"{rec['levels']} levels); band {rec['band_voxels']} voxels")
     if bidx.numel() == 0:
         raise RuntimeError("empty isosurface band")
+    if out:
+        # the reference's stage file after the RSDF (sdf.py:226-253)
+        from . import formats
+        formats.save_sdf_file(os.path.join(out, f"sdf_config{cid}.npz"), formats.sdf_record(
+            (img / A).cpu().numpy(), sc.x_locs, sc.y_locs, sc.z_locs, p0, sc.meas[:, :, None],
+            nrm_np, rsdf.cpu().numpy(), labeled, valid, coords, sc.f0,
+            sc.f0 + sc.df * cfg.n_freq, cfg.n_freq))
     if tau_search:
         tau_search_stage(sc, p0, rsdf, labeled, valid, coords, rays, fov, num_rx, seed, rec)
 

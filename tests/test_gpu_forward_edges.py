@@ -1,0 +1,84 @@
+"""Edge cases of the recurrence forward against the FP64 oracle: sample counts
+that leave a short last block, several RX offsets with the TI range offset,
+one-voxel and empty voxel lists, repeated voxels, and every row split."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from mmnorm_amd import ops, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def dev():
+    return torch.device("cuda", 0)
+
+
+def T(a, dt):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev())
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+@pytest.fixture(scope="module")
+def scene():
+    """12x12 TX positions x 3 RX offsets (bistatic), 77 samples, 12x10x8 voxels."""
+    cfg = synth.Config(1, 12, 2e-3, 77, (12, 10, 8), 2e-3, 20.0, "points", 5)
+    sc = synth.make_scene(1, cfg=cfg, seed=31)
+    rxo = np.array([[0.0, 0.0, 0.0], [-3.1e-3, 0.0, 0.0], [0.0, 2.6e-3, 0.0]])
+    rng = np.random.default_rng(2)
+    rows = np.repeat(sc.ant, 3, axis=0)
+    meas = (rng.normal(size=(len(rows), cfg.n_freq)) +
+            1j * rng.normal(size=(len(rows), cfg.n_freq))).astype(np.complex64)
+    return sc, rows, meas, rxo
+
+
+@pytest.mark.parametrize("K", [8, 16, 32, 64])
+@pytest.mark.parametrize("is_ti", [False, True])
+def test_ragged_bistatic_ti_vs_fp64(scene, oracle, K, is_ti):
+    """F = 77 is no multiple of any K: the tail block runs every time."""
+    sc, rows, meas, rxo = scene
+    img, mom = ops.backproject(T(meas, np.complex64), T(rows, np.float32),
+                               T(sc.x_locs, np.float32), T(sc.y_locs, np.float32),
+                               T(sc.z_locs, np.float32), T(rxo, np.float32), f0=sc.f0,
+                               df=sc.df, variant="rec", K=K, moment=True, is_ti=is_ti)
+    want, wmom = oracle.bp_f64(sc.x_locs, sc.y_locs, sc.z_locs, rows, meas, rxo, sc.f0, sc.df,
+                               is_ti=is_ti, moment=True)
+    assert rel(img.cpu().numpy(), want) <= 1e-5
+    assert rel(mom.cpu().numpy().reshape(-1, 3), wmom) <= 1e-5
+
+
+def test_single_and_repeated_voxels(scene, oracle):
+    sc, rows, meas, rxo = scene
+    args = (T(meas, np.complex64), T(rows, np.float32), T(sc.x_locs, np.float32),
+            T(sc.y_locs, np.float32), T(sc.z_locs, np.float32), T(rxo, np.float32))
+    for idx in ([417], [3, 3, 3, 959, 3]):
+        vi = np.array(idx, np.int32)
+        got = ops.backproject(*args, f0=sc.f0, df=sc.df, vox_idx=T(vi, np.int32)).cpu().numpy()
+        want = oracle.bp_f64(sc.x_locs, sc.y_locs, sc.z_locs, rows, meas, rxo, sc.f0, sc.df,
+                             vidx=vi)
+        assert rel(got, want) <= 1e-5
+        if len(idx) > 1:  # repeated voxels give identical values
+            assert got[0] == got[1] == got[2] == got[4]
+    empty = ops.backproject(*args, f0=sc.f0, df=sc.df, vox_idx=T(np.zeros(0, np.int32), np.int32))
+    assert empty.numel() == 0
+
+
+@pytest.mark.parametrize("rs", [1, 2, 7, 32])
+def test_every_row_split_agrees(scene, oracle, rs):
+    """Forced row-segment counts (the partial sums are added in FP64 in a
+    fixed order) against the FP64 oracle."""
+    sc, rows, meas, rxo = scene
+    os.environ["MMN_REC_RS"] = str(rs)
+    try:
+        img = ops.backproject(T(meas, np.complex64), T(rows, np.float32),
+                              T(sc.x_locs, np.float32), T(sc.y_locs, np.float32),
+                              T(sc.z_locs, np.float32), T(rxo, np.float32), f0=sc.f0, df=sc.df)
+    finally:
+        del os.environ["MMN_REC_RS"]
+    want = oracle.bp_f64(sc.x_locs, sc.y_locs, sc.z_locs, rows, meas, rxo, sc.f0, sc.df)
+    assert rel(img.cpu().numpy(), want) <= 1e-5

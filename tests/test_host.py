@@ -49,3 +49,51 @@ def test_range_window_bins():
     assert isosurface.MIN_DIST_IDX == np.count_nonzero(d < 0.35)
     assert isosurface.MAX_DIST_IDX == np.count_nonzero(d < 0.55)
     assert 0 < isosurface.MIN_DIST_IDX < isosurface.MAX_DIST_IDX < 512
+
+
+def test_normal_data_round_trip(tmp_path):
+    from mmnorm_amd import formats
+    rng = np.random.default_rng(0)
+    img = (rng.normal(size=(4, 3, 2)) + 1j * rng.normal(size=(4, 3, 2))).astype(np.complex64)
+    axes = [np.arange(n) * 1e-3 for n in (4, 3, 2)]
+    ant = rng.normal(size=(10, 3))
+    ch = rng.normal(size=(10, 8, 4)) + 0j
+    nrm = rng.normal(size=(4, 3, 2, 3))
+    p = tmp_path / "normals" / "normal_data.npz"
+    formats.save_normal_data(str(p), img, *axes, ant, ch, nrm)
+    got = formats.load_normal_data(str(p))
+    assert np.array_equal(got[0], img) and all(np.array_equal(a, b) for a, b in zip(got[1], axes))
+    assert np.array_equal(got[2], ant) and np.array_equal(got[3], ch)
+    assert np.array_equal(got[4], nrm)
+
+
+def test_sdf_record_round_trip(tmp_path):
+    from mmnorm_amd import formats
+    rng = np.random.default_rng(1)
+    xs, ys, zs = (np.arange(n) * 1e-3 for n in (4, 3, 2))
+    ch = rng.normal(size=(5, 16, 4)) + 1j * rng.normal(size=(5, 16, 4))
+    rec = formats.sdf_record(np.zeros((4, 3, 2), np.complex64), xs, ys, zs, rng.normal(size=(20, 3)),
+                             ch, np.zeros((4, 3, 2, 3)), np.zeros((4, 3, 2)),
+                             np.ones((4, 3, 2), np.uint32), [2], np.zeros((4, 3, 2, 3)),
+                             77e9, 81e9, 16)
+    assert tuple(rec) == formats.SDF_KEYS
+    # (N, samples, k) -> (N * k, samples): row 4k + j is antenna k's lane j
+    assert np.array_equal(rec["channels"][4 * 2 + 3], ch[2, :, 3])
+    assert rec["wavelengths"].shape == (16,)
+    assert rec["wavelengths"][0] == pytest.approx(2.99792458e8 / 77e9)
+    p = tmp_path / "sdf" / "sdf.npz"
+    formats.save_sdf_file(str(p), rec)
+    back = formats.load_sdf_file(str(p))
+    assert set(back) == set(formats.SDF_KEYS)
+    assert np.array_equal(back["channels"], rec["channels"])
+    assert formats.load_sdf_file(str(tmp_path / "missing.npz")) is None
+
+
+def test_read_adc_bin():
+    from mmnorm_amd import formats
+    vals = np.arange(16, dtype="<i2") - 8        # two samples x (4 I + 4 Q)
+    got = formats.read_adc_bin(vals.tobytes())
+    assert got.shape == (2, 4)
+    assert np.array_equal(got[1], vals[8:12] + 1j * vals[12:16])
+    with pytest.raises(ValueError):
+        formats.read_adc_bin(vals[:7].tobytes())
