@@ -188,6 +188,8 @@ def main(argv=None):
                                  "rec_over_plain_fast": f_s / k_avg,
                                  "kind": "hardware sin/cos of the FP32 phase fraction "
                                          "(revolutions)"}
+        rec["interpolated"] = interp_stage(sc, ant, xl, yl, zl, dev)
+        log(f"interpolated branch: {rec['interpolated']}")
         # backward of L = sum w |I|^2 (I = img / A)
         w = torch.rand(V, device=dev, generator=torch.Generator(dev).manual_seed(1))
         g = (2.0 * w * img.reshape(-1) / A / A).to(torch.complex64).contiguous()
@@ -213,6 +215,41 @@ def main(argv=None):
         rec["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(rec), flush=True)
     return 0
+
+
+def interp_stage(sc, ant, xl, yl, zl, dev):
+    """The reference's default interpolated branch on the same grid and rows
+    (4096 spectrum bins per row, seeded random spectra): the reference-arithmetic
+    kernel (FP64 cos/sin per voxel and row) against the MI355X kernel, both with
+    the image only, and the MI355X kernel with the vote moment fused."""
+    import torch
+    from mmnorm_amd import ops
+    S = 4096
+    g = torch.Generator(dev).manual_seed(5)
+    spec = torch.randn(ant.shape[0], S, dtype=torch.complex64, device=dev, generator=g)
+    lam = float(np.float32(3e8 / 77e9))
+    fs = float(np.float32(3e8 / (2 * 4e9) * 512 / S))
+
+    def t_of(f, n=3):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+    ref_ms = t_of(lambda: ops.cuda_image(spec, ant, xl, yl, zl, None, slope=0.0, wavelength=lam,
+                                         fft_spacing=fs, interpolated=True))
+    fast_ms = t_of(lambda: ops.backproject_interp(spec, ant, xl, yl, zl, wavelength=lam,
+                                                  fft_spacing=fs))
+    mom_ms = t_of(lambda: ops.backproject_interp(spec, ant, xl, yl, zl, wavelength=lam,
+                                                 fft_spacing=fs, moment=True))
+    a = ops.cuda_image(spec, ant, xl, yl, zl, None, slope=0.0, wavelength=lam, fft_spacing=fs,
+                       interpolated=True)
+    b = ops.backproject_interp(spec, ant, xl, yl, zl, wavelength=lam, fft_spacing=fs)
+    diff = float((torch.linalg.norm(a - b) / torch.linalg.norm(a)).item())
+    return {"ref_ms": ref_ms, "fast_ms": fast_ms, "fast_with_moment_ms": mom_ms,
+            "speedup": ref_ms / fast_ms, "rel_l2_fast_vs_ref": diff, "bins": S}
 
 
 def bfs_stage(img, mom, sc, dev):

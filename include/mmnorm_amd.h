@@ -121,6 +121,25 @@ int mmn_bp_forward(float *img, float *moment, const float *x_locs,
                    void *stream);
 
 /*
+ * mmn_bp_interp -- the interpolated (range-bin) branch of imaging_gpu.cu:90-105
+ * in MI355X form, plus the vote moment.  Per (voxel, row): the reference's
+ * FP32 distance and bin floor(d / fft_spacing / 2) with the same operations
+ * (so the same bins; rows with d > fft_spacing * samples are skipped), the
+ * phasor exp(-j 2 pi d / wavelength) from an FP64-reduced phase and the
+ * hardware sine/cosine (the reference: FP64 cos/sin), and, if moment !=
+ * NULL, moment[v][k] = sum_i (p_i + o_i/2 - v)_k * c_iv so that the vote of
+ * surface_normal_est.cu:110-149 is mmn_sn_from_moment(img, moment).  meas
+ * rows hold `samples` spectrum bins.  Voxel list and indexing as in
+ * mmn_bp_forward.  No reference counterpart for the fused moment.
+ */
+int mmn_bp_interp(float *img, float *moment, const float *x_locs, const float *y_locs,
+                  const float *z_locs, const int32_t *vox_idx, int64_t n_vox,
+                  const float *ant, const float *meas, const float *rx_off,
+                  float wavelength, float fft_spacing, int nx, int ny, int nz,
+                  int num_rows, int num_rx, int samples, int is_ti_radar,
+                  void *stream);
+
+/*
  * mmn_sn_from_moment -- normals[n][3] = Re(conj(I) * moment[n][3]) / |I| with
  * I = img[vox_idx[n]] (img indexed by linear voxel index).  Completes the
  * full-frequency vote (the reference's weight, surface_normal_est.cu:138-139,

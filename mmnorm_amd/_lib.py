@@ -43,6 +43,8 @@ SIGNATURES = {
                                 _c_d, _c_d, _c_int, _c_int, _c_int, _c_int,
                                 _c_int, _c_int, _c_int, _c_int, _c_int, _p, _sz,
                                 _p]),
+    "mmn_bp_interp": (_c_int, [_p, _p, _p, _p, _p, _p, _c_i64, _p, _p, _p, _c_f, _c_f,
+                               _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
     "mmn_sn_from_moment": (_c_int, [_p, _p, _p, _p, _c_i64, _p]),
     "mmn_bp_backward_workspace_size": (_sz, [_c_int, _c_int, _c_i64]),
     "mmn_bp_backward": (_c_int, [_p, _p, _p, _c_d, _p, _p, _p, _p, _c_i64, _p,

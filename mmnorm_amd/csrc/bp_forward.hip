@@ -216,6 +216,105 @@ __global__ __launch_bounds__(kBlock) void k_bp_plain(
 }
 
 // --------------------------------------------------------------------------
+// Interpolated branch, MI355X form (k_bp_interp): per (voxel, row) the
+// reference's FP32 round-trip distance and range bin, computed with the same
+// operations so that every bin is the reference's (imaging_gpu.cu:77-97);
+// the phasor exp(-j 2 pi d / lambda) from the FP64-reduced phase and the
+// hardware sine/cosine instead of FP64 cos/sin; and the normal-vote moment
+// M = sum_i (p_i + o_i/2 - v) c_i in the same pass, so that the vote of
+// surface_normal_est.cu:110-149 is Re(conj(I) M) / |I| without a second pass.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float interp_dist(float vx, float vy, float vz, float px, float py,
+                                             float pz, float ox, float oy, float oz, bool mono,
+                                             int is_ti) {
+#pragma clang fp contract(off)
+  const float ax = vx - px, ay = vy - py, az = vz - pz;
+  const float fwd = sqrtf(ax * ax + ay * ay + az * az);
+  float back = fwd;  // o = 0: the second leg is the first, bit for bit
+  if (!mono) {
+    const float bx = ax - ox, by = ay - oy, bz = az - oz;
+    back = sqrtf(bx * bx + by * by + bz * bz);
+  }
+  float d = fwd + back;
+  if (is_ti) d = (float)((double)d + 0.15);
+  return d;
+}
+
+__device__ __forceinline__ int interp_bin(float d, float fft_spacing) {
+#pragma clang fp contract(off)
+  return (int)floorf(d / fft_spacing / 2.0f);
+}
+
+// floor(RN(d / fs) / 2) without the correctly rounded division where it is
+// not needed: q = d * RN(1 / fs) is within 2 ulp of RN(d / fs), so when the
+// interval q * (1 -+ 4e-7) (3.3 ulp each side) holds no even integer the
+// floor is decided; otherwise the exact division runs.
+__device__ __forceinline__ int interp_bin_fast(float d, float fft_spacing, float inv_fs) {
+#pragma clang fp contract(off)
+  const float q = d * inv_fs;
+  const float a = floorf(q * (1.0f - 4e-7f) * 0.5f);
+  const float b = floorf(q * (1.0f + 4e-7f) * 0.5f);
+  if (a == b) return (int)a;
+  return interp_bin(d, fft_spacing);
+}
+
+__device__ __forceinline__ float half_off(float p, float o) {
+#pragma clang fp contract(off)
+  return p + o / 2.0f;
+}
+
+template <int VPT>
+__global__ __launch_bounds__(kBlock) void k_bp_interp(
+    float2 *__restrict__ img, float2 *__restrict__ mom,
+    const float *__restrict__ xl, const float *__restrict__ yl,
+    const float *__restrict__ zl, const int32_t *__restrict__ vidx,
+    int64_t nvox, const float *__restrict__ ant,
+    const float2 *__restrict__ meas, const float *__restrict__ rxo, double inv_lambda,
+    float fft_spacing, float inv_fs, int ny, int nz, int rows, int R, int S, int is_ti) {
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
+  float vx[VPT], vy[VPT], vz[VPT];
+  load_voxels<VPT>(xl, yl, zl, vidx, nvox, ny, nz, base, vx, vy, vz);
+  float Ir[VPT], Ii[VPT], M[VPT][6];
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    Ir[k] = Ii[k] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) M[k][q] = 0.0f;
+  }
+  const float dmax = fft_spacing * (float)S;
+  for (int i = 0; i < rows; ++i) {
+    const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+    const int rx = i % R;
+    const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+    const float2 *__restrict__ row = meas + (size_t)i * S;
+    const float vax = half_off(px, ox), vay = half_off(py, oy), vaz = half_off(pz, oz);
+    const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const float d = interp_dist(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono, is_ti);
+      const bool in = !(d < 0.0f || d > dmax);
+      // out-of-range rows read bin 0 and contribute nothing
+      const float2 mr = row[in ? interp_bin_fast(d, fft_spacing, inv_fs) : 0];
+      const float2 m = in ? mr : make_float2(0.0f, 0.0f);
+      float pr, pi;
+      phasor_hw((double)d * inv_lambda, pr, pi);
+      const float cr = m.x * pr - m.y * pi;
+      const float ci = m.x * pi + m.y * pr;
+      Ir[k] += cr;
+      Ii[k] += ci;
+      const float vecx = vax - vx[k], vecy = vay - vy[k], vecz = vaz - vz[k];
+      M[k][0] = fmaf(vecx, cr, M[k][0]);
+      M[k][1] = fmaf(vecx, ci, M[k][1]);
+      M[k][2] = fmaf(vecy, cr, M[k][2]);
+      M[k][3] = fmaf(vecy, ci, M[k][3]);
+      M[k][4] = fmaf(vecz, cr, M[k][4]);
+      M[k][5] = fmaf(vecz, ci, M[k][5]);
+    }
+  }
+  store_voxels<VPT>(img, mom, nvox, base, Ir, Ii, M);
+}
+
+// --------------------------------------------------------------------------
 // Recurrence.  For row i and voxel v the samples' phasors are
 //   exp(-j 2pi f_j d/c) = A_b * r^(j - bK),  A_b = exp(-j 2pi f_bK d/c),
 //   r = exp(-j 2pi df d/c),
@@ -522,6 +621,31 @@ int dispatch_rec(int K, hipStream_t st, const RecArgs &a) {
 }
 
 }  // namespace
+
+extern "C" int mmn_bp_interp(float *img, float *moment, const float *x_locs,
+                             const float *y_locs, const float *z_locs,
+                             const int32_t *vox_idx, int64_t n_vox, const float *ant,
+                             const float *meas, const float *rx_off, float wavelength,
+                             float fft_spacing, int nx, int ny, int nz, int num_rows,
+                             int num_rx, int samples, int is_ti_radar, void *stream) {
+  MMN_REQUIRE(img && x_locs && y_locs && z_locs && ant && meas && rx_off, MMN_ERR_NULL,
+              "mmn_bp_interp: NULL pointer argument");
+  MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 && samples > 0 &&
+                  wavelength > 0.0f && fft_spacing > 0.0f,
+              MMN_ERR_ARG, "mmn_bp_interp: sizes and scalars must be positive");
+  int64_t nvox = (int64_t)nx * ny * nz;
+  if (vox_idx) {
+    MMN_REQUIRE(n_vox >= 0, MMN_ERR_ARG, "mmn_bp_interp: n_vox < 0");
+    nvox = n_vox;
+  }
+  if (nvox == 0) return 0;
+  hipLaunchKernelGGL(k_bp_interp<2>, dim3(mmn_blocks(nvox, kBlock * 2)), dim3(kBlock), 0,
+                     (hipStream_t)stream, (float2 *)img, (float2 *)moment, x_locs, y_locs,
+                     z_locs, vox_idx, nvox, ant, (const float2 *)meas, rx_off,
+                     1.0 / (double)wavelength, fft_spacing, 1.0f / fft_spacing, ny, nz,
+                     num_rows, num_rx, samples, is_ti_radar);
+  return mmn_check_launch("mmn_bp_interp");
+}
 
 extern "C" int mmn_cuda_image(float *img, const float *x_locs,
                               const float *y_locs, const float *z_locs,

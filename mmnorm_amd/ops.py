@@ -119,6 +119,28 @@ def cuda_image(meas, ant, x_locs, y_locs, z_locs, rx_off, *, slope, wavelength,
     return img.view(x_locs.numel(), y_locs.numel(), z_locs.numel())
 
 
+def backproject_interp(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, wavelength,
+                       fft_spacing, vox_idx=None, is_ti=False, moment=False):
+    """Interpolated (range-bin) backprojection, MI355X form: the reference's
+    FP32 distance and bins, FP64-reduced phase with the hardware sine/cosine,
+    and optionally the vote moment.  meas complex64 (rows, S) spectrum bins.
+    Returns img (NX,NY,NZ) (or (n,) for vox_idx) and, with moment=True, also
+    the moment complex64 (n, 3)."""
+    rx_off, nvox = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
+    n = vox_idx.numel() if vox_idx is not None else nvox
+    dev = meas.device
+    img = torch.empty(n, dtype=torch.complex64, device=dev)
+    mom = torch.empty((n, 3), dtype=torch.complex64, device=dev) if moment else None
+    _lib.call("mmn_bp_interp", img.data_ptr(), _lib.ptr(mom), x_locs.data_ptr(),
+              y_locs.data_ptr(), z_locs.data_ptr(), _lib.ptr(vox_idx), n, ant.data_ptr(),
+              meas.data_ptr(), rx_off.data_ptr(), float(wavelength), float(fft_spacing),
+              x_locs.numel(), y_locs.numel(), z_locs.numel(), meas.shape[0], rx_off.shape[0],
+              meas.shape[1], int(bool(is_ti)), _lib.stream_handle(dev))
+    if vox_idx is None:
+        img = img.view(x_locs.numel(), y_locs.numel(), z_locs.numel())
+    return (img, mom) if moment else img
+
+
 def sn_est(img, vox_idx, meas, ant, x_locs, y_locs, z_locs, rx_off, *, wavelength,
            fft_spacing, is_ti=False):
     """The reference vote (surface_normal_est.cu:64-149) over vox_idx, float32 (n, 3)."""

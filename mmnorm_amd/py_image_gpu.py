@@ -9,7 +9,9 @@ libmmnorm_hip.so on the current ROCm device:
   re-anchoring every ``K`` samples); ``variant="ref"`` runs the reference
   kernel's own arithmetic (FP64 phase and cos/sin per sample) and
   ``variant="plain"`` the direct FP32 sincosf kernel;
-* interpolated range-bin branch: the reference kernel's arithmetic.
+* interpolated range-bin branch: by default ``mmn_bp_interp`` (the
+  reference's FP32 distances and bins, FP64-reduced phase with the hardware
+  sine/cosine); ``variant="ref"`` runs the reference kernel's arithmetic.
 
 Host steps mirror the reference: TX rows repeated per RX and measurements
 reordered to (meas, rx, sample) when ``use_4_rx`` (py_image_gpu.py:228-234);
@@ -67,10 +69,14 @@ def _run_image(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offset, slop
     meas_t = _to_dev(meas, np.complex64, dev)
     if meas_t.shape[0] != ant_t.shape[0]:
         raise ValueError(f"{meas_t.shape[0]} measurement rows for {ant_t.shape[0]} antenna rows")
-    if interpolated or variant == "ref":
+    if variant == "ref":
         raw = ops.cuda_image(meas_t, ant_t, xl, yl, zl, rx_t, slope=np.float32(slope),
                              wavelength=np.float32(wavelength), fft_spacing=fft_spacing,
                              is_ti=is_ti_radar, interpolated=interpolated)
+    elif interpolated:
+        raw = ops.backproject_interp(meas_t, ant_t, xl, yl, zl, rx_t,
+                                     wavelength=np.float32(wavelength),
+                                     fft_spacing=fft_spacing, is_ti=is_ti_radar)
     else:
         f0 = ops.C0 / float(np.float32(wavelength))
         df = float(np.float32(slope))

@@ -62,14 +62,17 @@ def test_image_cuda_full_rec_variant_close_to_fp64(oracle):
     assert np.linalg.norm(img - want) / np.linalg.norm(want) < 1e-5
 
 
-def test_image_cuda_interpolated_groups(oracle, monkeypatch):
-    """Antenna groups normalised by their own TX count, then summed (py_image_gpu.py:339-362)."""
+@pytest.mark.parametrize("variant", ["rec", "ref"])
+def test_image_cuda_interpolated_groups(oracle, monkeypatch, variant):
+    """Antenna groups normalised by their own TX count, then summed
+    (py_image_gpu.py:339-362); the default (MI355X) and the reference-arithmetic
+    kernels against the restatement of imaging_gpu.cu."""
     monkeypatch.setattr(py_image_gpu, "MAX_GROUP_ROWS", 32)  # 8 TX per group with 4 RX
     xl, yl, zl, locs, data, rxo, lam = real_like_inputs(seed=9, n_meas=19, spectra=True)
     fs = np.float32(3e8 / (2 * 3.0009e9) * 512 / 4096)
     img, centers = py_image_gpu.image_cuda(xl, yl, zl, locs, data, rxo, 5.86116e6, lam,
                                            3.0009e9, 512, use_4_rx=True, is_ti_radar=True,
-                                           use_interpolated_processing=True)
+                                           use_interpolated_processing=True, variant=variant)
     want = None
     for lo in range(0, 19, 8):
         ant, meas = _layout(locs[lo:lo + 8], data[lo:lo + 8], 4)
@@ -82,14 +85,16 @@ def test_image_cuda_interpolated_groups(oracle, monkeypatch):
     assert centers.shape == (19 * 4, 3)
 
 
-def test_normal_estimates_interpolated(oracle):
+@pytest.mark.parametrize("variant", ["fast", "ref"])
+def test_normal_estimates_interpolated(oracle, variant):
     xl, yl, zl, locs, data, rxo, lam = real_like_inputs(seed=10, spectra=True)
     img = py_image_gpu.image_cuda(xl, yl, zl, locs, data, rxo, 5.86116e6, lam, 3.0009e9, 512,
                                   use_4_rx=True, is_ti_radar=True,
                                   use_interpolated_processing=True, return_phase_centers=False)
     nrm = py_sn_est_gpu.normal_estimates_cuda(img, xl, yl, zl, locs, data, rxo, 5.86116e6,
                                               lam, 3.0009e9, 512, use_4_rx=True,
-                                              is_ti_radar=True, initial_filter_percent=0.3)
+                                              is_ti_radar=True, initial_filter_percent=0.3,
+                                              variant=variant)
     filt = np.abs(img) > np.max(np.abs(img)) * 0.3
     assert nrm.dtype == np.float64 and nrm.shape == img.shape + (3,)
     assert np.all(np.isnan(nrm[~filt])) and not np.any(np.isnan(nrm[filt]))
