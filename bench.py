@@ -188,6 +188,8 @@ def main(argv=None):
                                  "rec_over_plain_fast": f_s / k_avg,
                                  "kind": "hardware sin/cos of the FP32 phase fraction "
                                          "(revolutions)"}
+        rec["config4_backward"] = config4_stage(dev)
+        log(f"config 4 forward/backward: {rec['config4_backward']}")
         rec["interpolated"] = interp_stage(sc, ant, xl, yl, zl, dev)
         log(f"interpolated branch: {rec['interpolated']}")
         # backward of L = sum w |I|^2 (I = img / A)
@@ -215,6 +217,35 @@ def main(argv=None):
         rec["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(rec), flush=True)
     return 0
+
+
+def config4_stage(dev):
+    """The differentiable configuration at full size (40x40 aperture, 128
+    samples, 96^3 voxels): forward (image only) and backward (dL/dmeas and
+    dL/dant of L = sum w |I|^2) times."""
+    import torch
+    from mmnorm_amd import ops, synth
+    sc = synth.make_scene(4, device=dev)
+    T = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x, dtype=dt)).to(dev)
+    meas, ant = T(sc.meas, np.complex64), T(sc.ant, np.float32)
+    xl, yl, zl = (T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs))
+    A = ant.shape[0]
+
+    def t_of(f, n=3):
+        out = f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            out = f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3, out
+    f_ms, img = t_of(lambda: ops.backproject(meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, K=64))
+    w = torch.rand(img.numel(), device=dev, generator=torch.Generator(dev).manual_seed(2))
+    g = (2.0 * w * img.reshape(-1) / A / A).to(torch.complex64).contiguous()
+    b_ms, _ = t_of(lambda: ops.backproject_backward(g, meas, ant, xl, yl, zl, f0=sc.f0,
+                                                    df=sc.df, scale=1.0 / A, K=32))
+    return {"forward_ms": f_ms, "backward_ms": b_ms, "over_forward": b_ms / f_ms,
+            "units": int(sc.cfg.work)}
 
 
 def interp_stage(sc, ant, xl, yl, zl, dev):

@@ -108,3 +108,24 @@ def test_config2_full_size_sampled_parity(oracle):
                            f0=sc.f0, df=sc.df, variant="plain_fast",
                            vox_idx=T(vidx, np.int32)).cpu().numpy()
     assert np.linalg.norm(fast - ref) / np.linalg.norm(ref) <= 2e-3
+
+
+def test_config5_full_size_sampled_parity(oracle):
+    """Config 5 at full size (100x100 aperture, 512 samples, 256x256x128 voxels):
+    the recurrence on 96 voxels (the 48 strongest and 48 seeded) vs the FP64
+    oracle."""
+    dev = torch.device("cuda", 0)
+    sc = synth.make_scene(5, device=dev)
+    rng = np.random.default_rng(23)
+    args = (T(sc.meas, np.complex64), T(sc.ant, np.float32),
+            *(T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs)))
+    full = ops.backproject(*args, f0=sc.f0, df=sc.df).reshape(-1)
+    strong = torch.topk(full.abs(), 48).indices.cpu().numpy()
+    vidx = np.unique(np.concatenate([strong, rng.choice(sc.cfg.nvox, 48, replace=False)]))
+    ref = oracle.bp_f64(sc.x_locs, sc.y_locs, sc.z_locs, sc.ant, sc.meas, np.zeros((1, 3)),
+                        sc.f0, sc.df, vidx=vidx)
+    got = full[torch.from_numpy(vidx).to(dev)].cpu().numpy().astype(np.complex128)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-4
+    # the strongest voxels one by one
+    s = np.isin(vidx, strong)
+    assert np.max(np.abs(got[s] - ref[s]) / np.abs(ref[s])) <= 1e-4
