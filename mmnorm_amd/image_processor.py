@@ -136,3 +136,52 @@ def generate_mmwave_surface_normals(poses, radar_data, aperture_type, min_f, num
     centers = (ant_rows + (ant_rows + rx_rep)) / 2
     return (img.cpu().numpy(), (xl, yl, zl), normals.reshape(*img.shape, 3).cpu().numpy(),
             centers, meas_np)
+
+
+def correlate_locs_and_meas(robot_data, radar_data, loc_times, locs, num_frames, sweep_time,
+                            periodicity, samples_per_chirp, num_chirps, min_f,
+                            is_optitrack=False, num_lanes=4):
+    """Pose of every radar frame by linear interpolation of the robot
+    trajectory (image_processor.py:74-154), 77 GHz layout.
+
+    robot_data: {'all_radar_filenames', 'all_radar_created_ts'}; radar_data:
+    {int(filename): (samples, num_lanes) complex} as read_adc_bin gives;
+    loc_times (T,), locs (T, D) the trajectory.  Frame j of a file starts at
+    created_ts + j (sweep_time + periodicity); frames outside the trajectory's
+    time span, frames within 3/16 wavelength of an earlier frame of the same
+    file, and frames with too few samples are dropped -- and, as in the
+    reference, once a file's first frame is dropped the duplicate test fails
+    for its later frames, which are dropped too.  Returns {'radar_data': list
+    of (samples_per_chirp, num_lanes) arrays, 'poses': (n, D)}.
+    """
+    loc_times = np.asarray(loc_times, dtype=np.float64)
+    locs = np.asarray(locs, dtype=np.float64)
+    thr = C0 / min_f * (3 / 16)
+    t_lo, t_hi = loc_times.min(), loc_times.max()
+    all_locs, all_data = [], []
+    for i, fname in enumerate(robot_data["all_radar_filenames"]):
+        ts = robot_data["all_radar_created_ts"][i]
+        if ts == -1 or int(fname) not in radar_data:
+            continue
+        data = radar_data[int(fname)]
+        row = []
+        for j in range(num_frames):
+            start = ts + j * (sweep_time + periodicity)
+            if not (t_lo <= start <= t_hi):
+                continue  # interp1d raises outside the span
+            loc = np.array([np.interp(start, loc_times, locs[:, d]) for d in range(locs.shape[1])])
+            if j != 0:
+                if not row:
+                    continue  # the reference's min over an empty row raises
+                if np.min(np.linalg.norm(np.array(row)[:, :3] - loc[:3], axis=1)) < thr:
+                    continue
+            lo = j * samples_per_chirp * num_chirps
+            new = data[lo:lo + samples_per_chirp, :]
+            if new.shape != (samples_per_chirp, num_lanes):
+                continue
+            if is_optitrack:
+                loc = np.array([-loc[0], loc[2], loc[1]])  # image_processor.py:64-72
+            all_locs.append(loc)
+            row.append(loc)
+            all_data.append(new)
+    return {"radar_data": all_data, "poses": np.array(all_locs)}

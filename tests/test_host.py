@@ -97,3 +97,33 @@ def test_read_adc_bin():
     assert np.array_equal(got[1], vals[8:12] + 1j * vals[12:16])
     with pytest.raises(ValueError):
         formats.read_adc_bin(vals[:7].tobytes())
+
+
+def test_correlate_locs_and_meas():
+    from mmnorm_amd import image_processor as ip
+    lam = 2.99792458e8 / 77e9
+    # the robot moves along x at 1 mm per time unit from t = 100 to t = 200,
+    # then stands still until t = 300
+    t = np.array([100.0, 200.0, 300.0])
+    locs = np.array([[0.0, 0.0, 0.0, 7.0], [0.1, 0.0, 0.0, 7.0], [0.1, 0.0, 0.0, 7.0]])
+    spc, chirps, frames, sweep, per = 4, 2, 3, 1.0, 1.0
+    rd = {1: np.arange(frames * spc * chirps * 4).reshape(-1, 4) + 0j,
+          2: np.zeros((frames * spc * chirps, 4)) + 0j,
+          3: np.zeros((spc * chirps, 4)) + 0j,     # one frame's worth only
+          4: np.zeros((frames * spc * chirps, 4)) + 0j}
+    robot = {"all_radar_filenames": ["1", "2", "3", "4", "5"],
+             "all_radar_created_ts": [120.0, 250.0, 150.0, 199.0, 160.0]}
+    out = ip.correlate_locs_and_meas(robot, rd, t, locs, frames, sweep, per, spc, chirps, 77e9)
+    poses = out["poses"]
+    # file 1: three frames 2 mm apart (> 3/16 lambda = 0.73 mm), poses interpolated
+    assert np.allclose(poses[:3, 0], [0.020, 0.022, 0.024]) and np.all(poses[:3, 3] == 7.0)
+    assert np.array_equal(out["radar_data"][1], rd[1][8:12])
+    # file 2: the robot stands still -> later frames are duplicates of the first
+    assert np.allclose(poses[3, 0], 0.1)
+    # file 3: frame 0 kept, later frames lack samples; file 4: frames at
+    # t = 199 and 201 are 1 mm apart (kept), the one at 203 repeats 201
+    # (dropped); file 5 has no radar file
+    assert len(poses) == 3 + 1 + 1 + 2
+    assert np.allclose(poses[4, 0], 0.05)
+    assert np.allclose(poses[5:, 0], [0.099, 0.1])
+    assert 3 / 16 * lam > 2e-4
