@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OUT = os.path.join(PKG, "libmmnorm_hip.so")
 ARCH = "gfx950"
 
-SOURCES = ["capi.hip", "bp_forward.hip", "sn_vote.hip", "bp_backward.hip",
+SOURCES = ["capi.hip", "bp_forward.hip", "bp_profile.hip", "sn_vote.hip", "bp_backward.hip",
            "rsdf_bfs.hip", "segment.hip", "raymarch.hip"]
 
 HIPCC_FLAGS = [

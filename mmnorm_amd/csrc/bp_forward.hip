@@ -102,26 +102,6 @@ __device__ __forceinline__ void load_voxels(
   }
 }
 
-// Adds one row's contribution c to the image (Kahan) and the vote moment.
-__device__ __forceinline__ void add_row(float cr, float ci, float vecx,
-                                        float vecy, float vecz, float &Ir,
-                                        float &Ii, float &Cr, float &Ci,
-                                        float (&M)[6]) {
-  // Kahan: C carries the low-order part lost by the running sum.
-  const float yr = cr - Cr, yi = ci - Ci;
-  const float tr = Ir + yr, ti = Ii + yi;
-  Cr = (tr - Ir) - yr;
-  Ci = (ti - Ii) - yi;
-  Ir = tr;
-  Ii = ti;
-  M[0] = fmaf(vecx, cr, M[0]);
-  M[1] = fmaf(vecx, ci, M[1]);
-  M[2] = fmaf(vecy, cr, M[2]);
-  M[3] = fmaf(vecy, ci, M[3]);
-  M[4] = fmaf(vecz, cr, M[4]);
-  M[5] = fmaf(vecz, ci, M[5]);
-}
-
 template <int VPT>
 __device__ __forceinline__ void store_voxels(
     float2 *__restrict__ img, float2 *__restrict__ mom, int64_t nvox,
@@ -588,9 +568,7 @@ void launch_rec(hipStream_t st, const RecArgs &a) {
   hipLaunchKernelGGL((k_bp_rec<K, VPT, S>), dim3((unsigned)nblk, nseg), dim3(kBlock), 0, st,
                      img, mom, a.xl, a.yl, a.zl, a.vidx, a.nvox, a.ant, a.meas, a.rxo,
                      a.f0c, a.dfc, a.ny, a.nz, a.rows, a.R, a.F, a.is_ti, rps);
-  if (nseg > 1)
-    hipLaunchKernelGGL(k_rs_reduce, dim3(mmn_blocks(a.nvox, kBlock)), dim3(kBlock), 0, st,
-                       a.img, a.mom, img, mom, a.nvox, nseg);
+  if (nseg > 1) mmn_rs_reduce(st, a.img, a.mom, img, mom, a.nvox, nseg);
 }
 
 template <int VPT, int S>
@@ -621,6 +599,12 @@ int dispatch_rec(int K, hipStream_t st, const RecArgs &a) {
 }
 
 }  // namespace
+
+void mmn_rs_reduce(hipStream_t st, float2 *img, float2 *mom, const float2 *pimg,
+                   const float2 *pmom, int64_t n, int rs) {
+  hipLaunchKernelGGL(k_rs_reduce, dim3(mmn_blocks(n, kBlock)), dim3(kBlock), 0, st, img, mom,
+                     pimg, pmom, n, rs);
+}
 
 extern "C" int mmn_bp_interp(float *img, float *moment, const float *x_locs,
                              const float *y_locs, const float *z_locs,

@@ -1,0 +1,377 @@
+// Range-profile backprojection: the full-frequency sum of mmn_bp_forward,
+// evaluated per (row, voxel) from a per-row table instead of per sample.
+//
+// For row i write f_t = f0 + t df and centre the band at t_c = (F - 1) / 2:
+//   P_i(d) = sum_t s_it exp(-j 2 pi f_t d / c) = exp(-j 2 pi f_c d / c) Q_i(d),
+//   Q_i(d) = sum_t s_it exp(-j 2 pi (t - t_c) df d / c),   f_c = f0 + t_c df.
+// Q_i is a trigonometric polynomial in d with no component faster than
+// (F - 1) df / (2c) cycles per metre: it is band-limited, smooth on the scale
+// of the range resolution c / ((F - 1) df).  Sampled at h = c / (64 (F - 1) df),
+// 64 points per resolution cell, the cubic through the four samples around d
+// reproduces Q_i to a few 1e-8 of sum_t |s_it| (the Lagrange remainder of a
+// tone at pi/64 rad per sample is at most 0.0234 (pi/64)^4 = 1.4e-7 of its
+// amplitude, at mid-interval).  So:
+//   k_prof_rows   per row, the table start: a lower bound of the row's
+//                 round-trip distance over the grid's bounding box, less h;
+//   k_prof_table  Q_i at M points h apart, by Horner in exp(-j 2 pi df d / c)
+//                 re-anchored every 64 samples by an FP64-reduced phasor (the
+//                 recurrence kernel's arithmetic);
+//   k_prof_coef   each interval's cubic as four complex coefficients (32 B);
+//   k_bp_prof     per (row, voxel): the FP64 round-trip distance, interval
+//                 and fraction, one 32-byte coefficient fetch, the cubic
+//                 (6 FMA), the FP64-reduced carrier exp(-j 2 pi f_c d / c),
+//                 then the Kahan row sum and the vote moment exactly as the
+//                 recurrence kernel adds them.
+// A round trip over the box spans at most twice its diagonal, so M =
+// ceil(2 diag / h) + 6 points cover every voxel with the cubic's neighbours.
+//
+// Full-grid launches map each wave to a 4x4x4 brick of voxels (a 256-lane
+// workgroup to 8x8x4, twice that in z with two voxels per lane): the
+// brick's distances to a row span a few centimetres, so a wave's coefficient
+// fetches fall on a few cache lines of the row's table.
+#include <math.h>
+#include <stdlib.h>
+
+#include "mmn_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kOver = 64;     // table points per range-resolution cell
+constexpr int kHornerK = 64;  // samples per FP64 re-anchor in the table
+constexpr int kMaxSeg = 8;    // most row segments of one launch
+constexpr int kMaxM = 1 << 22;
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct ProfGeom {
+  double h;
+  int M;
+};
+
+// Table spacing and length for F samples df apart over a box (host).
+bool prof_geom(int F, double df, const double *bb, ProfGeom *g) {
+  const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
+  if (!(ex >= 0.0 && ey >= 0.0 && ez >= 0.0) || !isfinite(ex + ey + ez)) return false;
+  const double diag = sqrt(ex * ex + ey * ey + ez * ez);
+  const double band = (double)(F - 1) * fabs(df);
+  // with one sample or df = 0, Q_i is constant and any spacing is exact
+  g->h = band > 0.0 ? MMN_C0 / (kOver * band) : (2.0 * diag + 1e-3) / 8.0;
+  const double m = ceil(2.0 * diag / g->h) + 6.0;
+  if (!(m <= (double)kMaxM)) return false;
+  g->M = (int)m;
+  return true;
+}
+
+int seg_cap(int rows) {
+  int cap = rows / 32;
+  if (cap > kMaxSeg) cap = kMaxSeg;
+  return cap < 1 ? 1 : cap;
+}
+
+struct ProfWs {
+  double *dlo;   // [A]
+  float2 *tab;   // [A][M]
+  float4 *coef;  // [A][M][2]
+  float2 *part;  // [segments][n] images, then [segments][n][3] moments
+};
+
+size_t prof_layout(int A, int M, int64_t n, bool moment, ProfWs *w, char *base) {
+  const int segs = seg_cap(A);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *q = base ? base + off : nullptr;
+    off += align256(bytes);
+    return q;
+  };
+  char *d = take(sizeof(double) * (size_t)A);
+  char *t = take(sizeof(float2) * (size_t)A * M);
+  char *c = take(sizeof(float4) * 2 * (size_t)A * M);
+  char *p = segs > 1 ? take(sizeof(float2) * (size_t)segs * n * (moment ? 4 : 1)) : nullptr;
+  if (w) {
+    w->dlo = (double *)d;
+    w->tab = (float2 *)t;
+    w->coef = (float4 *)c;
+    w->part = (float2 *)p;
+  }
+  return off;
+}
+
+// Distance from a point to the box [lo, hi] (0 inside).
+__device__ __forceinline__ double box_dist(double px, double py, double pz, const double *bb) {
+  const double qx = fmin(fmax(px, bb[0]), bb[3]) - px;
+  const double qy = fmin(fmax(py, bb[1]), bb[4]) - py;
+  const double qz = fmin(fmax(pz, bb[2]), bb[5]) - pz;
+  return sqrt(fma(qz, qz, fma(qy, qy, qx * qx)));
+}
+
+struct Box {
+  double b[6];
+};
+
+__global__ __launch_bounds__(kBlock) void k_prof_rows(double *__restrict__ dlo,
+                                                      const float *__restrict__ ant,
+                                                      const float *__restrict__ rxo, Box box,
+                                                      int A, int R, int is_ti, double h) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= A) return;
+  const double px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+  const int rx = i % R;
+  const double qx = px + (double)rxo[3 * rx + 0], qy = py + (double)rxo[3 * rx + 1],
+               qz = pz + (double)rxo[3 * rx + 2];
+  double d = box_dist(px, py, pz, box.b) + box_dist(qx, qy, qz, box.b);
+  if (is_ti) d += 0.15;
+  dlo[i] = d - h;  // table point k sits at dlo + k h; voxels fall at k >= 1
+}
+
+// One workgroup row of lanes per table row (blockIdx.y): the row's samples
+// are wave-uniform and arrive through scalar loads.
+__global__ __launch_bounds__(kBlock) void k_prof_table(
+    float2 *__restrict__ tab, const float2 *__restrict__ meas,
+    const double *__restrict__ dlo, double h, double dfc, double tc, int M, int F) {
+  const int i = blockIdx.y;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const double d = dlo[i] + (double)min(k, M - 1) * h;
+  float rr, ri;
+  phasor_quarter(4.0 * d * dfc, rr, ri);  // exp(-j 2 pi df d / c)
+  const float2 *__restrict__ row = meas + (size_t)i * F;
+  float cr = 0.0f, ci = 0.0f;
+  for (int b0 = 0; b0 < F; b0 += kHornerK) {
+    const int len = min(kHornerK, F - b0);
+    const float2 top = row[b0 + len - 1];
+    float hr = top.x, hi = top.y;
+    for (int t = len - 2; t >= 0; --t) {
+      const float2 s = row[b0 + t];
+      const float tr = fmaf(-hi, ri, fmaf(hr, rr, s.x));
+      const float ti = fmaf(hi, rr, fmaf(hr, ri, s.y));
+      hr = tr;
+      hi = ti;
+    }
+    float ar, ai;
+    phasor_hw(d * (((double)b0 - tc) * dfc), ar, ai);
+    cr = fmaf(ar, hr, fmaf(-ai, hi, cr));
+    ci = fmaf(ar, hi, fmaf(ai, hr, ci));
+  }
+  if (k < M) tab[(size_t)i * M + k] = make_float2(cr, ci);
+}
+
+// The cubic through table points m-1 .. m+2 as a polynomial in the fraction
+// f in [0, 1) of interval [m, m+1]:
+//   P(f) = q + c1 f + c2 f^2 + c3 f^3,  c1 = -p/3 - q/2 + r - s/6,
+//   c2 = (p + r)/2 - q,  c3 = (s - p)/6 + (q - r)/2,
+// stored as (q, c1) and (c2, c3).  Intervals without four neighbours hold 0.
+__global__ __launch_bounds__(kBlock) void k_prof_coef(float4 *__restrict__ coef,
+                                                      const float2 *__restrict__ tab, int M,
+                                                      int64_t AM) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= AM) return;
+  const int m = (int)(t % M);
+  float4 lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), hi = lo;
+  if (m >= 1 && m <= M - 3) {
+    const float2 pf = tab[t - 1], qf = tab[t], rf = tab[t + 1], sf = tab[t + 2];
+    const double p[2] = {pf.x, pf.y}, q[2] = {qf.x, qf.y}, r[2] = {rf.x, rf.y},
+                 s[2] = {sf.x, sf.y};
+    double c1[2], c2[2], c3[2];
+    for (int k = 0; k < 2; ++k) {
+      c1[k] = -p[k] / 3.0 - q[k] / 2.0 + r[k] - s[k] / 6.0;
+      c2[k] = (p[k] + r[k]) / 2.0 - q[k];
+      c3[k] = (s[k] - p[k]) / 6.0 + (q[k] - r[k]) / 2.0;
+    }
+    lo = make_float4(qf.x, qf.y, (float)c1[0], (float)c1[1]);
+    hi = make_float4((float)c2[0], (float)c2[1], (float)c3[0], (float)c3[1]);
+  }
+  coef[2 * t + 0] = lo;
+  coef[2 * t + 1] = hi;
+}
+
+// Per (row, voxel).  With vidx, lane t of the launch takes vidx[t] and writes
+// output t; without, the brick mapping above and output = linear voxel index.
+// blockIdx.y is the row segment (rows [y rps, (y+1) rps)); with more than
+// one segment the partial sums go to img / mom + y n for a fixed-order
+// reduction.
+template <int VPT>
+__global__ __launch_bounds__(kBlock) void k_bp_prof(
+    float2 *__restrict__ img, float2 *__restrict__ mom, const float *__restrict__ xl,
+    const float *__restrict__ yl, const float *__restrict__ zl,
+    const int32_t *__restrict__ vidx, int64_t n, int nx, int ny, int nz,
+    const float *__restrict__ ant, const float *__restrict__ rxo,
+    const double *__restrict__ dlo, const float4 *__restrict__ coef, double inv_h,
+    double fcc, int M, int rows, int R, int is_ti, int rps) {
+  float vx[VPT], vy[VPT], vz[VPT];
+  int64_t out[VPT];
+  bool ok[VPT];
+  if (vidx) {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int64_t t = (int64_t)blockIdx.x * (kBlock * VPT) + (int64_t)k * kBlock + threadIdx.x;
+      ok[k] = t < n;
+      out[k] = t;
+      const int64_t lin = vidx[ok[k] ? t : n - 1];
+      voxel_coords(xl, yl, zl, ny, nz, lin, vx[k], vy[k], vz[k]);
+    }
+  } else {
+    const int nbz = (nz + 4 * VPT - 1) / (4 * VPT), nby = (ny + 7) / 8;
+    const int b = blockIdx.x;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int x = bx * 8 + (w & 1) * 4 + (lane >> 4);
+    const int y = by * 8 + (w >> 1) * 4 + ((lane >> 2) & 3);
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int z = bz * 4 * VPT + k * 4 + (lane & 3);
+      ok[k] = x < nx && y < ny && z < nz;
+      out[k] = ((int64_t)x * ny + y) * nz + z;
+      vx[k] = xl[min(x, nx - 1)];
+      vy[k] = yl[min(y, ny - 1)];
+      vz[k] = zl[min(z, nz - 1)];
+    }
+  }
+  float Ir[VPT], Ii[VPT], Cr[VPT], Ci[VPT], Mo[VPT][6];
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    Ir[k] = Ii[k] = Cr[k] = Ci[k] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Mo[k][q] = 0.0f;
+  }
+  const double umax = (double)(M - 3);
+  const int row_lo = blockIdx.y * rps;
+  const int row_hi = min(rows, row_lo + rps);
+  for (int i = row_lo; i < row_hi; ++i) {
+    const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+    const int rx = i % R;
+    const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+    const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
+    const double dl = dlo[i];
+    const float4 *__restrict__ crow = coef + (size_t)i * M * 2;
+    const float vax = px + ox / 2.0f, vay = py + oy / 2.0f, vaz = pz + oz / 2.0f;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      double d = roundtrip_dist64(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono);
+      if (is_ti) d += 0.15;
+      // interval and fraction; the clamp keeps any input (NaN included)
+      // inside the table
+      const double u = fmin(fmax((d - dl) * inv_h, 1.0), umax);
+      const int m = (int)u;
+      const float f = (float)(u - (double)m);
+      const float4 a = crow[2 * m], c = crow[2 * m + 1];
+      const float qr = fmaf(fmaf(fmaf(c.z, f, c.x), f, a.z), f, a.x);
+      const float qi = fmaf(fmaf(fmaf(c.w, f, c.y), f, a.w), f, a.y);
+      float ar, ai;
+      phasor_hw(d * fcc, ar, ai);
+      const float cr = fmaf(qr, ar, -(qi * ai));
+      const float ci = fmaf(qr, ai, qi * ar);
+      add_row(cr, ci, vax - vx[k], vay - vy[k], vaz - vz[k], Ir[k], Ii[k], Cr[k], Ci[k],
+              Mo[k]);
+    }
+  }
+  const size_t seg = blockIdx.y;
+  float2 *__restrict__ im = img + seg * n;
+  float2 *__restrict__ mo = mom ? mom + seg * n * 3 : nullptr;
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    if (!ok[k]) continue;
+    const int64_t t = out[k];
+    im[t] = make_float2(Ir[k], Ii[k]);
+    if (mo) {
+      mo[3 * t + 0] = make_float2(Mo[k][0], Mo[k][1]);
+      mo[3 * t + 1] = make_float2(Mo[k][2], Mo[k][3]);
+      mo[3 * t + 2] = make_float2(Mo[k][4], Mo[k][5]);
+    }
+  }
+}
+
+int env_int(const char *name, int dflt) {
+  const char *s = getenv(name);
+  return (s && *s) ? atoi(s) : dflt;
+}
+
+}  // namespace
+
+extern "C" size_t mmn_bp_profile_workspace_size(int64_t n_vox, int num_rows, int samples,
+                                                double df, const double *bbox,
+                                                int with_moment) {
+  ProfGeom g;
+  if (n_vox <= 0 || num_rows <= 0 || samples <= 0 || !bbox || !prof_geom(samples, df, bbox, &g))
+    return 0;
+  return prof_layout(num_rows, g.M, n_vox, with_moment != 0, nullptr, nullptr);
+}
+
+extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
+                              const float *y_locs, const float *z_locs,
+                              const int32_t *vox_idx, int64_t n_vox, const float *ant,
+                              const float *meas, const float *rx_off, double f0, double df,
+                              int nx, int ny, int nz, int num_rows, int num_rx, int samples,
+                              int is_ti_radar, const double *bbox, void *workspace,
+                              size_t workspace_bytes, void *stream) {
+  MMN_REQUIRE(img && x_locs && y_locs && z_locs && ant && meas && rx_off && bbox,
+              MMN_ERR_NULL, "mmn_bp_profile: NULL pointer argument");
+  MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 && samples > 0,
+              MMN_ERR_ARG, "mmn_bp_profile: sizes must be positive");
+  ProfGeom g;
+  MMN_REQUIRE(prof_geom(samples, df, bbox, &g), MMN_ERR_ARG,
+              "mmn_bp_profile: bounding box must be finite with min <= max, and its "
+              "round-trip span at most %d table points", kMaxM);
+  int64_t n = (int64_t)nx * ny * nz;
+  if (vox_idx) {
+    MMN_REQUIRE(n_vox >= 0, MMN_ERR_ARG, "mmn_bp_profile: n_vox < 0");
+    n = n_vox;
+  }
+  if (n == 0) return 0;
+  const int A = num_rows, F = samples, M = g.M;
+  const bool with_mom = moment != nullptr;
+  const size_t need = prof_layout(A, M, n, with_mom, nullptr, nullptr);
+  MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
+              "mmn_bp_profile: workspace of %zu bytes needed", need);
+  ProfWs w;
+  prof_layout(A, M, n, with_mom, &w, (char *)workspace);
+  hipStream_t st = (hipStream_t)stream;
+  Box box;
+  for (int k = 0; k < 6; ++k) box.b[k] = bbox[k];
+  const double inv_c = 1.0 / MMN_C0;
+  const double tc = 0.5 * (double)(F - 1);
+  hipLaunchKernelGGL(k_prof_rows, dim3(mmn_blocks(A, kBlock)), dim3(kBlock), 0, st, w.dlo, ant,
+                     rx_off, box, A, num_rx, is_ti_radar, g.h);
+  hipLaunchKernelGGL(k_prof_table, dim3(mmn_blocks(M, kBlock), A), dim3(kBlock), 0, st, w.tab,
+                     (const float2 *)meas, w.dlo, g.h, df * inv_c, tc, M, F);
+  hipLaunchKernelGGL(k_prof_coef, dim3(mmn_blocks((int64_t)A * M, kBlock)), dim3(kBlock), 0,
+                     st, w.coef, w.tab, M, (int64_t)A * M);
+
+  // VPT 2 (two voxels per lane, a 8x8x8 brick per workgroup) unless
+  // MMN_PROF_VPT says 1
+  const int vpt = env_int("MMN_PROF_VPT", 2) == 1 ? 1 : 2;
+  auto kern = vpt == 1 ? k_bp_prof<1> : k_bp_prof<2>;
+  int64_t nblk;
+  if (vox_idx)
+    nblk = mmn_blocks(n, kBlock * vpt);
+  else
+    nblk = (int64_t)((nx + 7) / 8) * ((ny + 7) / 8) * ((nz + 4 * vpt - 1) / (4 * vpt));
+  // row segments: enough workgroups for eight rounds of resident slots
+  int per_cu = 0, dev = 0, cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 2;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipGetLastError();
+  int rs = 1;
+  const int forced = env_int("MMN_PROF_RS", 0);
+  if (forced > 0) {
+    rs = forced;
+  } else {
+    while (nblk * rs < 8 * (int64_t)per_cu * cus && rs < kMaxSeg) ++rs;
+  }
+  if (rs > seg_cap(A)) rs = seg_cap(A);
+  const int rps = (A + rs - 1) / rs;
+  const int nseg = (A + rps - 1) / rps;
+  float2 *im = (float2 *)img, *mo = (float2 *)moment;
+  if (nseg > 1) {
+    im = w.part;
+    mo = with_mom ? w.part + (size_t)nseg * n : nullptr;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nseg), dim3(kBlock), 0, st, im, mo, x_locs,
+                     y_locs, z_locs, vox_idx, n, nx, ny, nz, ant, rx_off, w.dlo,
+                     (const float4 *)w.coef, 1.0 / g.h, (f0 + tc * df) * inv_c, M, A, num_rx,
+                     is_ti_radar, rps);
+  if (nseg > 1) mmn_rs_reduce(st, (float2 *)img, (float2 *)moment, im, mo, n, nseg);
+  return mmn_check_launch("mmn_bp_profile");
+}

@@ -139,6 +139,31 @@ __device__ __forceinline__ void phasor_rev32(float u, float &re, float &im) {
   im = -s;
 }
 
+// Adds one row's contribution c to the image (Kahan) and the vote moment.
+__device__ __forceinline__ void add_row(float cr, float ci, float vecx,
+                                        float vecy, float vecz, float &Ir,
+                                        float &Ii, float &Cr, float &Ci,
+                                        float (&M)[6]) {
+  // Kahan: C carries the low-order part lost by the running sum.
+  const float yr = cr - Cr, yi = ci - Ci;
+  const float tr = Ir + yr, ti = Ii + yi;
+  Cr = (tr - Ir) - yr;
+  Ci = (ti - Ii) - yi;
+  Ir = tr;
+  Ii = ti;
+  M[0] = fmaf(vecx, cr, M[0]);
+  M[1] = fmaf(vecx, ci, M[1]);
+  M[2] = fmaf(vecy, cr, M[2]);
+  M[3] = fmaf(vecy, ci, M[3]);
+  M[4] = fmaf(vecz, cr, M[4]);
+  M[5] = fmaf(vecz, ci, M[5]);
+}
+
+// Fixed-order FP64 sum of rs row segments' partial images (pimg, rs x n) and
+// moments (pmom, rs x n x 3, may be null) into img / mom (bp_forward.hip).
+void mmn_rs_reduce(hipStream_t st, float2 *img, float2 *mom, const float2 *pimg,
+                   const float2 *pmom, int64_t n, int rs);
+
 static inline unsigned mmn_blocks(int64_t n, int per_block) {
   return (unsigned)((n + per_block - 1) / per_block);
 }

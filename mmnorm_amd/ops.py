@@ -15,13 +15,16 @@ f0 = c / lambda and df = slope.
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import _lib
 
 C0 = 2.99792458e8
+# mmn_bp_forward's variant codes; "profile" is its own entry point, mmn_bp_profile
 VARIANTS = {"plain": _lib.MMN_BP_PLAIN, "plain_fast": _lib.MMN_BP_PLAIN_FAST,
-            "rec": _lib.MMN_BP_REC}
+            "rec": _lib.MMN_BP_REC, "profile": None}
 
 
 def _check(t, name, dtype, ndim=None, last=None):
@@ -75,13 +78,26 @@ def _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx):
     return rx_off, nvox_grid
 
 
+def grid_bbox(x_locs, y_locs, z_locs):
+    """(xmin, ymin, zmin, xmax, ymax, zmax) of the voxel axes, as host doubles."""
+    lo_hi = [torch.aminmax(t) for t in (x_locs, y_locs, z_locs)]
+    v = torch.stack([m for m, _ in lo_hi] + [M for _, M in lo_hi]).double().cpu()
+    return [float(a) for a in v]
+
+
 def backproject(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
-                variant="rec", K=64, moment=False, vox_idx=None, is_ti=False):
+                variant="rec", K=64, moment=False, vox_idx=None, is_ti=False, bbox=None):
     """Raw full-frequency sum  I_v = sum_ij meas_ij exp(-j 2 pi (f0 + j df) d_iv / c).
 
     Returns ``img`` complex64 (NX, NY, NZ) -- or (n,) with ``vox_idx`` -- and, if
-    ``moment``, the vote moment complex64 (.., 3).
+    ``moment``, the vote moment complex64 (.., 3).  ``variant="profile"``
+    evaluates the same sum from per-row range-profile tables (mmn_bp_profile);
+    ``bbox`` (6 floats containing every voxel) saves it reading the axes'
+    extent back from the device.
     """
+    if variant == "profile":
+        return _backproject_profile(meas, ant, x_locs, y_locs, z_locs, rx_off, f0=f0, df=df,
+                                    moment=moment, vox_idx=vox_idx, is_ti=is_ti, bbox=bbox)
     rx_off, nvox_grid = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
     if variant not in VARIANTS:
         raise ValueError(f"variant must be one of {sorted(VARIANTS)}")
@@ -99,6 +115,40 @@ def backproject(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
                   x_locs.numel(), y_locs.numel(), z_locs.numel(), meas.shape[0],
                   rx_off.shape[0], meas.shape[1], int(bool(is_ti)), VARIANTS[variant],
                   int(K), _lib.ptr(ws), ws_bytes, _lib.stream_handle(dev))
+    if vox_idx is None:
+        img = img.view(x_locs.numel(), y_locs.numel(), z_locs.numel())
+        if mom is not None:
+            mom = mom.view(x_locs.numel(), y_locs.numel(), z_locs.numel(), 3)
+    return (img, mom) if moment else img
+
+
+def _backproject_profile(meas, ant, x_locs, y_locs, z_locs, rx_off, *, f0, df, moment,
+                         vox_idx, is_ti, bbox):
+    rx_off, nvox_grid = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
+    dev = meas.device
+    n = vox_idx.numel() if vox_idx is not None else nvox_grid
+    img = torch.empty(n, dtype=torch.complex64, device=dev)
+    mom = torch.empty((n, 3), dtype=torch.complex64, device=dev) if moment else None
+    if n:
+        if bbox is None:
+            bbox = grid_bbox(x_locs, y_locs, z_locs)
+        if len(bbox) != 6:
+            raise ValueError("bbox must hold 6 values")
+        bb = (ctypes.c_double * 6)(*[float(v) for v in bbox])
+        lib = _lib.load()
+        ws_bytes = int(lib.mmn_bp_profile_workspace_size(n, meas.shape[0], meas.shape[1],
+                                                         float(df), ctypes.addressof(bb),
+                                                         int(moment)))
+        if ws_bytes == 0:
+            raise ValueError("mmn_bp_profile: bounding box not finite or too large for the "
+                             "range-profile tables")
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        _lib.call("mmn_bp_profile", img.data_ptr(), _lib.ptr(mom), x_locs.data_ptr(),
+                  y_locs.data_ptr(), z_locs.data_ptr(), _lib.ptr(vox_idx), n,
+                  ant.data_ptr(), meas.data_ptr(), rx_off.data_ptr(), float(f0), float(df),
+                  x_locs.numel(), y_locs.numel(), z_locs.numel(), meas.shape[0],
+                  rx_off.shape[0], meas.shape[1], int(bool(is_ti)), ctypes.addressof(bb),
+                  ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
     if vox_idx is None:
         img = img.view(x_locs.numel(), y_locs.numel(), z_locs.numel())
         if mom is not None:
