@@ -95,6 +95,8 @@ def main(argv=None):
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--K", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--profile-steps", type=int, default=5,
+                    help="timed launches of the range-profile forward (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="forward only")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -188,6 +190,21 @@ def main(argv=None):
                                  "rec_over_plain_fast": f_s / k_avg,
                                  "kind": "hardware sin/cos of the FP32 phase fraction "
                                          "(revolutions)"}
+        if a.profile_steps > 0:
+            bb = ops.grid_bbox(xl, yl, zl)
+            pw, pk, (pimg, pmom) = timed(
+                lambda: ops.backproject(meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df,
+                                        variant="profile", moment=True, bbox=bb),
+                a.profile_steps, 1)
+            q_s = float(np.mean(pk)) * 1e-3
+            diff = lambda u, v: float(torch.linalg.norm(u - v) / torch.linalg.norm(v))
+            rec["profile"] = {"value": W / q_s / 1e9, "kernel_ms": q_s * 1e3,
+                              "speedup_over_rec": k_avg / q_s,
+                              "rel_diff_vs_rec": diff(pimg, img), "moment_rel_diff_vs_rec":
+                              diff(pmom, mom),
+                              "kind": "same sum from per-row cubic range-profile tables "
+                                      "(mmn_bp_profile); value = voxel*row*sample / time"}
+            log(f"profile forward: {q_s * 1e3:.2f} ms, {k_avg / q_s:.1f}x the recurrence")
         rec["config4_backward"] = config4_stage(dev)
         log(f"config 4 forward/backward: {rec['config4_backward']}")
         rec["interpolated"] = interp_stage(sc, ant, xl, yl, zl, dev)

@@ -125,18 +125,19 @@ int mmn_bp_forward(float *img, float *moment, const float *x_locs,
  * arguments, image and moment), evaluated per (voxel, row) from a table of
  * the row's band-centred range profile
  *     Q_i(d) = sum_j meas[i][j] exp(-j 2 pi (j - (F-1)/2) df d / c)
- * sampled every h = c / (64 (F-1) |df|) metres and interpolated by the cubic
- * through the four nearest samples, times the carrier
- * exp(-j 2 pi (f0 + (F-1)/2 df) d / c) reduced in FP64.  The interpolation
- * error is a few 1e-8 of sum_j |meas[i][j]| per row.  bbox (host, 6
- * doubles: x, y, z minimum then maximum) must contain every voxel; it sets
- * the table length M = ceil(2 |bbox diagonal| / h) + 6 per row.  workspace
+ * sampled every h = min(c / (64 (F-1) |df|), c / (2 f_c)) metres and
+ * interpolated by the cubic through the four nearest samples, times the
+ * carrier exp(-j 2 pi f_c d / c), f_c = f0 + (F-1)/2 df, reduced in FP64 at
+ * each table point.  The interpolation error is a few 1e-8 of
+ * sum_j |meas[i][j]| per row.  bbox (host, 6 doubles: x, y, z minimum then
+ * maximum) must contain every voxel; it sets the table length
+ * M = ceil(2 |bbox diagonal| / h) + 6 per row.  workspace
  * is required: mmn_bp_profile_workspace_size() bytes (per-row tables, and
  * row-segment partials reduced in a fixed order).  No reference
  * counterpart: a faster evaluation of imaging_gpu.cu:106-117's sum.
  */
-size_t mmn_bp_profile_workspace_size(int64_t n_vox, int num_rows, int samples, double df,
-                                     const double *bbox, int with_moment);
+size_t mmn_bp_profile_workspace_size(int64_t n_vox, int num_rows, int samples, double f0,
+                                     double df, const double *bbox, int with_moment);
 int mmn_bp_profile(float *img, float *moment, const float *x_locs, const float *y_locs,
                    const float *z_locs, const int32_t *vox_idx, int64_t n_vox,
                    const float *ant, const float *meas, const float *rx_off, double f0,

@@ -43,7 +43,8 @@ SIGNATURES = {
                                 _c_d, _c_d, _c_int, _c_int, _c_int, _c_int,
                                 _c_int, _c_int, _c_int, _c_int, _c_int, _p, _sz,
                                 _p]),
-    "mmn_bp_profile_workspace_size": (_sz, [_c_i64, _c_int, _c_int, _c_d, _p, _c_int]),
+    "mmn_bp_profile_workspace_size": (_sz, [_c_i64, _c_int, _c_int, _c_d, _c_d, _p,
+                                             _c_int]),
     "mmn_bp_profile": (_c_int, [_p, _p, _p, _p, _p, _p, _c_i64, _p, _p, _p, _c_d, _c_d,
                                 _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p,
                                 _p, _sz, _p]),

@@ -11,17 +11,23 @@
 // reproduces Q_i to a few 1e-8 of sum_t |s_it| (the Lagrange remainder of a
 // tone at pi/64 rad per sample is at most 0.0234 (pi/64)^4 = 1.4e-7 of its
 // amplitude, at mid-interval).  So:
-//   k_prof_rows   per row, the table start: a lower bound of the row's
-//                 round-trip distance over the grid's bounding box, less h;
+//   k_prof_rows   per row, the table start (a lower bound of the row's
+//                 round-trip distance over the grid's bounding box, less h)
+//                 and the FP64 constants of the per-voxel evaluation;
 //   k_prof_table  Q_i at M points h apart, by Horner in exp(-j 2 pi df d / c)
 //                 re-anchored every 64 samples by an FP64-reduced phasor (the
 //                 recurrence kernel's arithmetic);
-//   k_prof_coef   each interval's cubic as four complex coefficients (32 B);
+//   k_prof_coef   each interval's cubic as four complex coefficients (32 B),
+//                 times the carrier exp(-j 2 pi f_c d_m / c) at the interval
+//                 start d_m (FP64 phase);
 //   k_bp_prof     per (row, voxel): the FP64 round-trip distance, interval
 //                 and fraction, one 32-byte coefficient fetch, the cubic
-//                 (6 FMA), the FP64-reduced carrier exp(-j 2 pi f_c d / c),
-//                 then the Kahan row sum and the vote moment exactly as the
-//                 recurrence kernel adds them.
+//                 (6 FMA), the carrier's remainder past the interval start
+//                 (under half a turn: FP32 phase, hardware sine/cosine), the
+//                 vote moment as the recurrence kernel forms it, and the row
+//                 sum -- plain FP32 within blocks of 8 rows, Kahan-compensated
+//                 across blocks.  The TI offset and a monostatic row's factor
+//                 2 are folded into per-row constants.
 // A round trip over the box spans at most twice its diagonal, so M =
 // ceil(2 diag / h) + 6 points cover every voxel with the cubic's neighbours.
 //
@@ -41,6 +47,7 @@ constexpr int kOver = 64;     // table points per range-resolution cell
 constexpr int kHornerK = 64;  // samples per FP64 re-anchor in the table
 constexpr int kMaxSeg = 8;    // most row segments of one launch
 constexpr int kMaxM = 1 << 22;
+constexpr int kRowBlock = 8;  // rows summed in plain FP32 between Kahan steps
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -49,14 +56,20 @@ struct ProfGeom {
   int M;
 };
 
-// Table spacing and length for F samples df apart over a box (host).
-bool prof_geom(int F, double df, const double *bb, ProfGeom *g) {
+// Table spacing and length for F samples from f0, df apart, over a box
+// (host).  h is also capped at half the centre wavelength, which keeps the
+// carrier's remainder within an interval under half a turn.
+bool prof_geom(int F, double f0, double df, const double *bb, ProfGeom *g) {
   const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
   if (!(ex >= 0.0 && ey >= 0.0 && ez >= 0.0) || !isfinite(ex + ey + ez)) return false;
+  if (!isfinite(f0) || !isfinite(df)) return false;
   const double diag = sqrt(ex * ex + ey * ey + ez * ez);
   const double band = (double)(F - 1) * fabs(df);
+  const double fc = fabs(f0 + 0.5 * (double)(F - 1) * df);
   // with one sample or df = 0, Q_i is constant and any spacing is exact
-  g->h = band > 0.0 ? MMN_C0 / (kOver * band) : (2.0 * diag + 1e-3) / 8.0;
+  double h = band > 0.0 ? MMN_C0 / (kOver * band) : (2.0 * diag + 1e-3) / 8.0;
+  if (fc > 0.0 && h > 0.5 * MMN_C0 / fc) h = 0.5 * MMN_C0 / fc;
+  g->h = h;
   const double m = ceil(2.0 * diag / g->h) + 6.0;
   if (!(m <= (double)kMaxM)) return false;
   g->M = (int)m;
@@ -69,8 +82,14 @@ int seg_cap(int rows) {
   return cap < 1 ? 1 : cap;
 }
 
+// Per-row constants of k_bp_prof, 12 doubles: u offset, |p|^2, -2p,
+// |p+o|^2, -2(p+o), the distance-to-u scale (doubled for a monostatic row,
+// whose round trip is twice one leg), pad.
+constexpr int kRowC = 12;
+
 struct ProfWs {
   double *dlo;   // [A]
+  double *rowc;  // [A][kRowC]
   float2 *tab;   // [A][M]
   float4 *coef;  // [A][M][2]
   float2 *part;  // [segments][n] images, then [segments][n][3] moments
@@ -85,11 +104,13 @@ size_t prof_layout(int A, int M, int64_t n, bool moment, ProfWs *w, char *base) 
     return q;
   };
   char *d = take(sizeof(double) * (size_t)A);
+  char *rc = take(sizeof(double) * kRowC * (size_t)A);
   char *t = take(sizeof(float2) * (size_t)A * M);
   char *c = take(sizeof(float4) * 2 * (size_t)A * M);
   char *p = segs > 1 ? take(sizeof(float2) * (size_t)segs * n * (moment ? 4 : 1)) : nullptr;
   if (w) {
     w->dlo = (double *)d;
+    w->rowc = (double *)rc;
     w->tab = (float2 *)t;
     w->coef = (float4 *)c;
     w->part = (float2 *)p;
@@ -110,6 +131,7 @@ struct Box {
 };
 
 __global__ __launch_bounds__(kBlock) void k_prof_rows(double *__restrict__ dlo,
+                                                      double *__restrict__ rowc,
                                                       const float *__restrict__ ant,
                                                       const float *__restrict__ rxo, Box box,
                                                       int A, int R, int is_ti, double h) {
@@ -117,11 +139,27 @@ __global__ __launch_bounds__(kBlock) void k_prof_rows(double *__restrict__ dlo,
   if (i >= A) return;
   const double px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
   const int rx = i % R;
-  const double qx = px + (double)rxo[3 * rx + 0], qy = py + (double)rxo[3 * rx + 1],
-               qz = pz + (double)rxo[3 * rx + 2];
-  double d = box_dist(px, py, pz, box.b) + box_dist(qx, qy, qz, box.b);
-  if (is_ti) d += 0.15;
-  dlo[i] = d - h;  // table point k sits at dlo + k h; voxels fall at k >= 1
+  const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+  const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
+  const double qx = px + (double)ox, qy = py + (double)oy, qz = pz + (double)oz;
+  const double ti = is_ti ? 0.15 : 0.0;
+  const double lo = box_dist(px, py, pz, box.b) + box_dist(qx, qy, qz, box.b) + ti - h;
+  dlo[i] = lo;  // table point k sits at lo + k h; voxels fall at k >= 1
+  const double inv_h = 1.0 / h, two = mono ? 2.0 : 1.0;
+  double *__restrict__ c = rowc + (size_t)kRowC * i;
+  // u = (d_geom + ti - lo) / h with d_geom = two * (the legs as evaluated)
+  c[0] = (ti - lo) * inv_h;
+  c[1] = fma(pz, pz, fma(py, py, px * px));
+  c[2] = -2.0 * px;
+  c[3] = -2.0 * py;
+  c[4] = -2.0 * pz;
+  c[5] = fma(qz, qz, fma(qy, qy, qx * qx));
+  c[6] = -2.0 * qx;
+  c[7] = -2.0 * qy;
+  c[8] = -2.0 * qz;
+  c[9] = two * inv_h;
+  c[10] = 0.0;
+  c[11] = 0.0;
 }
 
 // One workgroup row of lanes per table row (blockIdx.y): the row's samples
@@ -157,12 +195,17 @@ __global__ __launch_bounds__(kBlock) void k_prof_table(
 
 // The cubic through table points m-1 .. m+2 as a polynomial in the fraction
 // f in [0, 1) of interval [m, m+1]:
-//   P(f) = q + c1 f + c2 f^2 + c3 f^3,  c1 = -p/3 - q/2 + r - s/6,
+//   Q(f) = q + c1 f + c2 f^2 + c3 f^3,  c1 = -p/3 - q/2 + r - s/6,
 //   c2 = (p + r)/2 - q,  c3 = (s - p)/6 + (q - r)/2,
-// stored as (q, c1) and (c2, c3).  Intervals without four neighbours hold 0.
+// times the carrier at the interval start, A_m = exp(-j 2 pi f_c d_m / c)
+// with d_m = dlo + m h (FP64 phase, FP64 sine/cosine): the voxel then needs
+// only the carrier's remainder exp(-j 2 pi (f_c h / c) f), under half a turn
+// per interval.  Stored as A_m (q, c1) and A_m (c2, c3); intervals without
+// four neighbours hold 0.
 __global__ __launch_bounds__(kBlock) void k_prof_coef(float4 *__restrict__ coef,
-                                                      const float2 *__restrict__ tab, int M,
-                                                      int64_t AM) {
+                                                      const float2 *__restrict__ tab,
+                                                      const double *__restrict__ dlo, double h,
+                                                      double fcc, int M, int64_t AM) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= AM) return;
   const int m = (int)(t % M);
@@ -171,17 +214,37 @@ __global__ __launch_bounds__(kBlock) void k_prof_coef(float4 *__restrict__ coef,
     const float2 pf = tab[t - 1], qf = tab[t], rf = tab[t + 1], sf = tab[t + 2];
     const double p[2] = {pf.x, pf.y}, q[2] = {qf.x, qf.y}, r[2] = {rf.x, rf.y},
                  s[2] = {sf.x, sf.y};
-    double c1[2], c2[2], c3[2];
+    double c[4][2];
     for (int k = 0; k < 2; ++k) {
-      c1[k] = -p[k] / 3.0 - q[k] / 2.0 + r[k] - s[k] / 6.0;
-      c2[k] = (p[k] + r[k]) / 2.0 - q[k];
-      c3[k] = (s[k] - p[k]) / 6.0 + (q[k] - r[k]) / 2.0;
+      c[0][k] = q[k];
+      c[1][k] = -p[k] / 3.0 - q[k] / 2.0 + r[k] - s[k] / 6.0;
+      c[2][k] = (p[k] + r[k]) / 2.0 - q[k];
+      c[3][k] = (s[k] - p[k]) / 6.0 + (q[k] - r[k]) / 2.0;
     }
-    lo = make_float4(qf.x, qf.y, (float)c1[0], (float)c1[1]);
-    hi = make_float4((float)c2[0], (float)c2[1], (float)c3[0], (float)c3[1]);
+    double ph = (dlo[t / M] + (double)m * h) * fcc;
+    ph -= rint(ph);
+    double sn, cs;
+    sincos(2.0 * 3.14159265358979323846 * ph, &sn, &cs);
+    const double ar = cs, ai = -sn;
+    float o[8];
+    for (int k = 0; k < 4; ++k) {
+      o[2 * k] = (float)(c[k][0] * ar - c[k][1] * ai);
+      o[2 * k + 1] = (float)(c[k][0] * ai + c[k][1] * ar);
+    }
+    lo = make_float4(o[0], o[1], o[2], o[3]);
+    hi = make_float4(o[4], o[5], o[6], o[7]);
   }
   coef[2 * t + 0] = lo;
   coef[2 * t + 1] = hi;
+}
+
+// One leg |v - p| from |v|^2 - 2 v.p + |p|^2, with c pointing at the row
+// constants -2p and c[-1] = |p|^2:
+// the FP64 sum cancels by at most (|v|^2 + |p|^2) / |v - p|^2, a few units
+// here, so the leg keeps ~1e-15 relative accuracy.
+__device__ __forceinline__ double leg64(double vx, double vy, double vz, double vv,
+                                        const double *__restrict__ c) {
+  return sqrt_nr64(fma(vz, c[2], fma(vy, c[1], fma(vx, c[0], vv + c[-1]))));
 }
 
 // Per (row, voxel).  With vidx, lane t of the launch takes vidx[t] and writes
@@ -195,8 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
     const float *__restrict__ yl, const float *__restrict__ zl,
     const int32_t *__restrict__ vidx, int64_t n, int nx, int ny, int nz,
     const float *__restrict__ ant, const float *__restrict__ rxo,
-    const double *__restrict__ dlo, const float4 *__restrict__ coef, double inv_h,
-    double fcc, int M, int rows, int R, int is_ti, int rps) {
+    const double *__restrict__ rowc, const float4 *__restrict__ coef, float thf, int M,
+    int rows, int R, int rps) {
   float vx[VPT], vy[VPT], vz[VPT];
   int64_t out[VPT];
   bool ok[VPT];
@@ -226,44 +289,98 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
       vz[k] = zl[min(z, nz - 1)];
     }
   }
+  double dx[VPT], dy[VPT], dz[VPT], vv[VPT];
   float Ir[VPT], Ii[VPT], Cr[VPT], Ci[VPT], Mo[VPT][6];
 #pragma unroll
   for (int k = 0; k < VPT; ++k) {
+    dx[k] = vx[k];
+    dy[k] = vy[k];
+    dz[k] = vz[k];
+    vv[k] = fma(dz[k], dz[k], fma(dy[k], dy[k], dx[k] * dx[k]));
     Ir[k] = Ii[k] = Cr[k] = Ci[k] = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) Mo[k][q] = 0.0f;
   }
-  const double umax = (double)(M - 3);
+  const float umax = (float)(M - 3);
   const int row_lo = blockIdx.y * rps;
   const int row_hi = min(rows, row_lo + rps);
-  for (int i = row_lo; i < row_hi; ++i) {
-    const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+
+  // Row i's geometry for every voxel of the lane: the interval's two
+  // coefficient vectors (fetched) and the fraction.
+  auto fetch = [&](int i, float (&f)[VPT], float4 (&a)[VPT], float4 (&b)[VPT]) {
     const int rx = i % R;
-    const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
-    const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
-    const double dl = dlo[i];
+    const bool mono = (rxo[3 * rx + 0] == 0.0f) & (rxo[3 * rx + 1] == 0.0f) &
+                      (rxo[3 * rx + 2] == 0.0f);
+    const double *__restrict__ c = rowc + (size_t)kRowC * i;
     const float4 *__restrict__ crow = coef + (size_t)i * M * 2;
-    const float vax = px + ox / 2.0f, vay = py + oy / 2.0f, vaz = pz + oz / 2.0f;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      double d = roundtrip_dist64(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono);
-      if (is_ti) d += 0.15;
-      // interval and fraction; the clamp keeps any input (NaN included)
-      // inside the table
-      const double u = fmin(fmax((d - dl) * inv_h, 1.0), umax);
-      const int m = (int)u;
-      const float f = (float)(u - (double)m);
-      const float4 a = crow[2 * m], c = crow[2 * m + 1];
-      const float qr = fmaf(fmaf(fmaf(c.z, f, c.x), f, a.z), f, a.x);
-      const float qi = fmaf(fmaf(fmaf(c.w, f, c.y), f, a.w), f, a.y);
-      float ar, ai;
-      phasor_hw(d * fcc, ar, ai);
+      // the legs' sum; c[9] doubles it for a monostatic row
+      double ds = leg64(dx[k], dy[k], dz[k], vv[k], c + 2);
+      if (!mono) ds += leg64(dx[k], dy[k], dz[k], vv[k], c + 6);
+      const double u = fma(ds, c[9], c[0]);
+      // interval from the clamped FP32 u (NaN-safe), fraction from the FP64
+      // u; where FP32 rounding moved m across an integer the fraction lies
+      // just outside [0, 1), still inside the cubic's nodes [-1, 2]
+      const int m = (int)fminf(fmaxf((float)u, 1.0f), umax);
+      f[k] = __builtin_amdgcn_fmed3f((float)(u - (double)m), -1.0f, 2.0f);
+      a[k] = crow[2 * m];
+      b[k] = crow[2 * m + 1];
+    }
+  };
+  // Row i's contribution from its fetched coefficients: the cubic, the
+  // carrier's remainder past the interval start (thf f turns), the block
+  // sum and the vote moment.
+  auto finish = [&](int i, const float (&f)[VPT], const float4 (&a)[VPT],
+                    const float4 (&b)[VPT], float (&Sr)[VPT], float (&Si)[VPT]) {
+    const int rx = i % R;
+    const float vax = ant[3 * i + 0] + rxo[3 * rx + 0] / 2.0f;
+    const float vay = ant[3 * i + 1] + rxo[3 * rx + 1] / 2.0f;
+    const float vaz = ant[3 * i + 2] + rxo[3 * rx + 2] / 2.0f;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const float qr = fmaf(fmaf(fmaf(b[k].z, f[k], b[k].x), f[k], a[k].z), f[k], a[k].x);
+      const float qi = fmaf(fmaf(fmaf(b[k].w, f[k], b[k].y), f[k], a[k].w), f[k], a[k].y);
+      const float ph = thf * f[k];
+      const float ar = __builtin_amdgcn_cosf(ph), ai = -__builtin_amdgcn_sinf(ph);
       const float cr = fmaf(qr, ar, -(qi * ai));
       const float ci = fmaf(qr, ai, qi * ar);
-      add_row(cr, ci, vax - vx[k], vay - vy[k], vaz - vz[k], Ir[k], Ii[k], Cr[k], Ci[k],
-              Mo[k]);
+      Sr[k] += cr;
+      Si[k] += ci;
+      const float wx = vax - vx[k], wy = vay - vy[k], wz = vaz - vz[k];
+      Mo[k][0] = fmaf(wx, cr, Mo[k][0]);
+      Mo[k][1] = fmaf(wx, ci, Mo[k][1]);
+      Mo[k][2] = fmaf(wy, cr, Mo[k][2]);
+      Mo[k][3] = fmaf(wy, ci, Mo[k][3]);
+      Mo[k][4] = fmaf(wz, cr, Mo[k][4]);
+      Mo[k][5] = fmaf(wz, ci, Mo[k][5]);
     }
+  };
+  // rows are added in plain FP32 in blocks of kRowBlock, each block's sum
+  // then once into the Kahan-compensated total
+  auto flush = [&](float (&Sr)[VPT], float (&Si)[VPT]) {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const float yr = Sr[k] - Cr[k], yi = Si[k] - Ci[k];
+      const float tr = Ir[k] + yr, ti = Ii[k] + yi;
+      Cr[k] = (tr - Ir[k]) - yr;
+      Ci[k] = (ti - Ii[k]) - yi;
+      Ir[k] = tr;
+      Ii[k] = ti;
+      Sr[k] = Si[k] = 0.0f;
+    }
+  };
+  float Sr[VPT], Si[VPT];
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) Sr[k] = Si[k] = 0.0f;
+  for (int i = row_lo; i < row_hi; ++i) {
+    float f[VPT];
+    float4 a[VPT], b[VPT];
+    fetch(i, f, a, b);
+    finish(i, f, a, b, Sr, Si);
+    if (((i - row_lo) % kRowBlock) == kRowBlock - 1) flush(Sr, Si);
   }
+  flush(Sr, Si);
   const size_t seg = blockIdx.y;
   float2 *__restrict__ im = img + seg * n;
   float2 *__restrict__ mo = mom ? mom + seg * n * 3 : nullptr;
@@ -288,10 +405,11 @@ int env_int(const char *name, int dflt) {
 }  // namespace
 
 extern "C" size_t mmn_bp_profile_workspace_size(int64_t n_vox, int num_rows, int samples,
-                                                double df, const double *bbox,
+                                                double f0, double df, const double *bbox,
                                                 int with_moment) {
   ProfGeom g;
-  if (n_vox <= 0 || num_rows <= 0 || samples <= 0 || !bbox || !prof_geom(samples, df, bbox, &g))
+  if (n_vox <= 0 || num_rows <= 0 || samples <= 0 || !bbox ||
+      !prof_geom(samples, f0, df, bbox, &g))
     return 0;
   return prof_layout(num_rows, g.M, n_vox, with_moment != 0, nullptr, nullptr);
 }
@@ -308,9 +426,9 @@ extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
   MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 && samples > 0,
               MMN_ERR_ARG, "mmn_bp_profile: sizes must be positive");
   ProfGeom g;
-  MMN_REQUIRE(prof_geom(samples, df, bbox, &g), MMN_ERR_ARG,
-              "mmn_bp_profile: bounding box must be finite with min <= max, and its "
-              "round-trip span at most %d table points", kMaxM);
+  MMN_REQUIRE(prof_geom(samples, f0, df, bbox, &g), MMN_ERR_ARG,
+              "mmn_bp_profile: f0, df and the bounding box must be finite (box min <= "
+              "max), and the box's round-trip span at most %d table points", kMaxM);
   int64_t n = (int64_t)nx * ny * nz;
   if (vox_idx) {
     MMN_REQUIRE(n_vox >= 0, MMN_ERR_ARG, "mmn_bp_profile: n_vox < 0");
@@ -329,15 +447,18 @@ extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
   for (int k = 0; k < 6; ++k) box.b[k] = bbox[k];
   const double inv_c = 1.0 / MMN_C0;
   const double tc = 0.5 * (double)(F - 1);
-  hipLaunchKernelGGL(k_prof_rows, dim3(mmn_blocks(A, kBlock)), dim3(kBlock), 0, st, w.dlo, ant,
-                     rx_off, box, A, num_rx, is_ti_radar, g.h);
+  const double fcc = (f0 + tc * df) * inv_c;
+  hipLaunchKernelGGL(k_prof_rows, dim3(mmn_blocks(A, kBlock)), dim3(kBlock), 0, st, w.dlo,
+                     w.rowc, ant, rx_off, box, A, num_rx, is_ti_radar, g.h);
   hipLaunchKernelGGL(k_prof_table, dim3(mmn_blocks(M, kBlock), A), dim3(kBlock), 0, st, w.tab,
                      (const float2 *)meas, w.dlo, g.h, df * inv_c, tc, M, F);
   hipLaunchKernelGGL(k_prof_coef, dim3(mmn_blocks((int64_t)A * M, kBlock)), dim3(kBlock), 0,
-                     st, w.coef, w.tab, M, (int64_t)A * M);
+                     st, w.coef, w.tab, w.dlo, g.h, fcc, M, (int64_t)A * M);
 
-  // VPT 2 (two voxels per lane, a 8x8x8 brick per workgroup) unless
-  // MMN_PROF_VPT says 1
+  // Two voxels per lane (an 8x8x8 brick per workgroup) unless MMN_PROF_VPT
+  // says 1.  Measured on MI355X (config 2 / config 5): 8.5 / 124 ms at two,
+  // 9.4 / 145 ms at one, 9.0 / 135 ms at four; fetching row i + 1's
+  // coefficients while row i finishes (software pipelining) cost 7-25 %.
   const int vpt = env_int("MMN_PROF_VPT", 2) == 1 ? 1 : 2;
   auto kern = vpt == 1 ? k_bp_prof<1> : k_bp_prof<2>;
   int64_t nblk;
@@ -369,9 +490,8 @@ extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
     mo = with_mom ? w.part + (size_t)nseg * n : nullptr;
   }
   hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nseg), dim3(kBlock), 0, st, im, mo, x_locs,
-                     y_locs, z_locs, vox_idx, n, nx, ny, nz, ant, rx_off, w.dlo,
-                     (const float4 *)w.coef, 1.0 / g.h, (f0 + tc * df) * inv_c, M, A, num_rx,
-                     is_ti_radar, rps);
+                     y_locs, z_locs, vox_idx, n, nx, ny, nz, ant, rx_off, w.rowc,
+                     (const float4 *)w.coef, (float)(fcc * g.h), M, A, num_rx, rps);
   if (nseg > 1) mmn_rs_reduce(st, (float2 *)img, (float2 *)moment, im, mo, n, nseg);
   return mmn_check_launch("mmn_bp_profile");
 }

@@ -137,7 +137,8 @@ def _backproject_profile(meas, ant, x_locs, y_locs, z_locs, rx_off, *, f0, df, m
         bb = (ctypes.c_double * 6)(*[float(v) for v in bbox])
         lib = _lib.load()
         ws_bytes = int(lib.mmn_bp_profile_workspace_size(n, meas.shape[0], meas.shape[1],
-                                                         float(df), ctypes.addressof(bb),
+                                                         float(f0), float(df),
+                                                         ctypes.addressof(bb),
                                                          int(moment)))
         if ws_bytes == 0:
             raise ValueError("mmn_bp_profile: bounding box not finite or too large for the "
