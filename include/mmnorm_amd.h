@@ -146,6 +146,30 @@ int mmn_bp_profile(float *img, float *moment, const float *x_locs, const float *
                    size_t workspace_bytes, void *stream);
 
 /*
+ * mmn_bp_profile_backward -- the exact gradient of mmn_bp_profile for a real
+ * loss, arguments as mmn_bp_backward plus bbox (as mmn_bp_profile):
+ *     grad_ant[i]  = scale sum_v Im(conj(g_v) H_i(d_iv)) grad_p d_iv,
+ *     H_i(d) = sum_j k_j meas[i][j] exp(-j k_j d), k_j = 2 pi (f0 + j df) / c,
+ * interpolated from its own range-profile table; grad_meas the adjoint of
+ * the profile interpolation (each voxel spreads conj(phasor) g_v onto its
+ * four table points with the cubic's weights) followed by the table-to-
+ * sample transform.  Either output may be NULL.  The table must fit an LDS
+ * grid: M <= 2048 points per row (a bounding box up to ~1.2 m of round trip
+ * at 4 GHz); larger boxes get MMN_ERR_ARG.  workspace is required:
+ * mmn_bp_profile_backward_workspace_size() bytes.  Results are deterministic
+ * for a given device.  No reference counterpart.
+ */
+size_t mmn_bp_profile_backward_workspace_size(int64_t n_vox, int num_rows, int samples,
+                                              double f0, double df, const double *bbox);
+int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const float *g, double scale,
+                            const float *x_locs, const float *y_locs, const float *z_locs,
+                            const int32_t *vox_idx, int64_t n_vox, const float *ant,
+                            const float *meas, const float *rx_off, double f0, double df,
+                            int nx, int ny, int nz, int num_rows, int num_rx, int samples,
+                            int is_ti_radar, const double *bbox, void *workspace,
+                            size_t workspace_bytes, void *stream);
+
+/*
  * mmn_bp_interp -- the interpolated (range-bin) branch of imaging_gpu.cu:90-105
  * in MI355X form, plus the vote moment.  Per (voxel, row): the reference's
  * FP32 distance and bin floor(d / fft_spacing / 2) with the same operations

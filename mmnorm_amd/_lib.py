@@ -48,6 +48,11 @@ SIGNATURES = {
     "mmn_bp_profile": (_c_int, [_p, _p, _p, _p, _p, _p, _c_i64, _p, _p, _p, _c_d, _c_d,
                                 _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p,
                                 _p, _sz, _p]),
+    "mmn_bp_profile_backward_workspace_size": (_sz, [_c_i64, _c_int, _c_int, _c_d, _c_d,
+                                                      _p]),
+    "mmn_bp_profile_backward": (_c_int, [_p, _p, _p, _c_d, _p, _p, _p, _p, _c_i64, _p, _p,
+                                         _p, _c_d, _c_d, _c_int, _c_int, _c_int, _c_int,
+                                         _c_int, _c_int, _c_int, _p, _p, _sz, _p]),
     "mmn_bp_interp": (_c_int, [_p, _p, _p, _p, _p, _p, _c_i64, _p, _p, _p, _c_f, _c_f,
                                _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
     "mmn_sn_from_moment": (_c_int, [_p, _p, _p, _p, _c_i64, _p]),

@@ -164,9 +164,12 @@ __global__ __launch_bounds__(kBlock) void k_prof_rows(double *__restrict__ dlo,
 
 // One workgroup row of lanes per table row (blockIdx.y): the row's samples
 // are wave-uniform and arrive through scalar loads.
+// With kscale the samples are weighted by k_j = 2 pi f_j / c = k0 + j dk
+// (the derivative profile the backward's position term reads).
 __global__ __launch_bounds__(kBlock) void k_prof_table(
     float2 *__restrict__ tab, const float2 *__restrict__ meas,
-    const double *__restrict__ dlo, double h, double dfc, double tc, int M, int F) {
+    const double *__restrict__ dlo, double h, double dfc, double tc, int M, int F,
+    int kscale, float k0, float dk) {
   const int i = blockIdx.y;
   const int k = blockIdx.x * kBlock + threadIdx.x;
   const double d = dlo[i] + (double)min(k, M - 1) * h;
@@ -176,10 +179,19 @@ __global__ __launch_bounds__(kBlock) void k_prof_table(
   float cr = 0.0f, ci = 0.0f;
   for (int b0 = 0; b0 < F; b0 += kHornerK) {
     const int len = min(kHornerK, F - b0);
-    const float2 top = row[b0 + len - 1];
+    auto sample = [&](int j) {
+      float2 v = row[j];
+      if (kscale) {
+        const float kj = fmaf((float)j, dk, k0);
+        v.x *= kj;
+        v.y *= kj;
+      }
+      return v;
+    };
+    const float2 top = sample(b0 + len - 1);
     float hr = top.x, hi = top.y;
     for (int t = len - 2; t >= 0; --t) {
-      const float2 s = row[b0 + t];
+      const float2 s = sample(b0 + t);
       const float tr = fmaf(-hi, ri, fmaf(hr, rr, s.x));
       const float ti = fmaf(hi, rr, fmaf(hr, ri, s.y));
       hr = tr;
@@ -397,6 +409,271 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
   }
 }
 
+// --------------------------------------------------------------------------
+// Backward of the range-profile forward, for a real loss with g = dL/dI.
+//   dL/dp_i = scale sum_v Im(conj(g_v) H_i(d_iv)) grad_p d_iv,
+//   H_i(d)  = sum_j k_j s_ij exp(-j 2 pi f_j d / c)   (k_j = 2 pi f_j / c),
+// with H_i interpolated from its own profile table exactly as the forward
+// interpolates P_i; and the adjoint of the forward's interpolation (the
+// exact gradient of the profile forward): each voxel spreads
+//   w_k(f) exp(+j 2 pi thf (f + 1 - k)) g_v,   k = 0..3,
+// onto table points m - 1 + k of its row (w_k the cubic's Lagrange weights
+// at nodes -1, 0, 1, 2, thf = f_c h / c the carrier's turns per interval),
+// and then
+//   dL/ds_ij = scale sum_m G_i[m] exp(+j 2 pi f_j (dlo_i + m h) / c).
+// k_bwd_prof gives each wave one row and a chunk of voxels: the row's table
+// points accumulate in the wave's own LDS grid (ds_add, no other wave
+// touches it) and the position term in lanes, so nothing crosses waves until
+// the per-(chunk, row) partials are reduced in a fixed order.
+// --------------------------------------------------------------------------
+constexpr int kBwdMaxM = 2048;  // LDS grid per wave: M float2 (4 waves: 64 KiB)
+constexpr int kBwdPad = 128;    // voxel records padded to 2 per lane
+constexpr int kDftK = 32;       // grid points per FP64 re-anchor in k_bwd_prof_dft
+
+// One leg as leg64, also returning the FP32 inverse length.
+__device__ __forceinline__ double leg64y(double vx, double vy, double vz, double vv,
+                                         const double *__restrict__ c, float &inv) {
+  const double s = fma(vz, c[2], fma(vy, c[1], fma(vx, c[0], vv + c[-1])));
+  const float y = __builtin_amdgcn_rsqf((float)s);
+  inv = y;
+  const double yd = (double)y;
+  const double d0 = s * yd;
+  return fma(0.5 * yd, fma(-d0, d0, s), d0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bwdp_pack(
+    float4 *__restrict__ rec4, float *__restrict__ recg, const float2 *__restrict__ g,
+    const float *__restrict__ xl, const float *__restrict__ yl,
+    const float *__restrict__ zl, const int32_t *__restrict__ vidx, int64_t n,
+    int64_t npad, int ny, int nz) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= npad) return;
+  const bool real = t < n;
+  const int64_t src = real ? t : n - 1;
+  const int64_t lin = vidx ? (int64_t)vidx[src] : src;
+  float vx, vy, vz;
+  voxel_coords(xl, yl, zl, ny, nz, lin, vx, vy, vz);
+  const float2 gv = real ? g[t] : make_float2(0.0f, 0.0f);
+  rec4[t] = make_float4(vx, vy, vz, gv.x);
+  recg[t] = gv.y;
+}
+
+struct TapPhases {
+  float r[4], i[4];  // exp(+j 2 pi thf (1 - k)), k = 0..3
+};
+
+__global__ __launch_bounds__(kBlock) void k_bwd_prof(
+    float2 *__restrict__ gpart, float *__restrict__ ppart, const float4 *__restrict__ rec4,
+    const float *__restrict__ recg, int64_t npad, int64_t chunk,
+    const float *__restrict__ ant, const float *__restrict__ rxo,
+    const double *__restrict__ rowc, const float4 *__restrict__ coefh, float thf,
+    TapPhases tp, int M, int A, int R) {
+  extern __shared__ float2 lds_grid[];  // [4][M]
+  const int lane = threadIdx.x % 64;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int i_raw = blockIdx.x * 4 + wv;
+  const bool live = i_raw < A;
+  const int i = live ? i_raw : A - 1;
+  const int c = blockIdx.y;
+  float2 *__restrict__ grid = lds_grid + (size_t)wv * M;
+  for (int m = lane; m < M; m += 64) grid[m] = make_float2(0.0f, 0.0f);
+  __syncthreads();
+
+  const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+  const int rx = i % R;
+  const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+  const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
+  const float qx = px + ox, qy = py + oy, qz = pz + oz;
+  const double *__restrict__ rc = rowc + (size_t)kRowC * i;
+  const float4 *__restrict__ crow = coefh + (size_t)i * M * 2;
+  const float umax = (float)(M - 3);
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+
+  // chunk and npad are multiples of kBwdPad, so vb + 64 is a record too
+  const int64_t v0 = (int64_t)c * chunk;
+  const int64_t v1 = min(npad, v0 + chunk);
+  for (int64_t vb = v0 + lane; vb < v1; vb += kBwdPad) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const float4 r = rec4[vb + 64 * k];
+      const float gr = r.w, gi = recg[vb + 64 * k];
+      const double dx = r.x, dy = r.y, dz = r.z;
+      const double vv = fma(dz, dz, fma(dy, dy, dx * dx));
+      float y1, y2 = 0.0f;
+      double ds = leg64y(dx, dy, dz, vv, rc + 2, y1);
+      if (!mono) ds += leg64y(dx, dy, dz, vv, rc + 6, y2);
+      const double u = fma(ds, rc[9], rc[0]);
+      const int m = (int)fminf(fmaxf((float)u, 1.0f), umax);
+      const float f = __builtin_amdgcn_fmed3f((float)(u - (double)m), -1.0f, 2.0f);
+      // H at the voxel: the cubic times exp(-j 2 pi thf f)
+      const float4 a = crow[2 * m], b = crow[2 * m + 1];
+      const float qr = fmaf(fmaf(fmaf(b.z, f, b.x), f, a.z), f, a.x);
+      const float qi = fmaf(fmaf(fmaf(b.w, f, b.y), f, a.w), f, a.y);
+      const float ph = thf * f;
+      const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+      const float hr = fmaf(qr, cs, qi * sn), hi = fmaf(qi, cs, -(qr * sn));
+      // position term: Im(conj(g) H) grad_p d
+      const float tt = fmaf(gr, hi, -(gi * hr));
+      const float ax = px - r.x, ay = py - r.y, az = pz - r.z;
+      if (mono) {
+        const float w1 = 2.0f * tt * y1;
+        gx = fmaf(w1, ax, gx);
+        gy = fmaf(w1, ay, gy);
+        gz = fmaf(w1, az, gz);
+      } else {
+        const float w1 = tt * y1, w2 = tt * y2;
+        gx = fmaf(w1, ax, fmaf(w2, qx - r.x, gx));
+        gy = fmaf(w1, ay, fmaf(w2, qy - r.y, gy));
+        gz = fmaf(w1, az, fmaf(w2, qz - r.z, gz));
+      }
+      // adjoint: g exp(+j 2 pi thf f), spread with the Lagrange weights
+      const float er = fmaf(gr, cs, -(gi * sn)), ei = fmaf(gr, sn, gi * cs);
+      const float fp1 = f + 1.0f, fm1 = f - 1.0f, fm2 = f - 2.0f;
+      const float ff1 = f * fm1;
+      float w[4];
+      w[0] = -(ff1 * fm2) * (1.0f / 6.0f);
+      w[1] = (fp1 * fm1 * fm2) * 0.5f;
+      w[2] = -(fp1 * f * fm2) * 0.5f;
+      w[3] = (fp1 * ff1) * (1.0f / 6.0f);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float zr = w[q] * tp.r[q], zi = w[q] * tp.i[q];
+        atomicAdd(&grid[m - 1 + q].x, fmaf(zr, er, -(zi * ei)));
+        atomicAdd(&grid[m - 1 + q].y, fmaf(zr, ei, zi * er));
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    gx += __shfl_xor(gx, o, 64);
+    gy += __shfl_xor(gy, o, 64);
+    gz += __shfl_xor(gz, o, 64);
+  }
+  __syncthreads();
+  if (live) {
+    float2 *__restrict__ out = gpart + ((size_t)c * A + i) * M;
+    for (int m = lane; m < M; m += 64) out[m] = grid[m];
+    if (lane == 0) {
+      float *__restrict__ o = ppart + ((size_t)c * A + i) * 3;
+      o[0] = gx;
+      o[1] = gy;
+      o[2] = gz;
+    }
+  }
+}
+
+// Fixed-order FP64 sum of the chunks' grids.
+__global__ __launch_bounds__(kBlock) void k_bwd_prof_merge(float2 *__restrict__ graw,
+                                                           const float2 *__restrict__ gpart,
+                                                           int chunks, int64_t AM) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= AM) return;
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < chunks; ++q) {
+    const float2 v = gpart[(size_t)q * AM + t];
+    a += v.x;
+    b += v.y;
+  }
+  graw[t] = make_float2((float)a, (float)b);
+}
+
+// dL/ds_ij = scale sum_m G_i[m] exp(+j 2 pi f_j (dlo_i + m h) / c): one lane
+// per sample j of row blockIdx.y; the row's grid values are wave-uniform
+// (scalar loads).  Horner in exp(+j 2 pi f_j h / c) from the top, re-anchored
+// every kDftK points by an FP64-reduced phasor.
+__global__ __launch_bounds__(kBlock) void k_bwd_prof_dft(
+    float2 *__restrict__ grad, const float2 *__restrict__ graw,
+    const double *__restrict__ dlo, double h, double f0c, double dfc, int M, int F,
+    double scale) {
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  const double fj = fma((double)min(j, F - 1), dfc, f0c);  // revolutions per metre
+  float rr, ri;
+  phasor_quarter(-4.0 * fj * h, rr, ri);  // exp(+j 2 pi f_j h / c)
+  const float2 *__restrict__ gi = graw + (size_t)i * M;
+  const double d0 = dlo[i];
+  float cr = 0.0f, ci = 0.0f;
+  for (int b0 = 0; b0 < M; b0 += kDftK) {
+    const int len = min(kDftK, M - b0);
+    const float2 top = gi[b0 + len - 1];
+    float hr = top.x, hi = top.y;
+    for (int t = len - 2; t >= 0; --t) {
+      const float2 s = gi[b0 + t];
+      const float tr = fmaf(-hi, ri, fmaf(hr, rr, s.x));
+      const float ti = fmaf(hi, rr, fmaf(hr, ri, s.y));
+      hr = tr;
+      hi = ti;
+    }
+    float ar, ai;
+    phasor_hw(-fj * fma((double)b0, h, d0), ar, ai);  // exp(+j 2 pi f_j d_b0 / c)
+    cr = fmaf(ar, hr, fmaf(-ai, hi, cr));
+    ci = fmaf(ar, hi, fmaf(ai, hr, ci));
+  }
+  if (j < F)
+    grad[(size_t)i * F + j] = make_float2((float)(scale * cr), (float)(scale * ci));
+}
+
+__global__ __launch_bounds__(kBlock) void k_bwd_prof_ant(float *__restrict__ grad,
+                                                         const float *__restrict__ ppart,
+                                                         int chunks, int A, double scale) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= 3 * A) return;
+  double s = 0.0;
+  for (int q = 0; q < chunks; ++q) s += ppart[(size_t)q * A * 3 + t];
+  grad[t] = (float)(scale * s);
+}
+
+struct BwdPWs {
+  double *dlo, *rowc;
+  float2 *tab;
+  float4 *coef;
+  float4 *rec4;
+  float *recg;
+  float2 *gpart, *graw;
+  float *ppart;
+};
+
+int64_t bwdp_pad(int64_t n) { return (n + kBwdPad - 1) / kBwdPad * kBwdPad; }
+
+// Most voxel chunks a launch may use: ~1 GiB of partial grids, at most 64.
+int bwdp_chunks_max(int A, int M) {
+  int64_t c = ((int64_t)1 << 30) / ((int64_t)A * M * 8);
+  if (c > 64) c = 64;
+  return c < 1 ? 1 : (int)c;
+}
+
+size_t bwdp_layout(int A, int M, int64_t n, BwdPWs *w, char *base) {
+  const int64_t np = bwdp_pad(n);
+  const int cm = bwdp_chunks_max(A, M);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *q = base ? base + off : nullptr;
+    off += align256(bytes);
+    return q;
+  };
+  char *d = take(sizeof(double) * (size_t)A);
+  char *rc = take(sizeof(double) * kRowC * (size_t)A);
+  char *t = take(sizeof(float2) * (size_t)A * M);
+  char *cf = take(sizeof(float4) * 2 * (size_t)A * M);
+  char *r4 = take(sizeof(float4) * (size_t)np);
+  char *rg = take(sizeof(float) * (size_t)np);
+  char *gp = take(sizeof(float2) * (size_t)cm * A * M);
+  char *gr = take(sizeof(float2) * (size_t)A * M);
+  char *pp = take(sizeof(float) * 3 * (size_t)cm * A);
+  if (w) {
+    w->dlo = (double *)d;
+    w->rowc = (double *)rc;
+    w->tab = (float2 *)t;
+    w->coef = (float4 *)cf;
+    w->rec4 = (float4 *)r4;
+    w->recg = (float *)rg;
+    w->gpart = (float2 *)gp;
+    w->graw = (float2 *)gr;
+    w->ppart = (float *)pp;
+  }
+  return off;
+}
+
 int env_int(const char *name, int dflt) {
   const char *s = getenv(name);
   return (s && *s) ? atoi(s) : dflt;
@@ -451,7 +728,7 @@ extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
   hipLaunchKernelGGL(k_prof_rows, dim3(mmn_blocks(A, kBlock)), dim3(kBlock), 0, st, w.dlo,
                      w.rowc, ant, rx_off, box, A, num_rx, is_ti_radar, g.h);
   hipLaunchKernelGGL(k_prof_table, dim3(mmn_blocks(M, kBlock), A), dim3(kBlock), 0, st, w.tab,
-                     (const float2 *)meas, w.dlo, g.h, df * inv_c, tc, M, F);
+                     (const float2 *)meas, w.dlo, g.h, df * inv_c, tc, M, F, 0, 0.0f, 0.0f);
   hipLaunchKernelGGL(k_prof_coef, dim3(mmn_blocks((int64_t)A * M, kBlock)), dim3(kBlock), 0,
                      st, w.coef, w.tab, w.dlo, g.h, fcc, M, (int64_t)A * M);
 
@@ -494,4 +771,127 @@ extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
                      (const float4 *)w.coef, (float)(fcc * g.h), M, A, num_rx, rps);
   if (nseg > 1) mmn_rs_reduce(st, (float2 *)img, (float2 *)moment, im, mo, n, nseg);
   return mmn_check_launch("mmn_bp_profile");
+}
+
+extern "C" size_t mmn_bp_profile_backward_workspace_size(int64_t n_vox, int num_rows,
+                                                         int samples, double f0, double df,
+                                                         const double *bbox) {
+  ProfGeom g;
+  if (n_vox <= 0 || num_rows <= 0 || samples <= 0 || !bbox ||
+      !prof_geom(samples, f0, df, bbox, &g) || g.M > kBwdMaxM)
+    return 0;
+  return bwdp_layout(num_rows, g.M, n_vox, nullptr, nullptr);
+}
+
+extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const float *g,
+                                       double scale, const float *x_locs, const float *y_locs,
+                                       const float *z_locs, const int32_t *vox_idx,
+                                       int64_t n_vox, const float *ant, const float *meas,
+                                       const float *rx_off, double f0, double df, int nx,
+                                       int ny, int nz, int num_rows, int num_rx, int samples,
+                                       int is_ti_radar, const double *bbox, void *workspace,
+                                       size_t workspace_bytes, void *stream) {
+  MMN_REQUIRE(g && x_locs && y_locs && z_locs && ant && meas && rx_off && bbox, MMN_ERR_NULL,
+              "mmn_bp_profile_backward: NULL pointer argument");
+  MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 && samples > 0,
+              MMN_ERR_ARG, "mmn_bp_profile_backward: sizes must be positive");
+  ProfGeom geo;
+  MMN_REQUIRE(prof_geom(samples, f0, df, bbox, &geo), MMN_ERR_ARG,
+              "mmn_bp_profile_backward: f0, df and the bounding box must be finite (box min "
+              "<= max), and the box's round-trip span at most %d table points", kMaxM);
+  MMN_REQUIRE(geo.M <= kBwdMaxM, MMN_ERR_ARG,
+              "mmn_bp_profile_backward: %d table points per row exceed the %d an LDS grid "
+              "holds (a smaller box or mmn_bp_backward)", geo.M, kBwdMaxM);
+  int64_t n = (int64_t)nx * ny * nz;
+  if (vox_idx) {
+    MMN_REQUIRE(n_vox >= 0, MMN_ERR_ARG, "mmn_bp_profile_backward: n_vox < 0");
+    n = n_vox;
+  }
+  const int A = num_rows, F = samples, M = geo.M;
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0 || (!grad_meas && !grad_ant)) {
+    hipError_t e = hipSuccess;
+    if (grad_meas) e = hipMemsetAsync(grad_meas, 0, sizeof(float2) * (size_t)A * F, st);
+    if (e == hipSuccess && grad_ant)
+      e = hipMemsetAsync(grad_ant, 0, sizeof(float) * 3 * (size_t)A, st);
+    if (e != hipSuccess) {
+      mmn_set_error("mmn_bp_profile_backward: %s", hipGetErrorString(e));
+      return (int)e;
+    }
+    return 0;
+  }
+  const size_t need = bwdp_layout(A, M, n, nullptr, nullptr);
+  MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
+              "mmn_bp_profile_backward: workspace of %zu bytes needed", need);
+  BwdPWs w;
+  bwdp_layout(A, M, n, &w, (char *)workspace);
+  Box box;
+  for (int k = 0; k < 6; ++k) box.b[k] = bbox[k];
+  const double inv_c = 1.0 / MMN_C0;
+  const double two_pi = 6.283185307179586476925286766559;
+  const double tc = 0.5 * (double)(F - 1);
+  const double fcc = (f0 + tc * df) * inv_c;
+  hipLaunchKernelGGL(k_prof_rows, dim3(mmn_blocks(A, kBlock)), dim3(kBlock), 0, st, w.dlo,
+                     w.rowc, ant, rx_off, box, A, num_rx, is_ti_radar, geo.h);
+  const int64_t np = bwdp_pad(n);
+  if (grad_ant) {
+    hipLaunchKernelGGL(k_prof_table, dim3(mmn_blocks(M, kBlock), A), dim3(kBlock), 0, st,
+                       w.tab, (const float2 *)meas, w.dlo, geo.h, df * inv_c, tc, M, F, 1,
+                       (float)(two_pi * f0 * inv_c), (float)(two_pi * df * inv_c));
+    hipLaunchKernelGGL(k_prof_coef, dim3(mmn_blocks((int64_t)A * M, kBlock)), dim3(kBlock), 0,
+                       st, w.coef, w.tab, w.dlo, geo.h, fcc, M, (int64_t)A * M);
+  } else {
+    // the position term is not wanted: a zero table makes it vanish
+    hipError_t e = hipMemsetAsync(w.coef, 0, sizeof(float4) * 2 * (size_t)A * M, st);
+    if (e != hipSuccess) {
+      mmn_set_error("mmn_bp_profile_backward: %s", hipGetErrorString(e));
+      return (int)e;
+    }
+  }
+  hipLaunchKernelGGL(k_bwdp_pack, dim3(mmn_blocks(np, kBlock)), dim3(kBlock), 0, st, w.rec4,
+                     w.recg, (const float2 *)g, x_locs, y_locs, z_locs, vox_idx, n, np, ny,
+                     nz);
+  // voxel chunks: enough workgroups for eight rounds of resident slots, at
+  // least 1024 voxels per chunk, at most the workspace's count
+  const size_t lds = sizeof(float2) * 4 * (size_t)M;
+  int per_cu = 0, dev = 0, cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd_prof, kBlock, lds) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipGetLastError();
+  const int64_t groups = (A + 3) / 4;
+  int64_t ch = (8 * (int64_t)per_cu * cus + groups - 1) / groups;
+  const int forced = env_int("MMN_PROFB_CHUNKS", 0);
+  if (forced > 0) ch = forced;
+  if (ch > (np + 1023) / 1024) ch = (np + 1023) / 1024;
+  if (ch > bwdp_chunks_max(A, M)) ch = bwdp_chunks_max(A, M);
+  if (ch < 1) ch = 1;
+  int64_t chunk = (np + ch - 1) / ch;
+  chunk = (chunk + kBwdPad - 1) / kBwdPad * kBwdPad;
+  ch = (np + chunk - 1) / chunk;
+  TapPhases tp;
+  const double thf = fcc * geo.h;
+  for (int k = 0; k < 4; ++k) {
+    double ph = thf * (double)(1 - k);
+    ph -= rint(ph);
+    tp.r[k] = (float)cos(two_pi * ph);
+    tp.i[k] = (float)sin(two_pi * ph);
+  }
+  hipLaunchKernelGGL(k_bwd_prof, dim3((unsigned)groups, (unsigned)ch), dim3(kBlock), lds, st,
+                     w.gpart, w.ppart, w.rec4, w.recg, np, chunk, ant, rx_off, w.rowc,
+                     (const float4 *)w.coef, (float)thf, tp, M, A, num_rx);
+  if (grad_meas) {
+    hipLaunchKernelGGL(k_bwd_prof_merge, dim3(mmn_blocks((int64_t)A * M, kBlock)),
+                       dim3(kBlock), 0, st, w.graw, w.gpart, (int)ch, (int64_t)A * M);
+    hipLaunchKernelGGL(k_bwd_prof_dft, dim3(mmn_blocks(F, kBlock), A), dim3(kBlock), 0, st,
+                       (float2 *)grad_meas, w.graw, w.dlo, geo.h, f0 * inv_c, df * inv_c, M, F,
+                       scale);
+  }
+  if (grad_ant)
+    hipLaunchKernelGGL(k_bwd_prof_ant, dim3(mmn_blocks(3 * A, kBlock)), dim3(kBlock), 0, st,
+                       grad_ant, w.ppart, (int)ch, A, scale);
+  return mmn_check_launch("mmn_bp_profile_backward");
 }

@@ -227,8 +227,15 @@ def sn_from_moment(img, moment, vox_idx):
 
 def backproject_backward(g, meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
                          scale=1.0, K=32, vox_idx=None, is_ti=False, need_meas=True,
-                         need_ant=True):
-    """Adjoint of ``scale * backproject``: (dL/dmeas complex64 (rows,F), dL/dant float32 (rows,3))."""
+                         need_ant=True, variant="rec", bbox=None):
+    """Adjoint of ``scale * backproject``: (dL/dmeas complex64 (rows,F), dL/dant float32 (rows,3)).
+
+    ``variant="rec"`` recomputes every (row, sample, voxel) phase
+    (mmn_bp_backward); ``variant="profile"`` is the exact gradient of the
+    range-profile forward (mmn_bp_profile_backward), ``bbox`` as there.
+    """
+    if variant not in ("rec", "profile"):
+        raise ValueError('variant must be "rec" or "profile"')
     rx_off, nvox_grid = _geometry(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
     n = vox_idx.numel() if vox_idx is not None else nvox_grid
     g = g.reshape(-1)
@@ -240,6 +247,25 @@ def backproject_backward(g, meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f
     gm = torch.empty((A, F), dtype=torch.complex64, device=dev) if need_meas else None
     ga = torch.empty((A, 3), dtype=torch.float32, device=dev) if need_ant else None
     lib = _lib.load()
+    if variant == "profile":
+        if bbox is None:
+            bbox = grid_bbox(x_locs, y_locs, z_locs)
+        if len(bbox) != 6:
+            raise ValueError("bbox must hold 6 values")
+        bb = (ctypes.c_double * 6)(*[float(v) for v in bbox])
+        ws_bytes = int(lib.mmn_bp_profile_backward_workspace_size(
+            max(n, 1), A, F, float(f0), float(df), ctypes.addressof(bb)))
+        if ws_bytes == 0:
+            raise ValueError("mmn_bp_profile_backward: bounding box not finite, or its "
+                             "range-profile table too long for the profile backward")
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        _lib.call("mmn_bp_profile_backward", _lib.ptr(gm), _lib.ptr(ga), g.data_ptr(),
+                  float(scale), x_locs.data_ptr(), y_locs.data_ptr(), z_locs.data_ptr(),
+                  _lib.ptr(vox_idx), n, ant.data_ptr(), meas.data_ptr(), rx_off.data_ptr(),
+                  float(f0), float(df), x_locs.numel(), y_locs.numel(), z_locs.numel(), A,
+                  rx_off.shape[0], F, int(bool(is_ti)), ctypes.addressof(bb), ws.data_ptr(),
+                  ws_bytes, _lib.stream_handle(dev))
+        return gm, ga
     ws_bytes = int(lib.mmn_bp_backward_workspace_size(A, F, n))
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     _lib.call("mmn_bp_backward", _lib.ptr(gm), _lib.ptr(ga), g.data_ptr(), float(scale),
@@ -259,34 +285,40 @@ class Backproject(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, meas, ant, x_locs, y_locs, z_locs, rx_off, f0, df, scale, K, vox_idx,
-                is_ti):
+                is_ti, variant, bbox):
+        if variant == "profile" and bbox is None:
+            bbox = grid_bbox(x_locs, y_locs, z_locs)
         img = backproject(meas, ant, x_locs, y_locs, z_locs, rx_off, f0=f0, df=df,
-                          variant="rec", K=K, vox_idx=vox_idx, is_ti=is_ti)
+                          variant=variant, K=K, vox_idx=vox_idx, is_ti=is_ti, bbox=bbox)
         ctx.save_for_backward(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
-        ctx.args = (f0, df, scale, 32 if K >= 32 else 16, is_ti)
+        ctx.args = (f0, df, scale, 32 if K >= 32 else 16, is_ti, variant, bbox)
         return img * scale
 
     @staticmethod
     def backward(ctx, grad):
         meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx = ctx.saved_tensors
-        f0, df, scale, K, is_ti = ctx.args
+        f0, df, scale, K, is_ti, variant, bbox = ctx.args
         need_m, need_a = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         if not (need_m or need_a):
-            return (None,) * 12
+            return (None,) * 14
         gm, ga = backproject_backward(grad.contiguous(), meas, ant, x_locs, y_locs, z_locs,
                                       rx_off, f0=f0, df=df, scale=scale, K=K,
                                       vox_idx=vox_idx, is_ti=is_ti, need_meas=need_m,
-                                      need_ant=need_a)
-        return gm, ga, None, None, None, None, None, None, None, None, None, None
+                                      need_ant=need_a, variant=variant, bbox=bbox)
+        return (gm, ga) + (None,) * 12
 
 
 def backproject_ad(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df, scale=1.0,
-                   K=64, vox_idx=None, is_ti=False):
-    """Differentiable ``scale * backproject`` (recurrence kernel forward)."""
+                   K=64, vox_idx=None, is_ti=False, variant="rec", bbox=None):
+    """Differentiable ``scale * backproject``: the recurrence forward and its
+    backward (``variant="rec"``), or the range-profile forward and its exact
+    gradient (``variant="profile"``)."""
+    if variant not in ("rec", "profile"):
+        raise ValueError('variant must be "rec" or "profile"')
     if rx_off is None:
         rx_off = torch.zeros((1, 3), dtype=torch.float32, device=meas.device)
     return Backproject.apply(meas, ant, x_locs, y_locs, z_locs, rx_off, f0, df, scale, K,
-                             vox_idx, is_ti)
+                             vox_idx, is_ti, variant, bbox)
 
 
 def rsdf_bfs(normals, labels, seeds, dx):
