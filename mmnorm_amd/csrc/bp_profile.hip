@@ -37,6 +37,7 @@
 // fetches fall on a few cache lines of the row's table.
 #include <math.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "mmn_internal.h"
 
@@ -410,268 +411,415 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
 }
 
 // --------------------------------------------------------------------------
-// Backward of the range-profile forward, for a real loss with g = dL/dI.
-//   dL/dp_i = scale sum_v Im(conj(g_v) H_i(d_iv)) grad_p d_iv,
-//   H_i(d)  = sum_j k_j s_ij exp(-j 2 pi f_j d / c)   (k_j = 2 pi f_j / c),
-// with H_i interpolated from its own profile table exactly as the forward
-// interpolates P_i; and the adjoint of the forward's interpolation (the
-// exact gradient of the profile forward): each voxel spreads
-//   w_k(f) exp(+j 2 pi thf (f + 1 - k)) g_v,   k = 0..3,
-// onto table points m - 1 + k of its row (w_k the cubic's Lagrange weights
-// at nodes -1, 0, 1, 2, thf = f_c h / c the carrier's turns per interval),
-// and then
-//   dL/ds_ij = scale sum_m G_i[m] exp(+j 2 pi f_j (dlo_i + m h) / c).
-// k_bwd_prof gives each wave one row and a chunk of voxels: the row's table
-// points accumulate in the wave's own LDS grid (ds_add, no other wave
-// touches it) and the position term in lanes, so nothing crosses waves until
-// the per-(chunk, row) partials are reduced in a fixed order.
+// Backward of the range-profile forward, for a real loss with g = dL/dI:
+//   dL/ds_ij = scale sum_v g_v exp(+j k_j d_iv),
+//   dL/dp_i  = scale sum_v Im(conj(g_v) H_i(d_iv)) grad_p d_iv,
+//   H_i(d)   = sum_j k_j s_ij exp(-j k_j d),   k_j = 2 pi f_j / c.
+// The voxels are taken in groups (bricks of the grid, or runs of a voxel
+// list) of centre c_b and radius r_b.  For row i and group b let d0 be the
+// round trip to c_b and delta = d - d0, so |delta| <= rho = 2 max_b r_b, and
+// xi = delta / rho in [-1, 1].  With k_j = k_c + x_j, x_j = 2 pi (j - t_c)
+// df / c, the slow factor exp(-+j x_j delta) of each phasor is a power
+// series in xi with argument y_j xi, |y_j| = |x_j| rho <= a:
+//   H_i(d0 + delta) = exp(-j k_c delta) sum_k h_k xi^k,
+//   h_k = sum_j k_j s_ij exp(-j k_j d0) (-j y_j)^k / k!,
+//   sum_{v in b} g_v exp(+j k_j d_iv)
+//       = exp(+j k_j d0) sum_k (j y_j)^k / k! mu_k,
+//   mu_k = sum_{v in b} g_v exp(+j k_c delta_v) xi_v^k.
+// Truncated after K = 16 terms with a <= 2 the remainder is below
+// 2^16/16! e^2 = 2.3e-8 of sum_j |.|; the host sizes the groups for that.
+// k_bwd_taylor gives each lane one row (64 rows per wave) and walks the
+// groups of one chunk: per (row, group) the K coefficients h_k from the
+// row's samples, per voxel (wave-uniform, through scalar loads) the FP64
+// distance, the carrier exp(+j k_c delta) (FP64 phase), the position term
+// and the K moments, and per (row, group) the K moments expanded into the
+// row's F sample gradients.  Nothing is shared between lanes: no atomics, no
+// cross-lane sums; the per-chunk partials are reduced in a fixed order.
 // --------------------------------------------------------------------------
-constexpr int kBwdMaxM = 2048;  // LDS grid per wave: M float2 (4 waves: 64 KiB)
-constexpr int kBwdPad = 128;    // voxel records padded to 2 per lane
-constexpr int kDftK = 32;       // grid points per FP64 re-anchor in k_bwd_prof_dft
+constexpr int kTaylorK = 16;
+constexpr double kTaylorA = 2.0;  // bound on |y_j xi| the groups are sized for
+constexpr int kTJ = 32;           // samples per FP64 re-anchor of exp(-+j k_j d0)
 
-// One leg as leg64, also returning the FP32 inverse length.
-__device__ __forceinline__ double leg64y(double vx, double vy, double vz, double vv,
-                                         const double *__restrict__ c, float &inv) {
-  const double s = fma(vz, c[2], fma(vy, c[1], fma(vx, c[0], vv + c[-1])));
-  const float y = __builtin_amdgcn_rsqf((float)s);
-  inv = y;
-  const double yd = (double)y;
-  const double d0 = s * yd;
-  return fma(0.5 * yd, fma(-d0, d0, s), d0);
-}
-
-__global__ __launch_bounds__(kBlock) void k_bwdp_pack(
-    float4 *__restrict__ rec4, float *__restrict__ recg, const float2 *__restrict__ g,
-    const float *__restrict__ xl, const float *__restrict__ yl,
-    const float *__restrict__ zl, const int32_t *__restrict__ vidx, int64_t n,
-    int64_t npad, int ny, int nz) {
+__global__ __launch_bounds__(kBlock) void k_bt_pack(
+    double4 *__restrict__ recd, float4 *__restrict__ recf, float2 *__restrict__ recg,
+    const float2 *__restrict__ g, const float *__restrict__ xl, const float *__restrict__ yl,
+    const float *__restrict__ zl, const int32_t *__restrict__ vidx, int64_t n, int nx,
+    int ny, int nz, int ex, int ey, int ez, int64_t G, int64_t total) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= npad) return;
-  const bool real = t < n;
-  const int64_t src = real ? t : n - 1;
-  const int64_t lin = vidx ? (int64_t)vidx[src] : src;
+  if (t >= total) return;
   float vx, vy, vz;
-  voxel_coords(xl, yl, zl, ny, nz, lin, vx, vy, vz);
-  const float2 gv = real ? g[t] : make_float2(0.0f, 0.0f);
-  rec4[t] = make_float4(vx, vy, vz, gv.x);
-  recg[t] = gv.y;
+  float2 gv = make_float2(0.0f, 0.0f);
+  if (vidx) {
+    const bool real = t < n;
+    const int64_t src = real ? t : n - 1;
+    voxel_coords(xl, yl, zl, ny, nz, vidx[src], vx, vy, vz);
+    if (real) gv = g[t];
+  } else {
+    // group b = (bx, by, bz), bz fastest; offset o = (ox, oy, oz), oz fastest
+    const int64_t b = t / G, o = t % G;
+    const int nbz = (nz + ez - 1) / ez, nby = (ny + ey - 1) / ey;
+    const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), bx = (int)(b / ((int64_t)nbz * nby));
+    const int oz = (int)(o % ez), oy = (int)((o / ez) % ey), ox = (int)(o / ((int64_t)ez * ey));
+    const int x = bx * ex + ox, y = by * ey + oy, z = bz * ez + oz;
+    const bool real = x < nx && y < ny && z < nz;
+    vx = xl[min(x, nx - 1)];
+    vy = yl[min(y, ny - 1)];
+    vz = zl[min(z, nz - 1)];
+    if (real) gv = g[((int64_t)x * ny + y) * nz + z];
+  }
+  const double dx = vx, dy = vy, dz = vz;
+  recd[t] = make_double4(dx, dy, dz, fma(dz, dz, fma(dy, dy, dx * dx)));
+  recf[t] = make_float4(vx, vy, vz, 0.0f);
+  recg[t] = gv;
 }
 
-struct TapPhases {
-  float r[4], i[4];  // exp(+j 2 pi thf (1 - k)), k = 0..3
-};
+// One workgroup per group: the midpoint of its box and its radius about it
+// (FP64); the largest radius also goes to *rmax (as the bits of a positive
+// double, ordered like the values).
+__global__ __launch_bounds__(kBlock) void k_bt_groups(double4 *__restrict__ groupc,
+                                                      unsigned long long *__restrict__ rmax,
+                                                      const double4 *__restrict__ recd,
+                                                      int64_t G) {
+  __shared__ double red[6][kBlock / 64];
+  const int64_t b = blockIdx.x;
+  const double4 *__restrict__ r = recd + b * G;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t t = threadIdx.x; t < G; t += kBlock) {
+    const double4 v = r[t];
+    lo[0] = fmin(lo[0], v.x), hi[0] = fmax(hi[0], v.x);
+    lo[1] = fmin(lo[1], v.y), hi[1] = fmax(hi[1], v.y);
+    lo[2] = fmin(lo[2], v.z), hi[2] = fmax(hi[2], v.z);
+  }
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  for (int k = 0; k < 3; ++k) {
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, 64));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, 64));
+    }
+    if (lane == 0) {
+      red[k][w] = lo[k];
+      red[3 + k][w] = hi[k];
+    }
+  }
+  __syncthreads();
+  double c[3];
+  for (int k = 0; k < 3; ++k) {
+    double l = red[k][0], h = red[3 + k][0];
+    for (int q = 1; q < kBlock / 64; ++q) l = fmin(l, red[k][q]), h = fmax(h, red[3 + k][q]);
+    c[k] = 0.5 * (l + h);
+  }
+  __syncthreads();
+  double rr = 0.0;
+  for (int64_t t = threadIdx.x; t < G; t += kBlock) {
+    const double4 v = r[t];
+    const double ex = v.x - c[0], ey = v.y - c[1], ez = v.z - c[2];
+    rr = fmax(rr, sqrt(fma(ez, ez, fma(ey, ey, ex * ex))));
+  }
+  for (int o = 32; o > 0; o >>= 1) rr = fmax(rr, __shfl_xor(rr, o, 64));
+  if (lane == 0) red[0][w] = rr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kBlock / 64; ++q) rr = fmax(rr, red[0][q]);
+    groupc[b] = make_double4(c[0], c[1], c[2], rr);
+    atomicMax(rmax, (unsigned long long)__double_as_longlong(rr));
+  }
+}
 
-__global__ __launch_bounds__(kBlock) void k_bwd_prof(
-    float2 *__restrict__ gpart, float *__restrict__ ppart, const float4 *__restrict__ rec4,
-    const float *__restrict__ recg, int64_t npad, int64_t chunk,
+// k_j s_ij, transposed to [j][i] so that a wave's 64 rows read one sample
+// with one coalesced load.
+__global__ __launch_bounds__(kBlock) void k_bt_meas(float2 *__restrict__ mt,
+                                                    const float2 *__restrict__ meas, int A,
+                                                    int F, float k0, float dk) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)A * F) return;
+  const int j = (int)(t / A), i = (int)(t % A);
+  const float2 s = meas[(size_t)i * F + j];
+  const float kj = fmaf((float)j, dk, k0);
+  mt[t] = make_float2(kj * s.x, kj * s.y);
+}
+
+// Lane = row.  BI: some row has a nonzero RX offset (then both legs are
+// evaluated for every row; a monostatic row's second leg is its first).
+template <int K, bool BI>
+__global__ __launch_bounds__(kBlock) void k_bwd_taylor(
+    float2 *__restrict__ st, float *__restrict__ pp, const double4 *__restrict__ recd,
+    const float4 *__restrict__ recf, const float2 *__restrict__ recg,
+    const double4 *__restrict__ groupc, int64_t G, int ngroups, int gpc,
     const float *__restrict__ ant, const float *__restrict__ rxo,
-    const double *__restrict__ rowc, const float4 *__restrict__ coefh, float thf,
-    TapPhases tp, int M, int A, int R) {
-  extern __shared__ float2 lds_grid[];  // [4][M]
-  const int lane = threadIdx.x % 64;
+    const float2 *__restrict__ mt, double f0c, double dfc, double tc, double rho, int A,
+    int R, int F, int is_ti) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-  const int i_raw = blockIdx.x * 4 + wv;
+  const int i_raw = (blockIdx.x * 4 + wv) * 64 + (int)(threadIdx.x % 64);
   const bool live = i_raw < A;
   const int i = live ? i_raw : A - 1;
   const int c = blockIdx.y;
-  float2 *__restrict__ grid = lds_grid + (size_t)wv * M;
-  for (int m = lane; m < M; m += 64) grid[m] = make_float2(0.0f, 0.0f);
-  __syncthreads();
+  const int g_lo = c * gpc, g_hi = min(ngroups, g_lo + gpc);
 
   const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
   const int rx = i % R;
-  const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
-  const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
-  const float qx = px + ox, qy = py + oy, qz = pz + oz;
-  const double *__restrict__ rc = rowc + (size_t)kRowC * i;
-  const float4 *__restrict__ crow = coefh + (size_t)i * M * 2;
-  const float umax = (float)(M - 3);
+  const float qx = px + rxo[3 * rx + 0], qy = py + rxo[3 * rx + 1], qz = pz + rxo[3 * rx + 2];
+  const double pdx = px, pdy = py, pdz = pz, qdx = qx, qdy = qy, qdz = qz;
+  const double p2 = fma(pdz, pdz, fma(pdy, pdy, pdx * pdx));
+  const double q2 = fma(qdz, qdz, fma(qdy, qdy, qdx * qdx));
+  const double ti = is_ti ? 0.15 : 0.0;
+  const double fcc = fma(tc, dfc, f0c);            // f_c / c, turns per metre
+  const double inv_rho = 1.0 / rho;
+  const float ystep = (float)(6.283185307179586 * dfc * rho);  // y_{j+1} - y_j
+  const float y0 = (float)(-tc * 6.283185307179586 * dfc * rho);
   float gx = 0.0f, gy = 0.0f, gz = 0.0f;
 
-  // chunk and npad are multiples of kBwdPad, so vb + 64 is a record too
-  const int64_t v0 = (int64_t)c * chunk;
-  const int64_t v1 = min(npad, v0 + chunk);
-  for (int64_t vb = v0 + lane; vb < v1; vb += kBwdPad) {
+  for (int b = g_lo; b < g_hi; ++b) {
+    const double4 cen = groupc[b];
+    // d0: the row's round trip to the group centre
+    const double e1x = cen.x - pdx, e1y = cen.y - pdy, e1z = cen.z - pdz;
+    double d0 = sqrt(fma(e1z, e1z, fma(e1y, e1y, e1x * e1x)));
+    if (BI) {
+      const double e2x = cen.x - qdx, e2y = cen.y - qdy, e2z = cen.z - qdz;
+      d0 += sqrt(fma(e2z, e2z, fma(e2y, e2y, e2x * e2x)));
+    } else {
+      d0 += d0;
+    }
+    d0 += ti;
+    float sr, si;  // exp(-j k_j d0) steps by exp(-j 2 pi df d0 / c)
+    phasor_quarter(4.0 * d0 * dfc, sr, si);
+
+    // h_k = sum_j k_j s_ij exp(-j k_j d0) (-j y_j)^k / k!
+    float hr[K], hi[K];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const float4 r = rec4[vb + 64 * k];
-      const float gr = r.w, gi = recg[vb + 64 * k];
-      const double dx = r.x, dy = r.y, dz = r.z;
-      const double vv = fma(dz, dz, fma(dy, dy, dx * dx));
-      float y1, y2 = 0.0f;
-      double ds = leg64y(dx, dy, dz, vv, rc + 2, y1);
-      if (!mono) ds += leg64y(dx, dy, dz, vv, rc + 6, y2);
-      const double u = fma(ds, rc[9], rc[0]);
-      const int m = (int)fminf(fmaxf((float)u, 1.0f), umax);
-      const float f = __builtin_amdgcn_fmed3f((float)(u - (double)m), -1.0f, 2.0f);
-      // H at the voxel: the cubic times exp(-j 2 pi thf f)
-      const float4 a = crow[2 * m], b = crow[2 * m + 1];
-      const float qr = fmaf(fmaf(fmaf(b.z, f, b.x), f, a.z), f, a.x);
-      const float qi = fmaf(fmaf(fmaf(b.w, f, b.y), f, a.w), f, a.y);
-      const float ph = thf * f;
-      const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
-      const float hr = fmaf(qr, cs, qi * sn), hi = fmaf(qi, cs, -(qr * sn));
-      // position term: Im(conj(g) H) grad_p d
-      const float tt = fmaf(gr, hi, -(gi * hr));
-      const float ax = px - r.x, ay = py - r.y, az = pz - r.z;
-      if (mono) {
-        const float w1 = 2.0f * tt * y1;
-        gx = fmaf(w1, ax, gx);
-        gy = fmaf(w1, ay, gy);
-        gz = fmaf(w1, az, gz);
+    for (int k = 0; k < K; ++k) hr[k] = hi[k] = 0.0f;
+    for (int j0 = 0; j0 < F; j0 += kTJ) {
+      float er, ei;
+      phasor_hw(d0 * fma((double)j0, dfc, f0c), er, ei);
+      const int j1 = min(F, j0 + kTJ);
+      for (int j = j0; j < j1; ++j) {
+        const float2 a = mt[(size_t)j * A + i];
+        float tr = fmaf(a.x, er, -(a.y * ei)), tq = fmaf(a.x, ei, a.y * er);
+        const float y = fmaf((float)j, ystep, y0);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          hr[k] += tr;
+          hi[k] += tq;
+          // times -j y / (k + 1)
+          const float w = y * (1.0f / (float)(k + 1));
+          const float nr = tq * w, nq = -(tr * w);
+          tr = nr;
+          tq = nq;
+        }
+        const float nr = fmaf(er, sr, -(ei * si)), ni = fmaf(er, si, ei * sr);
+        er = nr;
+        ei = ni;
+      }
+    }
+
+    float mr[K], mi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) mr[k] = mi[k] = 0.0f;
+    const int64_t v0 = (int64_t)b * G, v1 = v0 + G;
+    for (int64_t v = v0; v < v1; ++v) {
+      const double4 rd = recd[v];
+      const float4 rf = recf[v];
+      const float2 gv = recg[v];
+      const double s1 = fma(rd.z, -2.0 * pdz, fma(rd.y, -2.0 * pdy, fma(rd.x, -2.0 * pdx, rd.w + p2)));
+      const float y1 = __builtin_amdgcn_rsqf((float)s1);
+      const double y1d = y1, a1 = s1 * y1d;
+      double d = fma(0.5 * y1d, fma(-a1, a1, s1), a1);
+      float y2 = y1;
+      if (BI) {
+        const double s2 =
+            fma(rd.z, -2.0 * qdz, fma(rd.y, -2.0 * qdy, fma(rd.x, -2.0 * qdx, rd.w + q2)));
+        y2 = __builtin_amdgcn_rsqf((float)s2);
+        const double y2d = y2, a2 = s2 * y2d;
+        d += fma(0.5 * y2d, fma(-a2, a2, s2), a2);
       } else {
-        const float w1 = tt * y1, w2 = tt * y2;
-        gx = fmaf(w1, ax, fmaf(w2, qx - r.x, gx));
-        gy = fmaf(w1, ay, fmaf(w2, qy - r.y, gy));
-        gz = fmaf(w1, az, fmaf(w2, qz - r.z, gz));
+        d += d;
       }
-      // adjoint: g exp(+j 2 pi thf f), spread with the Lagrange weights
-      const float er = fmaf(gr, cs, -(gi * sn)), ei = fmaf(gr, sn, gi * cs);
-      const float fp1 = f + 1.0f, fm1 = f - 1.0f, fm2 = f - 2.0f;
-      const float ff1 = f * fm1;
-      float w[4];
-      w[0] = -(ff1 * fm2) * (1.0f / 6.0f);
-      w[1] = (fp1 * fm1 * fm2) * 0.5f;
-      w[2] = -(fp1 * f * fm2) * 0.5f;
-      w[3] = (fp1 * ff1) * (1.0f / 6.0f);
+      const double delta = d + ti - d0;
+      const float xi = (float)(delta * inv_rho);
+      const float ph = (float)__builtin_amdgcn_fract(delta * fcc);
+      const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+      // c = g exp(+j k_c delta)
+      const float cr = fmaf(gv.x, cs, -(gv.y * sn)), ci = fmaf(gv.x, sn, gv.y * cs);
+      // position term: Im(conj(g) H) = Im(conj(c) sum_k h_k xi^k)
+      float Pr = hr[K - 1], Pi = hi[K - 1];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float zr = w[q] * tp.r[q], zi = w[q] * tp.i[q];
-        atomicAdd(&grid[m - 1 + q].x, fmaf(zr, er, -(zi * ei)));
-        atomicAdd(&grid[m - 1 + q].y, fmaf(zr, ei, zi * er));
+      for (int k = K - 2; k >= 0; --k) {
+        Pr = fmaf(Pr, xi, hr[k]);
+        Pi = fmaf(Pi, xi, hi[k]);
+      }
+      const float tt = fmaf(cr, Pi, -(ci * Pr));
+      const float w1 = tt * y1, w2 = tt * y2;
+      gx = fmaf(w1, px - rf.x, fmaf(w2, qx - rf.x, gx));
+      gy = fmaf(w1, py - rf.y, fmaf(w2, qy - rf.y, gy));
+      gz = fmaf(w1, pz - rf.z, fmaf(w2, qz - rf.z, gz));
+      // adjoint moments mu_k += c xi^k
+      float pw = 1.0f;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        mr[k] = fmaf(cr, pw, mr[k]);
+        mi[k] = fmaf(ci, pw, mi[k]);
+        pw *= xi;
+      }
+    }
+
+    // S_ij += exp(+j k_j d0) sum_k (j y_j)^k / k! mu_k  (conjugate steps)
+    float2 *__restrict__ srow = st + (size_t)c * F * A + i;
+    for (int j0 = 0; j0 < F; j0 += kTJ) {
+      float er, ei;
+      phasor_hw(d0 * fma((double)j0, dfc, f0c), er, ei);
+      ei = -ei;
+      const int j1 = min(F, j0 + kTJ);
+      for (int j = j0; j < j1; ++j) {
+        const float y = fmaf((float)j, ystep, y0);
+        float tr = mr[K - 1], tq = mi[K - 1];
+#pragma unroll
+        for (int k = K - 1; k >= 1; --k) {
+          // t = mu_{k-1} + (j y / k) t
+          const float w = y * (1.0f / (float)k);
+          const float nr = fmaf(-tq, w, mr[k - 1]), nq = fmaf(tr, w, mi[k - 1]);
+          tr = nr;
+          tq = nq;
+        }
+        const float vr = fmaf(er, tr, -(ei * tq)), vq = fmaf(er, tq, ei * tr);
+        if (live) {
+          float2 *__restrict__ o = srow + (size_t)j * A;
+          if (b == g_lo) {
+            *o = make_float2(vr, vq);
+          } else {
+            const float2 old = *o;
+            *o = make_float2(old.x + vr, old.y + vq);
+          }
+        }
+        const float nr = fmaf(er, sr, ei * si), ni = fmaf(ei, sr, -(er * si));
+        er = nr;
+        ei = ni;
       }
     }
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    gx += __shfl_xor(gx, o, 64);
-    gy += __shfl_xor(gy, o, 64);
-    gz += __shfl_xor(gz, o, 64);
-  }
-  __syncthreads();
   if (live) {
-    float2 *__restrict__ out = gpart + ((size_t)c * A + i) * M;
-    for (int m = lane; m < M; m += 64) out[m] = grid[m];
-    if (lane == 0) {
-      float *__restrict__ o = ppart + ((size_t)c * A + i) * 3;
-      o[0] = gx;
-      o[1] = gy;
-      o[2] = gz;
-    }
+    float *__restrict__ o = pp + ((size_t)c * A + i) * 3;
+    o[0] = gx;
+    o[1] = gy;
+    o[2] = gz;
   }
 }
 
-// Fixed-order FP64 sum of the chunks' grids.
-__global__ __launch_bounds__(kBlock) void k_bwd_prof_merge(float2 *__restrict__ graw,
-                                                           const float2 *__restrict__ gpart,
-                                                           int chunks, int64_t AM) {
+// grad_meas[i][j] = scale sum_c st[c][j][i] (FP64, fixed order).
+__global__ __launch_bounds__(kBlock) void k_bt_reduce_meas(float2 *__restrict__ grad,
+                                                           const float2 *__restrict__ st,
+                                                           int chunks, int A, int F,
+                                                           double scale) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= AM) return;
+  const int64_t AF = (int64_t)A * F;
+  if (t >= AF) return;
+  const int i = (int)(t / F), j = (int)(t % F);
   double a = 0.0, b = 0.0;
   for (int q = 0; q < chunks; ++q) {
-    const float2 v = gpart[(size_t)q * AM + t];
+    const float2 v = st[(size_t)q * AF + (size_t)j * A + i];
     a += v.x;
     b += v.y;
   }
-  graw[t] = make_float2((float)a, (float)b);
+  grad[t] = make_float2((float)(scale * a), (float)(scale * b));
 }
 
-// dL/ds_ij = scale sum_m G_i[m] exp(+j 2 pi f_j (dlo_i + m h) / c): one lane
-// per sample j of row blockIdx.y; the row's grid values are wave-uniform
-// (scalar loads).  Horner in exp(+j 2 pi f_j h / c) from the top, re-anchored
-// every kDftK points by an FP64-reduced phasor.
-__global__ __launch_bounds__(kBlock) void k_bwd_prof_dft(
-    float2 *__restrict__ grad, const float2 *__restrict__ graw,
-    const double *__restrict__ dlo, double h, double f0c, double dfc, int M, int F,
-    double scale) {
-  const int i = blockIdx.y;
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  const double fj = fma((double)min(j, F - 1), dfc, f0c);  // revolutions per metre
-  float rr, ri;
-  phasor_quarter(-4.0 * fj * h, rr, ri);  // exp(+j 2 pi f_j h / c)
-  const float2 *__restrict__ gi = graw + (size_t)i * M;
-  const double d0 = dlo[i];
-  float cr = 0.0f, ci = 0.0f;
-  for (int b0 = 0; b0 < M; b0 += kDftK) {
-    const int len = min(kDftK, M - b0);
-    const float2 top = gi[b0 + len - 1];
-    float hr = top.x, hi = top.y;
-    for (int t = len - 2; t >= 0; --t) {
-      const float2 s = gi[b0 + t];
-      const float tr = fmaf(-hi, ri, fmaf(hr, rr, s.x));
-      const float ti = fmaf(hi, rr, fmaf(hr, ri, s.y));
-      hr = tr;
-      hi = ti;
-    }
-    float ar, ai;
-    phasor_hw(-fj * fma((double)b0, h, d0), ar, ai);  // exp(+j 2 pi f_j d_b0 / c)
-    cr = fmaf(ar, hr, fmaf(-ai, hi, cr));
-    ci = fmaf(ar, hi, fmaf(ai, hr, ci));
-  }
-  if (j < F)
-    grad[(size_t)i * F + j] = make_float2((float)(scale * cr), (float)(scale * ci));
-}
-
-__global__ __launch_bounds__(kBlock) void k_bwd_prof_ant(float *__restrict__ grad,
-                                                         const float *__restrict__ ppart,
-                                                         int chunks, int A, double scale) {
+__global__ __launch_bounds__(kBlock) void k_bt_reduce_ant(float *__restrict__ grad,
+                                                          const float *__restrict__ pp,
+                                                          int chunks, int A, double scale) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= 3 * A) return;
   double s = 0.0;
-  for (int q = 0; q < chunks; ++q) s += ppart[(size_t)q * A * 3 + t];
+  for (int q = 0; q < chunks; ++q) s += pp[(size_t)q * A * 3 + t];
   grad[t] = (float)(scale * s);
 }
 
-struct BwdPWs {
-  double *dlo, *rowc;
-  float2 *tab;
-  float4 *coef;
-  float4 *rec4;
-  float *recg;
-  float2 *gpart, *graw;
-  float *ppart;
+// Group shape: for the grid, bricks whose edges (in voxels) are balanced
+// over each axis and at most emax; for a list, runs of G entries.
+struct BtShape {
+  int ex, ey, ez;
+  int64_t G, ngroups, total;
 };
 
-int64_t bwdp_pad(int64_t n) { return (n + kBwdPad - 1) / kBwdPad * kBwdPad; }
+BtShape bt_shape(const int32_t *vidx, int64_t n, int nx, int ny, int nz, int emax) {
+  BtShape s;
+  if (vidx) {
+    s.ex = s.ey = s.ez = 0;
+    s.G = (int64_t)emax * emax * emax;
+    if (s.G > n) s.G = n;
+    s.ngroups = (n + s.G - 1) / s.G;
+  } else {
+    auto edge = [&](int nax) {
+      const int nb = (nax + emax - 1) / emax;
+      return (nax + nb - 1) / nb;
+    };
+    s.ex = edge(nx);
+    s.ey = edge(ny);
+    s.ez = edge(nz);
+    s.G = (int64_t)s.ex * s.ey * s.ez;
+    s.ngroups = (int64_t)((nx + s.ex - 1) / s.ex) * ((ny + s.ey - 1) / s.ey) *
+                ((nz + s.ez - 1) / s.ez);
+  }
+  s.total = s.G * s.ngroups;
+  return s;
+}
 
-// Most voxel chunks a launch may use: ~1 GiB of partial grids, at most 64.
-int bwdp_chunks_max(int A, int M) {
-  int64_t c = ((int64_t)1 << 30) / ((int64_t)A * M * 8);
-  if (c > 64) c = 64;
+struct BtWs {
+  double4 *recd, *groupc;
+  float4 *recf;
+  float2 *recg, *mt, *st;
+  float *pp;
+  unsigned long long *rmax;
+};
+
+// Most chunks (partial copies of the sample gradients) a launch may use.
+int bt_chunks_max(int A, int F) {
+  int64_t c = ((int64_t)1 << 30) / ((int64_t)A * F * 8);
+  if (c > 256) c = 256;
   return c < 1 ? 1 : (int)c;
 }
 
-size_t bwdp_layout(int A, int M, int64_t n, BwdPWs *w, char *base) {
-  const int64_t np = bwdp_pad(n);
-  const int cm = bwdp_chunks_max(A, M);
+size_t bt_layout(int A, int F, int64_t total, int64_t ngroups, BtWs *w, char *base) {
+  const int cm = bt_chunks_max(A, F);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     char *q = base ? base + off : nullptr;
     off += align256(bytes);
     return q;
   };
-  char *d = take(sizeof(double) * (size_t)A);
-  char *rc = take(sizeof(double) * kRowC * (size_t)A);
-  char *t = take(sizeof(float2) * (size_t)A * M);
-  char *cf = take(sizeof(float4) * 2 * (size_t)A * M);
-  char *r4 = take(sizeof(float4) * (size_t)np);
-  char *rg = take(sizeof(float) * (size_t)np);
-  char *gp = take(sizeof(float2) * (size_t)cm * A * M);
-  char *gr = take(sizeof(float2) * (size_t)A * M);
-  char *pp = take(sizeof(float) * 3 * (size_t)cm * A);
+  char *rd = take(sizeof(double4) * (size_t)total);
+  char *rf = take(sizeof(float4) * (size_t)total);
+  char *rg = take(sizeof(float2) * (size_t)total);
+  char *gc = take(sizeof(double4) * (size_t)ngroups);
+  char *m = take(sizeof(float2) * (size_t)A * F);
+  char *s = take(sizeof(float2) * (size_t)cm * A * F);
+  char *p = take(sizeof(float) * 3 * (size_t)cm * A);
+  char *r = take(sizeof(unsigned long long));
   if (w) {
-    w->dlo = (double *)d;
-    w->rowc = (double *)rc;
-    w->tab = (float2 *)t;
-    w->coef = (float4 *)cf;
-    w->rec4 = (float4 *)r4;
-    w->recg = (float *)rg;
-    w->gpart = (float2 *)gp;
-    w->graw = (float2 *)gr;
-    w->ppart = (float *)pp;
+    w->recd = (double4 *)rd;
+    w->recf = (float4 *)rf;
+    w->recg = (float2 *)rg;
+    w->groupc = (double4 *)gc;
+    w->mt = (float2 *)m;
+    w->st = (float2 *)s;
+    w->pp = (float *)p;
+    w->rmax = (unsigned long long *)r;
   }
   return off;
+}
+
+// Largest brick edge (voxels) whose radius keeps |y_j xi| <= kTaylorA, from
+// the axis pitches (the group radii are measured again on the device).
+int bt_emax(int F, double df, const double *bb, int nx, int ny, int nz) {
+  const double band = 0.5 * (double)(F - 1) * fabs(df);  // t_c |df|
+  if (band <= 0.0) return 64;
+  // rho = 2 r, a = 2 pi band rho / c  ->  r <= kTaylorA c / (4 pi band)
+  const double rb = kTaylorA * MMN_C0 / (4.0 * 3.141592653589793 * band);
+  const double px = nx > 1 ? (bb[3] - bb[0]) / (nx - 1) : 0.0;
+  const double py = ny > 1 ? (bb[4] - bb[1]) / (ny - 1) : 0.0;
+  const double pz = nz > 1 ? (bb[5] - bb[2]) / (nz - 1) : 0.0;
+  const double pm = fmax(px, fmax(py, pz));
+  if (pm <= 0.0) return 64;
+  int e = (int)floor(2.0 * 0.95 * rb / (sqrt(3.0) * pm)) + 1;
+  if (e > 64) e = 64;
+  return e < 1 ? 1 : e;
 }
 
 int env_int(const char *name, int dflt) {
@@ -776,11 +924,13 @@ extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
 extern "C" size_t mmn_bp_profile_backward_workspace_size(int64_t n_vox, int num_rows,
                                                          int samples, double f0, double df,
                                                          const double *bbox) {
-  ProfGeom g;
-  if (n_vox <= 0 || num_rows <= 0 || samples <= 0 || !bbox ||
-      !prof_geom(samples, f0, df, bbox, &g) || g.M > kBwdMaxM)
-    return 0;
-  return bwdp_layout(num_rows, g.M, n_vox, nullptr, nullptr);
+  (void)f0;
+  if (n_vox <= 0 || num_rows <= 0 || samples <= 0 || !bbox) return 0;
+  // the list shape at edge 1 has the most records (n) and groups (n); the
+  // grid shapes fit inside n * 8 records (balanced edges pad < 2x per axis)
+  const int64_t total = n_vox * 8, ngroups = n_vox;
+  (void)df;
+  return bt_layout(num_rows, samples, total, ngroups, nullptr, nullptr);
 }
 
 extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const float *g,
@@ -795,19 +945,15 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
               "mmn_bp_profile_backward: NULL pointer argument");
   MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && num_rows > 0 && num_rx > 0 && samples > 0,
               MMN_ERR_ARG, "mmn_bp_profile_backward: sizes must be positive");
-  ProfGeom geo;
-  MMN_REQUIRE(prof_geom(samples, f0, df, bbox, &geo), MMN_ERR_ARG,
-              "mmn_bp_profile_backward: f0, df and the bounding box must be finite (box min "
-              "<= max), and the box's round-trip span at most %d table points", kMaxM);
-  MMN_REQUIRE(geo.M <= kBwdMaxM, MMN_ERR_ARG,
-              "mmn_bp_profile_backward: %d table points per row exceed the %d an LDS grid "
-              "holds (a smaller box or mmn_bp_backward)", geo.M, kBwdMaxM);
+  MMN_REQUIRE(isfinite(f0) && isfinite(df) && isfinite(bbox[0] + bbox[1] + bbox[2] + bbox[3] +
+                                                       bbox[4] + bbox[5]),
+              MMN_ERR_ARG, "mmn_bp_profile_backward: f0, df and bbox must be finite");
   int64_t n = (int64_t)nx * ny * nz;
   if (vox_idx) {
     MMN_REQUIRE(n_vox >= 0, MMN_ERR_ARG, "mmn_bp_profile_backward: n_vox < 0");
     n = n_vox;
   }
-  const int A = num_rows, F = samples, M = geo.M;
+  const int A = num_rows, F = samples;
   hipStream_t st = (hipStream_t)stream;
   if (n == 0 || (!grad_meas && !grad_ant)) {
     hipError_t e = hipSuccess;
@@ -820,78 +966,90 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
     }
     return 0;
   }
-  const size_t need = bwdp_layout(A, M, n, nullptr, nullptr);
+  const size_t need = mmn_bp_profile_backward_workspace_size(n, A, F, f0, df, bbox);
   MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
               "mmn_bp_profile_backward: workspace of %zu bytes needed", need);
-  BwdPWs w;
-  bwdp_layout(A, M, n, &w, (char *)workspace);
-  Box box;
-  for (int k = 0; k < 6; ++k) box.b[k] = bbox[k];
   const double inv_c = 1.0 / MMN_C0;
   const double two_pi = 6.283185307179586476925286766559;
   const double tc = 0.5 * (double)(F - 1);
-  const double fcc = (f0 + tc * df) * inv_c;
-  hipLaunchKernelGGL(k_prof_rows, dim3(mmn_blocks(A, kBlock)), dim3(kBlock), 0, st, w.dlo,
-                     w.rowc, ant, rx_off, box, A, num_rx, is_ti_radar, geo.h);
-  const int64_t np = bwdp_pad(n);
-  if (grad_ant) {
-    hipLaunchKernelGGL(k_prof_table, dim3(mmn_blocks(M, kBlock), A), dim3(kBlock), 0, st,
-                       w.tab, (const float2 *)meas, w.dlo, geo.h, df * inv_c, tc, M, F, 1,
-                       (float)(two_pi * f0 * inv_c), (float)(two_pi * df * inv_c));
-    hipLaunchKernelGGL(k_prof_coef, dim3(mmn_blocks((int64_t)A * M, kBlock)), dim3(kBlock), 0,
-                       st, w.coef, w.tab, w.dlo, geo.h, fcc, M, (int64_t)A * M);
-  } else {
-    // the position term is not wanted: a zero table makes it vanish
-    hipError_t e = hipMemsetAsync(w.coef, 0, sizeof(float4) * 2 * (size_t)A * M, st);
+  const double band = tc * fabs(df);  // t_c |df|
+  int emax = bt_emax(F, df, bbox, nx, ny, nz);
+  if (vox_idx) emax = env_int("MMN_PROFB_LIST_EDGE", 8);
+  // Pack the voxels in group order and measure the groups; halve the group
+  // edge until the largest radius satisfies the series bound (the pitch
+  // estimate above assumes evenly spaced axes).
+  BtShape sh;
+  BtWs w;
+  double rho = 0.0;
+  for (;;) {
+    sh = bt_shape(vox_idx, n, nx, ny, nz, emax);
+    bt_layout(A, F, sh.total, sh.ngroups, &w, (char *)workspace);
+    hipLaunchKernelGGL(k_bt_pack, dim3(mmn_blocks(sh.total, kBlock)), dim3(kBlock), 0, st,
+                       w.recd, w.recf, w.recg, (const float2 *)g, x_locs, y_locs, z_locs,
+                       vox_idx, n, nx, ny, nz, sh.ex, sh.ey, sh.ez, sh.G, sh.total);
+    hipError_t e = hipMemsetAsync(w.rmax, 0, sizeof(unsigned long long), st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_bt_groups, dim3((unsigned)sh.ngroups), dim3(kBlock), 0, st,
+                         w.groupc, w.rmax, w.recd, sh.G);
+      unsigned long long bits = 0;
+      e = hipMemcpyAsync(&bits, w.rmax, sizeof(bits), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      double r;
+      memcpy(&r, &bits, sizeof(r));
+      rho = 2.0 * r;
+    }
     if (e != hipSuccess) {
       mmn_set_error("mmn_bp_profile_backward: %s", hipGetErrorString(e));
       return (int)e;
     }
+    const double a = two_pi * band * rho * inv_c;
+    if (a <= kTaylorA * (1.0 + 1e-9) || emax == 1) break;
+    emax = emax / 2;
   }
-  hipLaunchKernelGGL(k_bwdp_pack, dim3(mmn_blocks(np, kBlock)), dim3(kBlock), 0, st, w.rec4,
-                     w.recg, (const float2 *)g, x_locs, y_locs, z_locs, vox_idx, n, np, ny,
-                     nz);
-  // voxel chunks: enough workgroups for eight rounds of resident slots, at
-  // least 1024 voxels per chunk, at most the workspace's count
-  const size_t lds = sizeof(float2) * 4 * (size_t)M;
-  int per_cu = 0, dev = 0, cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd_prof, kBlock, lds) !=
-          hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
+  MMN_REQUIRE(two_pi * band * rho * inv_c <= kTaylorA * (1.0 + 1e-9), MMN_ERR_ARG,
+              "mmn_bp_profile_backward: voxel spacing too coarse for the band (a single "
+              "voxel group exceeds the series bound)");
+  if (!(rho > 0.0)) rho = 1.0;  // single-voxel groups: xi = 0 throughout
+  hipLaunchKernelGGL(k_bt_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)), dim3(kBlock), 0, st,
+                     w.mt, (const float2 *)meas, A, F, (float)(two_pi * f0 * inv_c),
+                     (float)(two_pi * df * inv_c));
+  // chunks of groups: enough waves for ~16 per CU, at most the workspace's
+  bool bi = false;
+  {
+    // any RX offset nonzero -> both legs (host copy of a few floats)
+    float ro[3 * 64];
+    const int nr = num_rx < 64 ? num_rx : 64;
+    hipError_t e = hipMemcpyAsync(ro, rx_off, sizeof(float) * 3 * nr, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      mmn_set_error("mmn_bp_profile_backward: %s", hipGetErrorString(e));
+      return (int)e;
+    }
+    for (int k = 0; k < 3 * nr; ++k) bi |= ro[k] != 0.0f;
+    if (num_rx > 64) bi = true;
+  }
+  int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   (void)hipGetLastError();
-  const int64_t groups = (A + 3) / 4;
-  int64_t ch = (8 * (int64_t)per_cu * cus + groups - 1) / groups;
+  const int64_t rowblocks = (A + 255) / 256;
+  int64_t ch = (4 * (int64_t)cus + rowblocks - 1) / rowblocks;
   const int forced = env_int("MMN_PROFB_CHUNKS", 0);
   if (forced > 0) ch = forced;
-  if (ch > (np + 1023) / 1024) ch = (np + 1023) / 1024;
-  if (ch > bwdp_chunks_max(A, M)) ch = bwdp_chunks_max(A, M);
+  if (ch > sh.ngroups) ch = sh.ngroups;
+  if (ch > bt_chunks_max(A, F)) ch = bt_chunks_max(A, F);
   if (ch < 1) ch = 1;
-  int64_t chunk = (np + ch - 1) / ch;
-  chunk = (chunk + kBwdPad - 1) / kBwdPad * kBwdPad;
-  ch = (np + chunk - 1) / chunk;
-  TapPhases tp;
-  const double thf = fcc * geo.h;
-  for (int k = 0; k < 4; ++k) {
-    double ph = thf * (double)(1 - k);
-    ph -= rint(ph);
-    tp.r[k] = (float)cos(two_pi * ph);
-    tp.i[k] = (float)sin(two_pi * ph);
-  }
-  hipLaunchKernelGGL(k_bwd_prof, dim3((unsigned)groups, (unsigned)ch), dim3(kBlock), lds, st,
-                     w.gpart, w.ppart, w.rec4, w.recg, np, chunk, ant, rx_off, w.rowc,
-                     (const float4 *)w.coef, (float)thf, tp, M, A, num_rx);
-  if (grad_meas) {
-    hipLaunchKernelGGL(k_bwd_prof_merge, dim3(mmn_blocks((int64_t)A * M, kBlock)),
-                       dim3(kBlock), 0, st, w.graw, w.gpart, (int)ch, (int64_t)A * M);
-    hipLaunchKernelGGL(k_bwd_prof_dft, dim3(mmn_blocks(F, kBlock), A), dim3(kBlock), 0, st,
-                       (float2 *)grad_meas, w.graw, w.dlo, geo.h, f0 * inv_c, df * inv_c, M, F,
-                       scale);
-  }
+  const int gpc = (int)((sh.ngroups + ch - 1) / ch);
+  ch = (sh.ngroups + gpc - 1) / gpc;
+  auto kern = bi ? k_bwd_taylor<kTaylorK, true> : k_bwd_taylor<kTaylorK, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)rowblocks, (unsigned)ch), dim3(kBlock), 0, st, w.st,
+                     w.pp, w.recd, w.recf, w.recg, w.groupc, sh.G, (int)sh.ngroups, gpc, ant,
+                     rx_off, w.mt, f0 * inv_c, df * inv_c, tc, rho, A, num_rx, F, is_ti_radar);
+  if (grad_meas)
+    hipLaunchKernelGGL(k_bt_reduce_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)),
+                       dim3(kBlock), 0, st, (float2 *)grad_meas, w.st, (int)ch, A, F, scale);
   if (grad_ant)
-    hipLaunchKernelGGL(k_bwd_prof_ant, dim3(mmn_blocks(3 * A, kBlock)), dim3(kBlock), 0, st,
-                       grad_ant, w.ppart, (int)ch, A, scale);
+    hipLaunchKernelGGL(k_bt_reduce_ant, dim3(mmn_blocks(3 * A, kBlock)), dim3(kBlock), 0, st,
+                       grad_ant, w.pp, (int)ch, A, scale);
   return mmn_check_launch("mmn_bp_profile_backward");
 }
