@@ -16,6 +16,10 @@ in HBM.  Prints ONE JSON line:
              rocprofv3 --pmc summary when present
 * plain      the direct-sincos FP32 kernel on the same inputs (fewer steps),
              and plain_fast, the same with the hardware sine/cosine
+* profile    the same sum from per-row range-profile tables (mmn_bp_profile):
+             its time, its speed-up over the recurrence and its difference from
+             the recurrence's image and moment, and its grouped-series
+             backward (mmn_bp_profile_backward)
 * backward   one backward (dL/dmeas, dL/dant) of L = sum w|I|^2
 * rsdf_bfs   GPU 6-connected labelling and GPU BFS over the 5%-filtered vote
              field
@@ -205,6 +209,15 @@ def main(argv=None):
                               "kind": "same sum from per-row cubic range-profile tables "
                                       "(mmn_bp_profile); value = voxel*row*sample / time"}
             log(f"profile forward: {q_s * 1e3:.2f} ms, {k_avg / q_s:.1f}x the recurrence")
+            # its backward for L = sum w |I|^2 (I = img / A), both outputs
+            wq = torch.rand(V, device=dev, generator=torch.Generator(dev).manual_seed(1))
+            gq = (2.0 * wq * pimg.reshape(-1) / A / A).to(torch.complex64).contiguous()
+            qw, qk, _ = timed(lambda: ops.backproject_backward(
+                gq, meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, scale=1.0 / A,
+                variant="profile", bbox=bb), a.profile_steps, 1)
+            qb = float(np.mean(qk)) * 1e-3
+            rec["profile"].update({"backward_ms": qb * 1e3, "backward_over_forward": qb / q_s})
+            log(f"profile backward: {qb * 1e3:.2f} ms")
         rec["config4_backward"] = config4_stage(dev)
         log(f"config 4 forward/backward: {rec['config4_backward']}")
         rec["interpolated"] = interp_stage(sc, ant, xl, yl, zl, dev)
@@ -261,8 +274,15 @@ def config4_stage(dev):
     g = (2.0 * w * img.reshape(-1) / A / A).to(torch.complex64).contiguous()
     b_ms, _ = t_of(lambda: ops.backproject_backward(g, meas, ant, xl, yl, zl, f0=sc.f0,
                                                     df=sc.df, scale=1.0 / A, K=32))
+    bb = ops.grid_bbox(xl, yl, zl)
+    pf_ms, _ = t_of(lambda: ops.backproject(meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df,
+                                            variant="profile", bbox=bb))
+    pb_ms, _ = t_of(lambda: ops.backproject_backward(g, meas, ant, xl, yl, zl, f0=sc.f0,
+                                                     df=sc.df, scale=1.0 / A,
+                                                     variant="profile", bbox=bb))
     return {"forward_ms": f_ms, "backward_ms": b_ms, "over_forward": b_ms / f_ms,
-            "units": int(sc.cfg.work)}
+            "profile_forward_ms": pf_ms, "profile_backward_ms": pb_ms,
+            "profile_over_forward": pb_ms / pf_ms, "units": int(sc.cfg.work)}
 
 
 def interp_stage(sc, ant, xl, yl, zl, dev):

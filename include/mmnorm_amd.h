@@ -146,18 +146,23 @@ int mmn_bp_profile(float *img, float *moment, const float *x_locs, const float *
                    size_t workspace_bytes, void *stream);
 
 /*
- * mmn_bp_profile_backward -- the exact gradient of mmn_bp_profile for a real
- * loss, arguments as mmn_bp_backward plus bbox (as mmn_bp_profile):
+ * mmn_bp_profile_backward -- gradients of the full-frequency sum for a real
+ * loss (the same quantities as mmn_bp_backward, arguments as there plus bbox
+ * as in mmn_bp_profile), evaluated by groups of voxels instead of per sample:
  *     grad_ant[i]  = scale sum_v Im(conj(g_v) H_i(d_iv)) grad_p d_iv,
- *     H_i(d) = sum_j k_j meas[i][j] exp(-j k_j d), k_j = 2 pi (f0 + j df) / c,
- * interpolated from its own range-profile table; grad_meas the adjoint of
- * the profile interpolation (each voxel spreads conj(phasor) g_v onto its
- * four table points with the cubic's weights) followed by the table-to-
- * sample transform.  Either output may be NULL.  The table must fit an LDS
- * grid: M <= 2048 points per row (a bounding box up to ~1.2 m of round trip
- * at 4 GHz); larger boxes get MMN_ERR_ARG.  workspace is required:
- * mmn_bp_profile_backward_workspace_size() bytes.  Results are deterministic
- * for a given device.  No reference counterpart.
+ *     H_i(d)       = sum_j k_j meas[i][j] exp(-j k_j d), k_j = 2 pi (f0 + j df)/c,
+ *     grad_meas[i][j] = scale sum_v g_v exp(+j k_j d_iv).
+ * Voxels are grouped (bricks of the grid, runs of a voxel list) so that
+ * within a group d_iv = d0 + delta with |delta| small enough that the slow
+ * factor exp(-+j 2 pi (j - (F-1)/2) df delta / c) is a short series in delta
+ * (16 power-series terms for the position term, 12 Chebyshev terms for the
+ * adjoint; truncation below 3e-8 of the row's sum of magnitudes).  One lane
+ * per row; no atomics, deterministic.  Either output may be NULL.  bbox
+ * (host, x, y, z minimum then maximum) sizes the groups; the call reads two
+ * small values back to the host (the largest group radius, the RX offsets),
+ * so it synchronises the stream.  workspace is required:
+ * mmn_bp_profile_backward_workspace_size() bytes.  No reference
+ * counterpart.
  */
 size_t mmn_bp_profile_backward_workspace_size(int64_t n_vox, int num_rows, int samples,
                                               double f0, double df, const double *bbox);

@@ -424,10 +424,14 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
 //   H_i(d0 + delta) = exp(-j k_c delta) sum_k h_k xi^k,
 //   h_k = sum_j k_j s_ij exp(-j k_j d0) (-j y_j)^k / k!,
 //   sum_{v in b} g_v exp(+j k_j d_iv)
-//       = exp(+j k_j d0) sum_k (j y_j)^k / k! mu_k,
-//   mu_k = sum_{v in b} g_v exp(+j k_c delta_v) xi_v^k.
-// Truncated after K = 16 terms with a <= 2 the remainder is below
-// 2^16/16! e^2 = 2.3e-8 of sum_j |.|; the host sizes the groups for that.
+//       = exp(+j k_j d0) sum_k eps_k j^k J_k(y_j) mu_k,
+//   mu_k = sum_{v in b} g_v exp(+j k_c delta_v) T_k(xi_v).
+// The position term keeps K = 16 power-series terms (remainder below
+// 2^16/16! e^2 = 2.3e-8 of sum_j |.| for a <= 2); the adjoint uses the
+// Chebyshev form exp(+j y xi) = sum_k eps_k j^k J_k(y) T_k(xi) (Jacobi-Anger)
+// with 12 terms (remainder ~ 2 J_12(2) = 4e-9), its moments taken against
+// T_k(xi_v).  The host sizes the groups for a <= 2; the factors y_j^k / k! and
+// eps_k J_k(y_j) depend on j only and come from a table (scalar loads).
 // k_bwd_taylor gives each lane one row (64 rows per wave) and walks the
 // groups of one chunk: per (row, group) the K coefficients h_k from the
 // row's samples, per voxel (wave-uniform, through scalar loads) the FP64
@@ -436,9 +440,38 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
 // row's F sample gradients.  Nothing is shared between lanes: no atomics, no
 // cross-lane sums; the per-chunk partials are reduced in a fixed order.
 // --------------------------------------------------------------------------
-constexpr int kTaylorK = 16;
+constexpr int kTaylorK = 16;      // power-series terms of the position term
+constexpr int kChebK = 12;        // Chebyshev terms of the adjoint
 constexpr double kTaylorA = 2.0;  // bound on |y_j xi| the groups are sized for
 constexpr int kTJ = 32;           // samples per FP64 re-anchor of exp(-+j k_j d0)
+
+// Per sample j: y_j^k / k! (k < kTaylorK) and eps_k J_k(y_j) (k < kChebK,
+// eps_0 = 1, eps_k = 2), y_j = 2 pi (j - t_c) df rho / c, in FP64 from the
+// power series of J_k (|y| <= 2: 20 terms leave < 1e-20).
+__global__ __launch_bounds__(kBlock) void k_bt_tables(float *__restrict__ tay,
+                                                      float *__restrict__ bes, int F,
+                                                      double ystep, double tc) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= F) return;
+  const double y = ((double)j - tc) * ystep;
+  double t = 1.0;
+  for (int k = 0; k < kTaylorK; ++k) {
+    tay[(size_t)j * kTaylorK + k] = (float)t;
+    t *= y / (double)(k + 1);
+  }
+  const double h = 0.5 * y, h2 = h * h;
+  double hk = 1.0, kf = 1.0;  // (y/2)^k, k!
+  for (int k = 0; k < kChebK; ++k) {
+    double term = hk / kf, sum = 0.0;
+    for (int m = 0; m < 20; ++m) {
+      sum += term;
+      term *= -h2 / ((double)(m + 1) * (double)(m + 1 + k));
+    }
+    bes[(size_t)j * kChebK + k] = (float)((k ? 2.0 : 1.0) * sum);
+    hk *= h;
+    kf *= (double)(k + 1);
+  }
+}
 
 __global__ __launch_bounds__(kBlock) void k_bt_pack(
     double4 *__restrict__ recd, float4 *__restrict__ recf, float2 *__restrict__ recg,
@@ -540,13 +573,14 @@ __global__ __launch_bounds__(kBlock) void k_bt_meas(float2 *__restrict__ mt,
 
 // Lane = row.  BI: some row has a nonzero RX offset (then both legs are
 // evaluated for every row; a monostatic row's second leg is its first).
-template <int K, bool BI>
+template <int K, int KC, bool BI>
 __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
     float2 *__restrict__ st, float *__restrict__ pp, const double4 *__restrict__ recd,
     const float4 *__restrict__ recf, const float2 *__restrict__ recg,
     const double4 *__restrict__ groupc, int64_t G, int ngroups, int gpc,
     const float *__restrict__ ant, const float *__restrict__ rxo,
-    const float2 *__restrict__ mt, double f0c, double dfc, double tc, double rho, int A,
+    const float2 *__restrict__ mt, const float *__restrict__ tay,
+    const float *__restrict__ bes, double f0c, double dfc, double tc, double rho, int A,
     int R, int F, int is_ti) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const int i_raw = (blockIdx.x * 4 + wv) * 64 + (int)(threadIdx.x % 64);
@@ -564,8 +598,9 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
   const double ti = is_ti ? 0.15 : 0.0;
   const double fcc = fma(tc, dfc, f0c);            // f_c / c, turns per metre
   const double inv_rho = 1.0 / rho;
-  const float ystep = (float)(6.283185307179586 * dfc * rho);  // y_{j+1} - y_j
-  const float y0 = (float)(-tc * 6.283185307179586 * dfc * rho);
+  // delta = two * (leg sum as evaluated) + ti - d0; the monostatic sum is one leg
+  const double two = BI ? 1.0 : 2.0;
+  const double xs = two * inv_rho, ps = two * fcc;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f;
 
   for (int b = g_lo; b < g_hi; ++b) {
@@ -580,10 +615,11 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
       d0 += d0;
     }
     d0 += ti;
+    const double xo = (ti - d0) * inv_rho, po = (ti - d0) * fcc;
     float sr, si;  // exp(-j k_j d0) steps by exp(-j 2 pi df d0 / c)
     phasor_quarter(4.0 * d0 * dfc, sr, si);
 
-    // h_k = sum_j k_j s_ij exp(-j k_j d0) (-j y_j)^k / k!
+    // h_k = sum_j k_j s_ij exp(-j k_j d0) (-j)^k y_j^k / k!
     float hr[K], hi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) hr[k] = hi[k] = 0.0f;
@@ -593,17 +629,16 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
       const int j1 = min(F, j0 + kTJ);
       for (int j = j0; j < j1; ++j) {
         const float2 a = mt[(size_t)j * A + i];
-        float tr = fmaf(a.x, er, -(a.y * ei)), tq = fmaf(a.x, ei, a.y * er);
-        const float y = fmaf((float)j, ystep, y0);
+        const float tr = fmaf(a.x, er, -(a.y * ei)), tq = fmaf(a.x, ei, a.y * er);
+        const float *__restrict__ tw = tay + (size_t)j * K;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          hr[k] += tr;
-          hi[k] += tq;
-          // times -j y / (k + 1)
-          const float w = y * (1.0f / (float)(k + 1));
-          const float nr = tq * w, nq = -(tr * w);
-          tr = nr;
-          tq = nq;
+          // (-j)^k t: k mod 4 = 0: t, 1: (tq, -tr), 2: -t, 3: (-tq, tr)
+          const float ur = (k & 1) ? tq : tr, uq = (k & 1) ? tr : tq;
+          const float sgr = ((k & 3) == 2 || (k & 3) == 3) ? -1.0f : 1.0f;
+          const float sgq = ((k & 3) == 1 || (k & 3) == 2) ? -1.0f : 1.0f;
+          hr[k] = fmaf(sgr * tw[k], ur, hr[k]);
+          hi[k] = fmaf(sgq * tw[k], uq, hi[k]);
         }
         const float nr = fmaf(er, sr, -(ei * si)), ni = fmaf(er, si, ei * sr);
         er = nr;
@@ -611,15 +646,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
       }
     }
 
-    float mr[K], mi[K];
+    float mr[KC], mi[KC];
 #pragma unroll
-    for (int k = 0; k < K; ++k) mr[k] = mi[k] = 0.0f;
-    const int64_t v0 = (int64_t)b * G, v1 = v0 + G;
-    for (int64_t v = v0; v < v1; ++v) {
-      const double4 rd = recd[v];
-      const float4 rf = recf[v];
-      const float2 gv = recg[v];
-      const double s1 = fma(rd.z, -2.0 * pdz, fma(rd.y, -2.0 * pdy, fma(rd.x, -2.0 * pdx, rd.w + p2)));
+    for (int k = 0; k < KC; ++k) mr[k] = mi[k] = 0.0f;
+    // one voxel's work: FP64 round trip, carrier, position term, moments
+    auto voxel = [&](const double4 &rd, const float4 &rf, float gr, float gi) {
+      const double s1 =
+          fma(rd.z, -2.0 * pdz, fma(rd.y, -2.0 * pdy, fma(rd.x, -2.0 * pdx, rd.w + p2)));
       const float y1 = __builtin_amdgcn_rsqf((float)s1);
       const double y1d = y1, a1 = s1 * y1d;
       double d = fma(0.5 * y1d, fma(-a1, a1, s1), a1);
@@ -630,15 +663,12 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
         y2 = __builtin_amdgcn_rsqf((float)s2);
         const double y2d = y2, a2 = s2 * y2d;
         d += fma(0.5 * y2d, fma(-a2, a2, s2), a2);
-      } else {
-        d += d;
       }
-      const double delta = d + ti - d0;
-      const float xi = (float)(delta * inv_rho);
-      const float ph = (float)__builtin_amdgcn_fract(delta * fcc);
+      const float xi = (float)fma(d, xs, xo);
+      const float ph = (float)__builtin_amdgcn_fract(fma(d, ps, po));
       const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
       // c = g exp(+j k_c delta)
-      const float cr = fmaf(gv.x, cs, -(gv.y * sn)), ci = fmaf(gv.x, sn, gv.y * cs);
+      const float cr = fmaf(gr, cs, -(gi * sn)), ci = fmaf(gr, sn, gi * cs);
       // position term: Im(conj(g) H) = Im(conj(c) sum_k h_k xi^k)
       float Pr = hr[K - 1], Pi = hi[K - 1];
 #pragma unroll
@@ -647,18 +677,37 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
         Pi = fmaf(Pi, xi, hi[k]);
       }
       const float tt = fmaf(cr, Pi, -(ci * Pr));
-      const float w1 = tt * y1, w2 = tt * y2;
-      gx = fmaf(w1, px - rf.x, fmaf(w2, qx - rf.x, gx));
-      gy = fmaf(w1, py - rf.y, fmaf(w2, qy - rf.y, gy));
-      gz = fmaf(w1, pz - rf.z, fmaf(w2, qz - rf.z, gz));
-      // adjoint moments mu_k += c xi^k
-      float pw = 1.0f;
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        mr[k] = fmaf(cr, pw, mr[k]);
-        mi[k] = fmaf(ci, pw, mi[k]);
-        pw *= xi;
+      if (BI) {
+        const float w1 = tt * y1, w2 = tt * y2;
+        gx = fmaf(w1, px - rf.x, fmaf(w2, qx - rf.x, gx));
+        gy = fmaf(w1, py - rf.y, fmaf(w2, qy - rf.y, gy));
+        gz = fmaf(w1, pz - rf.z, fmaf(w2, qz - rf.z, gz));
+      } else {
+        const float w1 = 2.0f * tt * y1;
+        gx = fmaf(w1, px - rf.x, gx);
+        gy = fmaf(w1, py - rf.y, gy);
+        gz = fmaf(w1, pz - rf.z, gz);
       }
+      // adjoint moments mu_k += c T_k(xi)
+      float t0 = 1.0f, t1 = xi;
+      mr[0] += cr;
+      mi[0] += ci;
+#pragma unroll
+      for (int k = 1; k < KC; ++k) {
+        mr[k] = fmaf(cr, t1, mr[k]);
+        mi[k] = fmaf(ci, t1, mi[k]);
+        const float t2 = fmaf(2.0f * xi, t1, -t0);
+        t0 = t1;
+        t1 = t2;
+      }
+    };
+    // Voxels one at a time: measured on MI355X, two per iteration (more
+    // independent chains, 156 VGPRs) and prefetching the next record gained
+    // nothing at config 2 (18.6 ms either way).
+    const int64_t v0 = (int64_t)b * G, v1 = v0 + G;
+    for (int64_t v = v0; v < v1; ++v) {
+      const float2 gv = recg[v];
+      voxel(recd[v], recf[v], gv.x, gv.y);
     }
 
     // S_ij += exp(+j k_j d0) sum_k (j y_j)^k / k! mu_k  (conjugate steps)
@@ -669,15 +718,17 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
       ei = -ei;
       const int j1 = min(F, j0 + kTJ);
       for (int j = j0; j < j1; ++j) {
-        const float y = fmaf((float)j, ystep, y0);
-        float tr = mr[K - 1], tq = mi[K - 1];
+        // sum_k eps_k J_k(y_j) i^k mu_k; i^k mu: k mod 4 = 0: mu, 1: (-mi, mr),
+        // 2: -mu, 3: (mi, -mr)
+        const float *__restrict__ bw = bes + (size_t)j * KC;
+        float tr = 0.0f, tq = 0.0f;
 #pragma unroll
-        for (int k = K - 1; k >= 1; --k) {
-          // t = mu_{k-1} + (j y / k) t
-          const float w = y * (1.0f / (float)k);
-          const float nr = fmaf(-tq, w, mr[k - 1]), nq = fmaf(tr, w, mi[k - 1]);
-          tr = nr;
-          tq = nq;
+        for (int k = 0; k < KC; ++k) {
+          const float ur = (k & 1) ? mi[k] : mr[k], uq = (k & 1) ? mr[k] : mi[k];
+          const float sgr = ((k & 3) == 1 || (k & 3) == 2) ? -1.0f : 1.0f;
+          const float sgq = ((k & 3) == 2 || (k & 3) == 3) ? -1.0f : 1.0f;
+          tr = fmaf(sgr * bw[k], ur, tr);
+          tq = fmaf(sgq * bw[k], uq, tq);
         }
         const float vr = fmaf(er, tr, -(ei * tq)), vq = fmaf(er, tq, ei * tr);
         if (live) {
@@ -765,7 +816,7 @@ struct BtWs {
   double4 *recd, *groupc;
   float4 *recf;
   float2 *recg, *mt, *st;
-  float *pp;
+  float *pp, *tay, *bes;
   unsigned long long *rmax;
 };
 
@@ -792,7 +843,11 @@ size_t bt_layout(int A, int F, int64_t total, int64_t ngroups, BtWs *w, char *ba
   char *s = take(sizeof(float2) * (size_t)cm * A * F);
   char *p = take(sizeof(float) * 3 * (size_t)cm * A);
   char *r = take(sizeof(unsigned long long));
+  char *ty = take(sizeof(float) * kTaylorK * (size_t)F);
+  char *bs = take(sizeof(float) * kChebK * (size_t)F);
   if (w) {
+    w->tay = (float *)ty;
+    w->bes = (float *)bs;
     w->recd = (double4 *)rd;
     w->recf = (float4 *)rf;
     w->recg = (float2 *)rg;
@@ -1009,7 +1064,9 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
   MMN_REQUIRE(two_pi * band * rho * inv_c <= kTaylorA * (1.0 + 1e-9), MMN_ERR_ARG,
               "mmn_bp_profile_backward: voxel spacing too coarse for the band (a single "
               "voxel group exceeds the series bound)");
-  if (!(rho > 0.0)) rho = 1.0;  // single-voxel groups: xi = 0 throughout
+  // single-voxel groups (xi = 0 throughout): any rho within the bound keeps
+  // |y_j| <= a and the tables' series accurate
+  if (!(rho > 0.0)) rho = band > 0.0 ? kTaylorA * MMN_C0 / (two_pi * band) : 1.0;
   hipLaunchKernelGGL(k_bt_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)), dim3(kBlock), 0, st,
                      w.mt, (const float2 *)meas, A, F, (float)(two_pi * f0 * inv_c),
                      (float)(two_pi * df * inv_c));
@@ -1041,10 +1098,13 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
   if (ch < 1) ch = 1;
   const int gpc = (int)((sh.ngroups + ch - 1) / ch);
   ch = (sh.ngroups + gpc - 1) / gpc;
-  auto kern = bi ? k_bwd_taylor<kTaylorK, true> : k_bwd_taylor<kTaylorK, false>;
+  hipLaunchKernelGGL(k_bt_tables, dim3(mmn_blocks(F, kBlock)), dim3(kBlock), 0, st, w.tay,
+                     w.bes, F, two_pi * df * inv_c * rho, tc);
+  auto kern = bi ? k_bwd_taylor<kTaylorK, kChebK, true> : k_bwd_taylor<kTaylorK, kChebK, false>;
   hipLaunchKernelGGL(kern, dim3((unsigned)rowblocks, (unsigned)ch), dim3(kBlock), 0, st, w.st,
                      w.pp, w.recd, w.recf, w.recg, w.groupc, sh.G, (int)sh.ngroups, gpc, ant,
-                     rx_off, w.mt, f0 * inv_c, df * inv_c, tc, rho, A, num_rx, F, is_ti_radar);
+                     rx_off, w.mt, w.tay, w.bes, f0 * inv_c, df * inv_c, tc, rho, A, num_rx, F,
+                     is_ti_radar);
   if (grad_meas)
     hipLaunchKernelGGL(k_bt_reduce_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)),
                        dim3(kBlock), 0, st, (float2 *)grad_meas, w.st, (int)ch, A, F, scale);

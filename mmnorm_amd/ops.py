@@ -231,8 +231,9 @@ def backproject_backward(g, meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f
     """Adjoint of ``scale * backproject``: (dL/dmeas complex64 (rows,F), dL/dant float32 (rows,3)).
 
     ``variant="rec"`` recomputes every (row, sample, voxel) phase
-    (mmn_bp_backward); ``variant="profile"`` is the exact gradient of the
-    range-profile forward (mmn_bp_profile_backward), ``bbox`` as there.
+    (mmn_bp_backward); ``variant="profile"`` evaluates the same gradients by
+    voxel groups with short series in the distance (mmn_bp_profile_backward;
+    it synchronises the stream once), ``bbox`` as for the profile forward.
     """
     if variant not in ("rec", "profile"):
         raise ValueError('variant must be "rec" or "profile"')
@@ -256,8 +257,7 @@ def backproject_backward(g, meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f
         ws_bytes = int(lib.mmn_bp_profile_backward_workspace_size(
             max(n, 1), A, F, float(f0), float(df), ctypes.addressof(bb)))
         if ws_bytes == 0:
-            raise ValueError("mmn_bp_profile_backward: bounding box not finite, or its "
-                             "range-profile table too long for the profile backward")
+            raise ValueError("mmn_bp_profile_backward: sizes must be positive")
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
         _lib.call("mmn_bp_profile_backward", _lib.ptr(gm), _lib.ptr(ga), g.data_ptr(),
                   float(scale), x_locs.data_ptr(), y_locs.data_ptr(), z_locs.data_ptr(),
@@ -311,8 +311,8 @@ class Backproject(torch.autograd.Function):
 def backproject_ad(meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df, scale=1.0,
                    K=64, vox_idx=None, is_ti=False, variant="rec", bbox=None):
     """Differentiable ``scale * backproject``: the recurrence forward and its
-    backward (``variant="rec"``), or the range-profile forward and its exact
-    gradient (``variant="profile"``)."""
+    backward (``variant="rec"``), or the range-profile forward and the
+    grouped-series backward (``variant="profile"``)."""
     if variant not in ("rec", "profile"):
         raise ValueError('variant must be "rec" or "profile"')
     if rx_off is None:
