@@ -2,7 +2,7 @@
 and full-size (config 2) parity on a seeded voxel sample.
 
 Reported per K in {8, 16, 32, 64} and for the plain (library sincosf and
-hardware sine/cosine) and reference-arithmetic kernels: relative L2 image error over the grid, max relative |I| error and max
+hardware sine/cosine), range-profile and reference-arithmetic kernels: relative L2 image error over the grid, max relative |I| error and max
 phase error (rad) on voxels above 5 % of the max, and the max normal-angle
 error (deg) on those voxels.  Set MMN_REPORT_DIR to also write the table as
 JSON there.
@@ -68,6 +68,9 @@ def test_precision_report_config3(cfg3):
     img, mom = ops.backproject(m, a, x, y, z, f0=sc.f0, df=sc.df, variant="plain_fast",
                                moment=True)
     table["plain_fast"] = metrics(img.cpu().numpy(), mom.cpu().numpy(), ref, rmom)
+    img, mom = ops.backproject(m, a, x, y, z, f0=sc.f0, df=sc.df, variant="profile",
+                               moment=True)
+    table["profile"] = metrics(img.cpu().numpy(), mom.cpu().numpy(), ref, rmom)
     img = ops.cuda_image(m, a, x, y, z, None, slope=np.float32(sc.slope),
                          wavelength=np.float32(sc.wavelength))
     table["reference_arithmetic"] = metrics(img.cpu().numpy(), None, ref, None)
@@ -80,6 +83,8 @@ def test_precision_report_config3(cfg3):
     for K in (8, 16, 32, 64):
         assert table[f"rec_K{K}"]["rel_l2"] <= 1e-4
         assert table[f"rec_K{K}"]["max_normal_deg"] <= 0.01
+    assert table["profile"]["rel_l2"] <= 1e-5
+    assert table["profile"]["max_normal_deg"] <= 0.01
 
 
 def test_config2_full_size_sampled_parity(oracle):
