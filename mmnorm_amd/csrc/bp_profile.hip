@@ -39,6 +39,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "mmn_internal.h"
 
 namespace {
@@ -473,20 +475,39 @@ __global__ __launch_bounds__(kBlock) void k_bt_tables(float *__restrict__ tay,
   }
 }
 
+// Brick of each listed voxel (the sort key) and its list position.
+__global__ __launch_bounds__(kBlock) void k_bt_keys(uint32_t *__restrict__ keys,
+                                                    int32_t *__restrict__ vals,
+                                                    const int32_t *__restrict__ vidx, int64_t n,
+                                                    int ny, int nz, int ex, int ey, int ez) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int64_t lin = vidx[t];
+  const int z = (int)(lin % nz), y = (int)((lin / nz) % ny), x = (int)(lin / ((int64_t)ny * nz));
+  const int nbz = (nz + ez - 1) / ez, nby = (ny + ey - 1) / ey;
+  keys[t] = (uint32_t)(((int64_t)(x / ex) * nby + y / ey) * nbz + z / ez);
+  vals[t] = (int32_t)t;
+}
+
+__global__ void k_bt_set(int32_t *__restrict__ p, int32_t v) { *p = v; }
+
+// Voxel records in group order.  List mode (vidx): record t is list entry
+// perm[t] (the list sorted by brick), total = n.  Grid mode: bricks of
+// ex x ey x ez voxels, positions outside the grid padded with weight 0.
 __global__ __launch_bounds__(kBlock) void k_bt_pack(
     double4 *__restrict__ recd, float4 *__restrict__ recf, float2 *__restrict__ recg,
     const float2 *__restrict__ g, const float *__restrict__ xl, const float *__restrict__ yl,
-    const float *__restrict__ zl, const int32_t *__restrict__ vidx, int64_t n, int nx,
-    int ny, int nz, int ex, int ey, int ez, int64_t G, int64_t total) {
+    const float *__restrict__ zl, const int32_t *__restrict__ vidx,
+    const int32_t *__restrict__ perm, int64_t n, int nx, int ny, int nz, int ex, int ey,
+    int ez, int64_t G, int64_t total) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= total) return;
   float vx, vy, vz;
   float2 gv = make_float2(0.0f, 0.0f);
   if (vidx) {
-    const bool real = t < n;
-    const int64_t src = real ? t : n - 1;
+    const int64_t src = perm[t];
     voxel_coords(xl, yl, zl, ny, nz, vidx[src], vx, vy, vz);
-    if (real) gv = g[t];
+    gv = g[src];
   } else {
     // group b = (bx, by, bz), bz fastest; offset o = (ox, oy, oz), oz fastest
     const int64_t b = t / G, o = t % G;
@@ -506,16 +527,19 @@ __global__ __launch_bounds__(kBlock) void k_bt_pack(
   recg[t] = gv;
 }
 
-// One workgroup per group: the midpoint of its box and its radius about it
-// (FP64); the largest radius also goes to *rmax (as the bits of a positive
-// double, ordered like the values).
+// One workgroup per group (G records each, or [goff[b], goff[b+1])): the
+// midpoint of its box and its radius about it (FP64); the largest radius also
+// goes to *rmax (as the bits of a positive double, ordered like the values).
 __global__ __launch_bounds__(kBlock) void k_bt_groups(double4 *__restrict__ groupc,
                                                       unsigned long long *__restrict__ rmax,
                                                       const double4 *__restrict__ recd,
-                                                      int64_t G) {
+                                                      int64_t G,
+                                                      const int32_t *__restrict__ goff) {
   __shared__ double red[6][kBlock / 64];
   const int64_t b = blockIdx.x;
-  const double4 *__restrict__ r = recd + b * G;
+  const int64_t b0 = goff ? (int64_t)goff[b] : b * G;
+  const double4 *__restrict__ r = recd + b0;
+  if (goff) G = (int64_t)goff[b + 1] - b0;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int64_t t = threadIdx.x; t < G; t += kBlock) {
     const double4 v = r[t];
@@ -577,8 +601,8 @@ template <int K, int KC, bool BI>
 __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
     float2 *__restrict__ st, float *__restrict__ pp, const double4 *__restrict__ recd,
     const float4 *__restrict__ recf, const float2 *__restrict__ recg,
-    const double4 *__restrict__ groupc, int64_t G, int ngroups, int gpc,
-    const float *__restrict__ ant, const float *__restrict__ rxo,
+    const double4 *__restrict__ groupc, int64_t G, const int32_t *__restrict__ goff,
+    int ngroups, int gpc, const float *__restrict__ ant, const float *__restrict__ rxo,
     const float2 *__restrict__ mt, const float *__restrict__ tay,
     const float *__restrict__ bes, double f0c, double dfc, double tc, double rho, int A,
     int R, int F, int is_ti) {
@@ -704,7 +728,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
     // Voxels one at a time: measured on MI355X, two per iteration (more
     // independent chains, 156 VGPRs) and prefetching the next record gained
     // nothing at config 2 (18.6 ms either way).
-    const int64_t v0 = (int64_t)b * G, v1 = v0 + G;
+    const int64_t v0 = goff ? (int64_t)goff[b] : (int64_t)b * G;
+    const int64_t v1 = goff ? (int64_t)goff[b + 1] : v0 + G;
     for (int64_t v = v0; v < v1; ++v) {
       const float2 gv = recg[v];
       voxel(recd[v], recf[v], gv.x, gv.y);
@@ -782,34 +807,44 @@ __global__ __launch_bounds__(kBlock) void k_bt_reduce_ant(float *__restrict__ gr
   grad[t] = (float)(scale * s);
 }
 
-// Group shape: for the grid, bricks whose edges (in voxels) are balanced
-// over each axis and at most emax; for a list, runs of G entries.
+// Group shape: bricks whose edges (in voxels) are balanced over each axis
+// and at most emax.  The grid is packed brick by brick; a voxel list is
+// sorted by brick and each non-empty brick is one group.
 struct BtShape {
   int ex, ey, ez;
   int64_t G, ngroups, total;
 };
 
-BtShape bt_shape(const int32_t *vidx, int64_t n, int nx, int ny, int nz, int emax) {
+BtShape bt_shape(int nx, int ny, int nz, int emax) {
   BtShape s;
-  if (vidx) {
-    s.ex = s.ey = s.ez = 0;
-    s.G = (int64_t)emax * emax * emax;
-    if (s.G > n) s.G = n;
-    s.ngroups = (n + s.G - 1) / s.G;
-  } else {
-    auto edge = [&](int nax) {
-      const int nb = (nax + emax - 1) / emax;
-      return (nax + nb - 1) / nb;
-    };
-    s.ex = edge(nx);
-    s.ey = edge(ny);
-    s.ez = edge(nz);
-    s.G = (int64_t)s.ex * s.ey * s.ez;
-    s.ngroups = (int64_t)((nx + s.ex - 1) / s.ex) * ((ny + s.ey - 1) / s.ey) *
-                ((nz + s.ez - 1) / s.ez);
-  }
+  auto edge = [&](int nax) {
+    const int nb = (nax + emax - 1) / emax;
+    return (nax + nb - 1) / nb;
+  };
+  s.ex = edge(nx);
+  s.ey = edge(ny);
+  s.ez = edge(nz);
+  s.G = (int64_t)s.ex * s.ey * s.ez;
+  s.ngroups = (int64_t)((nx + s.ex - 1) / s.ex) * ((ny + s.ey - 1) / s.ey) *
+              ((nz + s.ez - 1) / s.ez);
   s.total = s.G * s.ngroups;
   return s;
+}
+
+// Temporary storage of the list sort: radix sort by brick, run-length
+// encoding of the sorted bricks, exclusive scan of the run lengths.
+size_t bt_cub_bytes(int64_t n) {
+  size_t a = 0, b = 0, c = 0;
+  const int m = (int)n;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, (const uint32_t *)nullptr,
+                                          (uint32_t *)nullptr, (const int32_t *)nullptr,
+                                          (int32_t *)nullptr, m, 0, 32, (hipStream_t)0);
+  (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, b, (const uint32_t *)nullptr,
+                                             (uint32_t *)nullptr, (int32_t *)nullptr,
+                                             (int32_t *)nullptr, m, (hipStream_t)0);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, c, (const int32_t *)nullptr,
+                                        (int32_t *)nullptr, m, (hipStream_t)0);
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
 }
 
 struct BtWs {
@@ -817,6 +852,11 @@ struct BtWs {
   float4 *recf;
   float2 *recg, *mt, *st;
   float *pp, *tay, *bes;
+  // the list sort (list mode only)
+  uint32_t *k_in, *k_out, *uniq;
+  int32_t *v_in, *v_out, *cnt, *goff, *nruns;
+  void *cub;
+  size_t cub_bytes;
   unsigned long long *rmax;
 };
 
@@ -827,8 +867,10 @@ int bt_chunks_max(int A, int F) {
   return c < 1 ? 1 : (int)c;
 }
 
-size_t bt_layout(int A, int F, int64_t total, int64_t ngroups, BtWs *w, char *base) {
+size_t bt_layout(int A, int F, int64_t total, int64_t ngroups, int64_t nlist, BtWs *w,
+                 char *base) {
   const int cm = bt_chunks_max(A, F);
+  const size_t cubb = nlist > 0 ? bt_cub_bytes(nlist) : 0;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     char *q = base ? base + off : nullptr;
@@ -845,7 +887,23 @@ size_t bt_layout(int A, int F, int64_t total, int64_t ngroups, BtWs *w, char *ba
   char *r = take(sizeof(unsigned long long));
   char *ty = take(sizeof(float) * kTaylorK * (size_t)F);
   char *bs = take(sizeof(float) * kChebK * (size_t)F);
+  const size_t nl = nlist > 0 ? (size_t)nlist : 0;
+  char *ki = take(sizeof(uint32_t) * nl), *ko = take(sizeof(uint32_t) * nl);
+  char *vi = take(sizeof(int32_t) * nl), *vo = take(sizeof(int32_t) * nl);
+  char *un = take(sizeof(uint32_t) * nl), *ct = take(sizeof(int32_t) * nl);
+  char *go = take(sizeof(int32_t) * (nl + 1)), *nr = take(sizeof(int32_t));
+  char *cb = take(cubb);
   if (w) {
+    w->k_in = (uint32_t *)ki;
+    w->k_out = (uint32_t *)ko;
+    w->v_in = (int32_t *)vi;
+    w->v_out = (int32_t *)vo;
+    w->uniq = (uint32_t *)un;
+    w->cnt = (int32_t *)ct;
+    w->goff = (int32_t *)go;
+    w->nruns = (int32_t *)nr;
+    w->cub = cb;
+    w->cub_bytes = cubb;
     w->tay = (float *)ty;
     w->bes = (float *)bs;
     w->recd = (double4 *)rd;
@@ -980,12 +1038,13 @@ extern "C" size_t mmn_bp_profile_backward_workspace_size(int64_t n_vox, int num_
                                                          int samples, double f0, double df,
                                                          const double *bbox) {
   (void)f0;
-  if (n_vox <= 0 || num_rows <= 0 || samples <= 0 || !bbox) return 0;
-  // the list shape at edge 1 has the most records (n) and groups (n); the
-  // grid shapes fit inside n * 8 records (balanced edges pad < 2x per axis)
-  const int64_t total = n_vox * 8, ngroups = n_vox;
   (void)df;
-  return bt_layout(num_rows, samples, total, ngroups, nullptr, nullptr);
+  if (n_vox <= 0 || n_vox >= ((int64_t)1 << 31) || num_rows <= 0 || samples <= 0 || !bbox)
+    return 0;
+  // records: a grid's bricks pad each axis by less than a factor of two
+  // (< 8 n); a list needs n and the sort buffers; groups: at most n
+  const int64_t total = n_vox * 8, ngroups = n_vox;
+  return bt_layout(num_rows, samples, total, ngroups, n_vox, nullptr, nullptr);
 }
 
 extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const float *g,
@@ -1029,23 +1088,53 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
   const double tc = 0.5 * (double)(F - 1);
   const double band = tc * fabs(df);  // t_c |df|
   int emax = bt_emax(F, df, bbox, nx, ny, nz);
-  if (vox_idx) emax = env_int("MMN_PROFB_LIST_EDGE", 8);
-  // Pack the voxels in group order and measure the groups; halve the group
+  // Pack the voxels in group order and measure the groups; halve the brick
   // edge until the largest radius satisfies the series bound (the pitch
   // estimate above assumes evenly spaced axes).
   BtShape sh;
   BtWs w;
   double rho = 0.0;
+  const int32_t *goff = nullptr;
   for (;;) {
-    sh = bt_shape(vox_idx, n, nx, ny, nz, emax);
-    bt_layout(A, F, sh.total, sh.ngroups, &w, (char *)workspace);
-    hipLaunchKernelGGL(k_bt_pack, dim3(mmn_blocks(sh.total, kBlock)), dim3(kBlock), 0, st,
-                       w.recd, w.recf, w.recg, (const float2 *)g, x_locs, y_locs, z_locs,
-                       vox_idx, n, nx, ny, nz, sh.ex, sh.ey, sh.ez, sh.G, sh.total);
-    hipError_t e = hipMemsetAsync(w.rmax, 0, sizeof(unsigned long long), st);
+    sh = bt_shape(nx, ny, nz, emax);
+    hipError_t e = hipSuccess;
+    if (vox_idx) {
+      // sort the list by brick; each run of one brick is a group
+      bt_layout(A, F, n, n, n, &w, (char *)workspace);
+      hipLaunchKernelGGL(k_bt_keys, dim3(mmn_blocks(n, kBlock)), dim3(kBlock), 0, st, w.k_in,
+                         w.v_in, vox_idx, n, ny, nz, sh.ex, sh.ey, sh.ez);
+      int bits = 1;
+      while (bits < 32 && ((int64_t)1 << bits) < sh.ngroups) ++bits;
+      size_t tb = w.cub_bytes;
+      e = hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.k_in, w.k_out, w.v_in, w.v_out, (int)n,
+                                             0, bits, st);
+      tb = w.cub_bytes;
+      if (e == hipSuccess)
+        e = hipcub::DeviceRunLengthEncode::Encode(w.cub, tb, w.k_out, w.uniq, w.cnt, w.nruns,
+                                                  (int)n, st);
+      int32_t runs = 0;
+      if (e == hipSuccess) e = hipMemcpyAsync(&runs, w.nruns, sizeof(runs), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      tb = w.cub_bytes;
+      if (e == hipSuccess)
+        e = hipcub::DeviceScan::ExclusiveSum(w.cub, tb, w.cnt, w.goff, runs, st);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_bt_set, dim3(1), dim3(1), 0, st, w.goff + runs, (int32_t)n);
+      sh.ngroups = runs;
+      sh.total = n;
+      goff = w.goff;
+    } else {
+      bt_layout(A, F, sh.total, sh.ngroups, 0, &w, (char *)workspace);
+    }
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_bt_pack, dim3(mmn_blocks(sh.total, kBlock)), dim3(kBlock), 0, st,
+                         w.recd, w.recf, w.recg, (const float2 *)g, x_locs, y_locs, z_locs,
+                         vox_idx, w.v_out, n, nx, ny, nz, sh.ex, sh.ey, sh.ez, sh.G, sh.total);
+      e = hipMemsetAsync(w.rmax, 0, sizeof(unsigned long long), st);
+    }
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_bt_groups, dim3((unsigned)sh.ngroups), dim3(kBlock), 0, st,
-                         w.groupc, w.rmax, w.recd, sh.G);
+                         w.groupc, w.rmax, w.recd, sh.G, goff);
       unsigned long long bits = 0;
       e = hipMemcpyAsync(&bits, w.rmax, sizeof(bits), hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1102,7 +1191,7 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
                      w.bes, F, two_pi * df * inv_c * rho, tc);
   auto kern = bi ? k_bwd_taylor<kTaylorK, kChebK, true> : k_bwd_taylor<kTaylorK, kChebK, false>;
   hipLaunchKernelGGL(kern, dim3((unsigned)rowblocks, (unsigned)ch), dim3(kBlock), 0, st, w.st,
-                     w.pp, w.recd, w.recf, w.recg, w.groupc, sh.G, (int)sh.ngroups, gpc, ant,
+                     w.pp, w.recd, w.recf, w.recg, w.groupc, sh.G, goff, (int)sh.ngroups, gpc, ant,
                      rx_off, w.mt, w.tay, w.bes, f0 * inv_c, df * inv_c, tc, rho, A, num_rx, F,
                      is_ti_radar);
   if (grad_meas)

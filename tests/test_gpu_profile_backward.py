@@ -177,3 +177,15 @@ def test_config4_matches_recurrence_backward():
                                    variant="profile")
     assert rel(got[0], ref[0]) <= 1e-5
     assert rel(got[1], ref[1]) <= 1e-4
+
+
+def test_shuffled_full_list_matches_grid(mid_case):
+    """Every voxel as a list in random order (the list path sorts it by brick
+    and groups each brick's entries) against the grid path."""
+    sc, g, args = mid_case
+    kw = dict(f0=sc.f0, df=sc.df, scale=0.5, variant="profile")
+    gm, ga = ops.backproject_backward(g, *args, **kw)
+    perm = np.random.default_rng(5).permutation(g.numel()).astype(np.int32)
+    pt = T(perm, np.int32)
+    gm2, ga2 = ops.backproject_backward(g[pt.long()].contiguous(), *args, vox_idx=pt, **kw)
+    assert rel(gm2, gm) <= 1e-6 and rel(ga2, ga) <= 1e-5
