@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -22,7 +23,9 @@ def test_header_declares_the_entry_points():
                  "mmn_bp_backward", "mmn_bp_backward_workspace_size", "mmn_rsdf_bfs",
                  "mmn_rsdf_bfs_workspace_size", "mmn_li_threshold",
                  "mmn_li_threshold_workspace_size", "mmn_ccl6", "mmn_ccl6_workspace_size",
-                 "mmn_label_sizes", "mmn_abi_version", "mmn_last_error"):
+                 "mmn_label_sizes", "mmn_abi_version", "mmn_last_error", "mmn_bp_profile",
+                 "mmn_bp_profile_workspace_size", "mmn_bp_profile_backward",
+                 "mmn_bp_profile_backward_workspace_size"):
         assert want in names
 
 
@@ -76,4 +79,32 @@ def test_workspace_queries_and_argument_errors_without_gpu():
     with pytest.raises(ValueError, match="sizes"):
         _lib.call("mmn_cuda_image", 8, 8, 8, 8, 8, 8, 8, 1.0, 1.0, 1.0, 0, 1, 1, 1, 1, 1, 0,
                   0, None)
+    # range-profile pair: workspace sizes from the bounding box, argument checks
+    bb = (ctypes.c_double * 6)(0.0, 0.0, 0.0, 0.1, 0.1, 0.05)
+    bad = (ctypes.c_double * 6)(0.0, 0.0, 0.0, -0.1, 0.1, 0.05)
+    pb, pbad = ctypes.addressof(bb), ctypes.addressof(bad)
+    # M = ceil(2 |diag| / h) + 6 table points per row, h = c / (64 (F-1) df):
+    # tables of 8 + 32 bytes per point at least
+    h = 2.99792458e8 / (64 * 63 * 62.5e6)
+    M = int(np.ceil(2 * np.sqrt(0.1 ** 2 + 0.1 ** 2 + 0.05 ** 2) / h)) + 6
+    assert lib.mmn_bp_profile_workspace_size(1000, 100, 64, 77e9, 62.5e6, pb, 1) >= 100 * M * 40
+    assert lib.mmn_bp_profile_workspace_size(1000, 100, 64, 77e9, 62.5e6, pbad, 1) == 0
+    assert lib.mmn_bp_profile_workspace_size(1000, 100, 64, float("nan"), 62.5e6, pb, 1) == 0
+    assert lib.mmn_bp_profile_backward_workspace_size(1000, 100, 64, 77e9, 62.5e6, pb) > 0
+    assert lib.mmn_bp_profile_backward_workspace_size(0, 100, 64, 77e9, 62.5e6, pb) == 0
+    with pytest.raises(ValueError, match="NULL"):
+        _lib.call("mmn_bp_profile", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 62.5e6, 1, 1, 1,
+                  1, 1, 64, 0, None, None, 0, None)
+    with pytest.raises(ValueError, match="bounding box"):
+        _lib.call("mmn_bp_profile", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 62.5e6, 1, 1, 1,
+                  1, 1, 64, 0, pbad, None, 0, None)
+    with pytest.raises(ValueError, match="workspace"):
+        _lib.call("mmn_bp_profile", 8, None, 8, 8, 8, None, 0, 8, 8, 8, 77e9, 62.5e6, 1, 1, 1,
+                  1, 1, 64, 0, pb, None, 0, None)
+    with pytest.raises(ValueError, match="finite"):
+        _lib.call("mmn_bp_profile_backward", 8, 8, 8, 1.0, 8, 8, 8, None, 0, 8, 8, 8,
+                  float("inf"), 62.5e6, 1, 1, 1, 1, 1, 64, 0, pb, None, 0, None)
+    with pytest.raises(ValueError, match="workspace"):
+        _lib.call("mmn_bp_profile_backward", 8, 8, 8, 1.0, 8, 8, 8, None, 0, 8, 8, 8, 77e9,
+                  62.5e6, 1, 1, 1, 1, 1, 64, 0, pb, None, 0, None)
     ctypes.get_errno()
