@@ -15,8 +15,9 @@ above ``initial_filter_percent`` of the image maximum
   (surface_normal_est.cu:64-149).
 * ``use_interpolated_processing=False`` (the reference raises here,
   py_sn_est_gpu.py:404-405): the full-frequency vote, from the moment
-  M = sum_i vec_i c_i that the recurrence kernel accumulates with the image,
-  n = Re(conj(I) M) / |I| with I the image passed in.
+  M = sum_i vec_i c_i that the recurrence kernel accumulates with the image
+  (``variant="profile"``: the range-profile kernel), n = Re(conj(I) M) / |I|
+  with I the image passed in.
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ import math
 import numpy as np
 
 from . import ops
-from .py_image_gpu import MAX_GROUP_ROWS, _device, _to_dev
+from .py_image_gpu import MAX_GROUP_ROWS, _device, _to_dev, host_bbox
 
 
 def _prep_inputs(radar_data, rx_offset, antenna_locs, use_4_rx):
@@ -78,9 +79,11 @@ def _sn_est(sum_image, x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offs
     else:
         f0 = ops.C0 / float(np.float32(wavelength))
         df = float(np.float32(slope))
+        fwd = "profile" if variant == "profile" else "rec"
+        bbox = host_bbox(x_locs, y_locs, z_locs) if fwd == "profile" else None
         _, mom = ops.backproject(meas_t, ant_t, xl, yl, zl, rx_t, f0=f0, df=df,
-                                 variant="rec", K=K, moment=True, vox_idx=vidx,
-                                 is_ti=is_ti_radar)
+                                 variant=fwd, K=K, moment=True, vox_idx=vidx,
+                                 is_ti=is_ti_radar, bbox=bbox)
         normals = ops.sn_from_moment(img_t, mom, vidx)
     all_normals[filt] = normals.cpu().numpy()
     return all_normals
@@ -91,12 +94,12 @@ def normal_estimates_cuda(sum_image, x_locs, y_locs, z_locs, antenna_locs, radar
                           use_4_rx=False, is_ti_radar=False, initial_filter_percent=0.05,
                           use_interpolated_processing=True, *, K=64, variant="fast"):
     """Surface-normal votes on the GPU (see module docstring)."""
-    if variant not in ("fast", "ref"):
-        raise ValueError("variant must be 'fast' or 'ref'")
+    if variant not in ("fast", "profile", "ref"):
+        raise ValueError("variant must be 'fast', 'profile' or 'ref'")
     if not use_interpolated_processing:
         return _sn_est(sum_image, x_locs, y_locs, z_locs, antenna_locs, radar_data,
                        rx_offset, slope, wavelength, bandwidth, num_samples, use_4_rx,
-                       is_ti_radar, initial_filter_percent, False, K)
+                       is_ti_radar, initial_filter_percent, False, K, variant)
     num_rx = len(rx_offset)
     antenna_locs = np.asarray(antenna_locs)
     radar_data = np.asarray(radar_data)

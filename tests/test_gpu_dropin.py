@@ -49,12 +49,13 @@ def test_image_cuda_full_ref_variant(oracle):
     np.testing.assert_array_equal(centers, wc)
 
 
-def test_image_cuda_full_rec_variant_close_to_fp64(oracle):
+@pytest.mark.parametrize("variant", ["rec", "profile"])
+def test_image_cuda_full_variant_close_to_fp64(oracle, variant):
     xl, yl, zl, locs, data, rxo, lam = real_like_inputs(seed=8)
     slope = 5.86116e6
     img = py_image_gpu.image_cuda(xl, yl, zl, locs, data, rxo, slope, lam, 3.0009e9, 512,
                                   use_4_rx=True, is_ti_radar=True,
-                                  return_phase_centers=False)
+                                  return_phase_centers=False, variant=variant)
     ant, meas = _layout(locs, data, 4)
     f0 = C0 / float(np.float32(lam))
     want = oracle.bp_f64(xl, yl, zl, ant, meas, rxo, f0, float(np.float32(slope)),
@@ -105,14 +106,17 @@ def test_normal_estimates_interpolated(oracle, variant):
     assert np.max(np.abs(nrm[filt] - want)) <= 1e-5 * np.max(np.abs(want))
 
 
-def test_normal_estimates_full_frequency(oracle):
+@pytest.mark.parametrize("variant", ["fast", "profile"])
+def test_normal_estimates_full_frequency(oracle, variant):
     xl, yl, zl, locs, data, rxo, lam = real_like_inputs(seed=12)
     img = py_image_gpu.image_cuda(xl, yl, zl, locs, data, rxo, 5.86116e6, lam, 3.0009e9, 512,
-                                  use_4_rx=True, is_ti_radar=True, return_phase_centers=False)
+                                  use_4_rx=True, is_ti_radar=True, return_phase_centers=False,
+                                  variant="rec" if variant == "fast" else "profile")
     nrm = py_sn_est_gpu.normal_estimates_cuda(img, xl, yl, zl, locs, data, rxo, 5.86116e6,
                                               lam, 3.0009e9, 512, use_4_rx=True,
                                               is_ti_radar=True,
-                                              use_interpolated_processing=False)
+                                              use_interpolated_processing=False,
+                                              variant=variant)
     filt = np.abs(img) > np.max(np.abs(img)) * 0.05
     ant, meas = _layout(locs, data, 4)
     f0 = C0 / float(np.float32(lam))
