@@ -105,9 +105,11 @@ def test_voxel_lists(ragged, oracle):
     assert empty.numel() == 0
 
 
-@pytest.mark.parametrize("F,df", [(1, 15.7e6), (2, 15.7e6), (5, 15.7e6), (3, 0.0)])
+@pytest.mark.parametrize("F,df", [(1, 15.7e6), (2, 15.7e6), (5, 15.7e6), (3, 0.0),
+                                  (33, -62.5e6)])
 def test_few_samples_and_zero_step(ragged, oracle, F, df):
-    """One sample (the table is constant), two and five samples, and df = 0."""
+    """One sample (the table is constant), two and five samples, df = 0, and
+    a descending sweep (df < 0)."""
     sc, rows, _, rxo = ragged
     rng = np.random.default_rng(F)
     meas = (rng.normal(size=(len(rows), F)) + 1j * rng.normal(size=(len(rows), F)))
