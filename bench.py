@@ -23,8 +23,13 @@ in HBM.  Prints ONE JSON line:
 * backward   one backward (dL/dmeas, dL/dant) of L = sum w|I|^2
 * rsdf_bfs   GPU 6-connected labelling and GPU BFS over the 5%-filtered vote
              field
+* dropin_end_to_end_ms  image_cuda (full branch) with NumPy in and out, the
+             host-device copies included
 * cpu_baseline  the CPU restatement of the reference's imaging loop
-             (imaging.cpp:60-139, OpenMP) on a bounded voxel sample
+             (imaging.cpp:60-139, OpenMP) on a bounded voxel sample; .bfs the
+             C restatement of the reference's FIFO BFS on one core and the Python
+             deque restatement (the reference's own formulation), on the same
+             field, and whether the GPU BFS equals them bit for bit
 """
 from __future__ import annotations
 
@@ -95,7 +100,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--K", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=2)
@@ -154,6 +159,7 @@ def main(argv=None):
     ms_step = wall / a.steps * 1e3
     log(f"rec forward: {ms_step:.2f} ms/step")
     k_avg = float(np.mean(kms)) * 1e-3
+    k_med = float(np.median(kms)) * 1e-3
     achieved = flops_fwd(V, A, F) / k_avg
     rec = {"metric": "G voxel*antenna*freq/s (recurrence fwd+vote, FP32 VALU)",
            "value": W / (wall / a.steps) / 1e9, "unit": "G voxel*row*sample/s",
@@ -166,7 +172,7 @@ def main(argv=None):
                                   f"{cfg.n_scat} scatterers",
                       "V": V, "A": A, "F": F, "K": a.K, "kernel": "k_bp_rec",
                       "parallelism": "single GPU"},
-           "kernel_ms": k_avg * 1e3,
+           "kernel_ms": k_avg * 1e3, "kernel_ms_median": k_med * 1e3,
            "roofline": {"bound": "valu", "achieved": achieved / 1e12,
                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP32,
@@ -232,8 +238,20 @@ def main(argv=None):
         rec["backward"] = {"ms": b_s * 1e3, "over_forward": b_s / k_avg,
                            "tflops": V * A * (16.0 * F + 30.0) / b_s / 1e12}
         log(f"backward: {b_s * 1e3:.2f} ms")
-        rec["rsdf_bfs"] = bfs_stage(img, mom, sc, dev)
+        rec["rsdf_bfs"], bfs_state = bfs_stage(img, mom, sc, dev)
         log(f"rsdf bfs: {rec['rsdf_bfs']}")
+        # the drop-in end to end: NumPy in, NumPy out (host-device copies included)
+        from mmnorm_amd import py_image_gpu
+        lam = synth.C0 / sc.f0
+        dropin = lambda: py_image_gpu.image_cuda(
+            sc.x_locs, sc.y_locs, sc.z_locs, sc.ant, sc.meas, np.zeros((1, 3)), sc.df, lam,
+            sc.df * (F - 1), F, use_interpolated_processing=False, return_phase_centers=False)
+        dropin()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            dropin()
+        rec["dropin_end_to_end_ms"] = (time.perf_counter() - t0) / 2 * 1e3
+        log(f"image_cuda drop-in end to end: {rec['dropin_end_to_end_ms']:.1f} ms")
     rec["scene_s"] = t_scene
     if a.copy_calibrate > 0:
         src = torch.empty(a.copy_calibrate // 4, dtype=torch.float32, device=dev).fill_(1.0)
@@ -245,6 +263,9 @@ def main(argv=None):
     if not a.no_cpu:
         log("cpu baseline ...")
         rec["cpu_baseline"] = cpu_baseline(sc)
+        if not a.no_extras:
+            rec["cpu_baseline"]["bfs"] = cpu_bfs_baseline(bfs_state)
+            log(f"cpu bfs: {rec['cpu_baseline']['bfs']}")
     print(json.dumps(rec), flush=True)
     return 0
 
@@ -354,9 +375,38 @@ def bfs_stage(img, mom, sc, dev):
     rsdf, hops = ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return {"ms": dt * 1e3, "ccl_ms": ccl_ms, "voxels_reached": int((hops >= 0).sum().item()),
-            "components": int(len(seeds)), "levels": int(hops.max().item()) + 1,
-            "valid_voxels": int(vidx.numel())}
+    res = {"ms": dt * 1e3, "ccl_ms": ccl_ms, "voxels_reached": int((hops >= 0).sum().item()),
+           "components": int(len(seeds)), "levels": int(hops.max().item()) + 1,
+           "valid_voxels": int(vidx.numel())}
+    # what the CPU baseline of the BFS needs: the same inputs and the GPU result
+    state = (nrm_np, labeled, seeds, dx, rsdf.cpu().numpy(), hops.cpu().numpy())
+    return res, state
+
+
+def cpu_bfs_baseline(state):
+    """The reference's FIFO BFS (sdf.py:180-222) restated in C, one core, on
+    the same normals, labels and seeds; and whether the GPU result equals it
+    bit for bit."""
+    from oracle import oracle as O
+    nrm_np, labeled, seeds, dx, rsdf_gpu, hops_gpu = state
+    t0 = time.perf_counter()
+    rsdf, hops = O.bfs_fifo(nrm_np, labeled, seeds, dx)
+    dt = time.perf_counter() - t0
+    same = bool(np.array_equal(hops, hops_gpu) and
+                np.array_equal(rsdf.view(np.int64), rsdf_gpu.view(np.int64)))
+    out = {"ms": dt * 1e3, "cores": 1, "kind": "port",
+           "sample": f"the whole BFS of the bench's field ({int((hops >= 0).sum())} voxels "
+                     f"reached), C restatement of sdf.py:180-222, {cpu_model()}",
+           "gpu_bit_identical": same}
+    # the reference's own formulation (deque + np.dot per voxel), when bounded
+    if int((hops >= 0).sum()) <= 500_000:
+        t0 = time.perf_counter()
+        prs, phops = O.py_bfs_fifo(nrm_np, labeled, seeds, dx)
+        out["python_ms"] = (time.perf_counter() - t0) * 1e3
+        out["python_bit_identical"] = bool(
+            np.array_equal(phops, hops_gpu) and
+            np.array_equal(np.nan_to_num(prs, nan=-1.0), np.nan_to_num(rsdf_gpu, nan=-1.0)))
+    return out
 
 
 if __name__ == "__main__":
