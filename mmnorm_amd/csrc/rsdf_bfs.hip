@@ -3,26 +3,41 @@
 // The reference (sdf.py:180-222) runs one FIFO BFS per component.  FIFO order
 // is level order, and inside level L+1 it is the order in which level-L
 // voxels discovered their children: a child belongs to the first frontier
-// position p (then the first direction d) that reaches it.  So for every
-// level:
-//   claim  each frontier position p offers key = 6p + d to every enterable
-//          neighbour; the neighbour keeps the minimum (atomicMin on unique
-//          keys: the winner does not depend on scheduling),
-//   count  each p finds which of its neighbours it won (bit mask) and each
-//          block sums its winners,
-//   scan   one block turns the block sums into block offsets and writes the
-//          next frontier size,
-//   emit   each p writes its winners, in direction order, at its offset: the
-//          next frontier comes out sorted by key, i.e. in FIFO order, and
-//          every child's rsdf uses its winning parent:
+// position p (then the first direction d) that reaches it.  Frontier entry p
+// of a level whose first entry has queue index q0 (the number of voxels the
+// reference's FIFO queue held before it) offers the key 6 (q0 + p) + d, so
+// keys grow along the queue.  For every level:
+//   claim  each entry offers its keys to its enterable neighbours (in the
+//          grid, same label, normal not NaN: a static 6-bit mask per voxel,
+//          computed once, sdf.py:169-177); a neighbour keeps the minimum
+//          (atomicMin on unique keys: the winner does not depend on
+//          scheduling).  A voxel reached at an earlier level already holds a
+//          smaller key (a seed holds -1), so offering to it changes nothing:
+//          no visited test is needed,
+//   count  each entry finds which neighbours hold its keys (its won mask),
+//   scan   exclusive sums of the won counts give output offsets and the next
+//          frontier size,
+//   emit   each entry writes its children, in direction order, at its
+//          offset: the next frontier comes out sorted by key, i.e. in FIFO
+//          order, and every child's rsdf uses its winning parent:
 //              rsdf[c] = rsdf[p] + dot(dir * dx, normals[c])  (sdf.py:222).
 // Components never share voxels (labels differ), so all components run in
 // one frontier, seeded in the reference's component order; each keeps its
 // own FIFO order because keys only compare voxels of the same component.
 //
-// Each block owns one contiguous chunk of the frontier, so block order is
-// position order.  The frontier size lives in device memory; the host
-// enqueues levels in batches and reads the size back once per batch.
+// Two forms of a level.  Frontiers of up to about a thousand entries run in
+// k_bfs_small, a single 1024-thread workgroup that loops over levels with
+// the frontier, its rsdf and its masks in LDS: no launches, and the
+// frontier's data never goes through global memory, but one CU issues all
+// of a level's scattered requests.  Larger frontiers run as four kernels
+// per level over kGrid blocks, each block owning one contiguous chunk of
+// the frontier (block order is position order); the host enqueues those
+// levels in batches, reads the frontier size back once per batch, and
+// hands back to k_bfs_small when the frontier shrinks.
+#include <stdlib.h>
+
+#include <algorithm>
+
 #include "mmn_internal.h"
 
 namespace {
@@ -31,19 +46,18 @@ constexpr int kGrid = 1024;
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kLevelsPerBatch = 16;
-
-__constant__ int kDX[6] = {-1, 0, 1, 0, 0, 0};
-__constant__ int kDY[6] = {0, 1, 0, -1, 0, 0};
-__constant__ int kDZ[6] = {0, 0, 0, 0, 1, -1};
+constexpr int kSmallThreads = 1024;  // one workgroup; PER positions per thread
 
 struct Ws {
-  int32_t *claim;   // [V]  minimum key offered this level
+  int32_t *claim;   // [V]  smallest key offered (-1 at seeds, INT32_MAX never)
   int32_t *front0;  // [V]
   int32_t *front1;  // [V]
-  uint8_t *pmask;   // [V]  winners of frontier position p (bit d)
+  uint8_t *pmask;   // [V]  won mask of frontier position p (bit d)
+  uint8_t *emask;   // [V]  enterable neighbours of each voxel (bit d)
   int32_t *bcnt;    // [kGrid]
   int32_t *boff;    // [kGrid]
-  int32_t *nf;      // [2]  frontier sizes by level parity
+  int32_t *nf;      // [4]  frontier sizes by level parity; the level k_bfs_small
+                    //      stopped at; queue index of the current level's first entry
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -59,14 +73,16 @@ size_t ws_layout(int64_t V, Ws *w, char *base) {
   char *f0 = take(sizeof(int32_t) * V);
   char *f1 = take(sizeof(int32_t) * V);
   char *pm = take(V);
+  char *em = take(V);
   char *bc = take(sizeof(int32_t) * kGrid);
   char *bo = take(sizeof(int32_t) * kGrid);
-  char *nf = take(sizeof(int32_t) * 2);
+  char *nf = take(sizeof(int32_t) * 4);
   if (w) {
     w->claim = (int32_t *)c;
     w->front0 = (int32_t *)f0;
     w->front1 = (int32_t *)f1;
     w->pmask = (uint8_t *)pm;
+    w->emask = (uint8_t *)em;
     w->bcnt = (int32_t *)bc;
     w->boff = (int32_t *)bo;
     w->nf = (int32_t *)nf;
@@ -80,21 +96,46 @@ __device__ __forceinline__ void chunk_of(int n, int &start, int &end) {
   end = min(n, start + chunk);
 }
 
-// Linear index of neighbour d of (x, y, z), or -1 when outside the grid.
-__device__ __forceinline__ int64_t neighbour(int x, int y, int z, int d,
-                                             int nx, int ny, int nz) {
-  const int ax = x + kDX[d], ay = y + kDY[d], az = z + kDZ[d];
-  if (ax < 0 || ay < 0 || az < 0 || ax >= nx || ay >= ny || az >= nz) return -1;
-  return ((int64_t)ax * ny + ay) * nz + az;
+// Step of direction d in linear index.  The reference's neighbour order
+// (dX, dY, dZ of sdf.py:145-147): d = 0..5 is (-1, 0, 0), (0, 1, 0), (1, 0, 0), (0, -1, 0),
+// (0, 0, 1), (0, 0, -1) in (x, y, z), index x * ny * nz + y * nz + z.
+__device__ __forceinline__ int32_t dir_step(int d, int ny, int nz) {
+  const int32_t yz = ny * nz;
+  return d == 0 ? -yz : d == 1 ? nz : d == 2 ? yz : d == 3 ? -nz : d == 4 ? 1 : -1;
 }
 
-__device__ __forceinline__ void decode(int64_t u, int ny, int nz, int &x,
-                                       int &y, int &z) {
-  const int64_t yz = (int64_t)ny * nz;
-  x = (int)(u / yz);
-  const int64_t r = u - (int64_t)x * yz;
-  y = (int)(r / nz);
-  z = (int)(r - (int64_t)y * nz);
+// Bit d set when voxel u has a neighbour in direction d inside the grid.
+__device__ __forceinline__ unsigned inside_mask(int32_t u, int nx, int ny, int nz) {
+  const uint32_t yz = (uint32_t)(ny * nz);
+  const uint32_t x = (uint32_t)u / yz, r = (uint32_t)u - x * yz;
+  const uint32_t y = r / (uint32_t)nz, z = r - y * (uint32_t)nz;
+  return (unsigned)(x > 0) | (unsigned)(y + 1 < (uint32_t)ny) << 1 |
+         (unsigned)(x + 1 < (uint32_t)nx) << 2 | (unsigned)(y > 0) << 3 |
+         (unsigned)(z + 1 < (uint32_t)nz) << 4 | (unsigned)(z > 0) << 5;
+}
+
+// a[i] with a 32-bit byte offset (every voxel index: V <= 2^31 / 6), so the
+// access takes the scalar-base form with one VGPR of address.
+template <typename T>
+__device__ __forceinline__ T &at32(T *a, int32_t i) {
+  return *(T *)((char *)a + (uint32_t)i * (uint32_t)sizeof(T));
+}
+
+// Key of frontier position p in direction d at queue index q0 + p.  Unsigned
+// arithmetic: positions past the frontier may compute keys that are never used.
+__device__ __forceinline__ int32_t key_of(int32_t q0, int p, int d) {
+  return (int32_t)(6u * (uint32_t)(q0 + p) + (uint32_t)d);
+}
+
+// dot(dir_d * dx, n) as the reference forms it: three products, summed.
+__device__ __forceinline__ double step_dot(int d, double dx, double n0, double n1, double n2) {
+  const double ex = d == 0 ? -1.0 : d == 2 ? 1.0 : 0.0;
+  const double ey = d == 1 ? 1.0 : d == 3 ? -1.0 : 0.0;
+  const double ez = d == 4 ? 1.0 : d == 5 ? -1.0 : 0.0;
+  const double t0 = __dmul_rn(ex * dx, n0);
+  const double t1 = __dmul_rn(ey * dx, n1);
+  const double t2 = __dmul_rn(ez * dx, n2);
+  return __dadd_rn(__dadd_rn(t0, t1), t2);
 }
 
 __global__ void k_init(double *__restrict__ rsdf, int32_t *__restrict__ hops,
@@ -108,40 +149,56 @@ __global__ void k_init(double *__restrict__ rsdf, int32_t *__restrict__ hops,
   }
 }
 
+// emask[v] bit d: neighbour d of v is in the grid, has a non-NaN normal and
+// v's label (the enterability test of sdf.py:169-177 without "not visited").
+__global__ void k_emask(uint8_t *__restrict__ emask, const double *__restrict__ nrm,
+                        const int32_t *__restrict__ lab, int nx, int ny, int nz) {
+  const int32_t V = nx * ny * nz;
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+    const unsigned in = inside_mask(v, nx, ny, nz);
+    const int32_t lv = at32(lab, v);
+    unsigned m = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      if (!(in >> d & 1)) continue;
+      const int32_t c = v + dir_step(d, ny, nz);
+      if (!isnan(nrm[3 * (int64_t)c]) && at32(lab, c) == lv) m |= 1u << d;
+    }
+    at32(emask, v) = (uint8_t)m;
+  }
+}
+
 __global__ void k_seed(double *__restrict__ rsdf, int32_t *__restrict__ hops,
-                       int32_t *__restrict__ front, int32_t *__restrict__ nf,
-                       const int32_t *__restrict__ seeds, int n) {
+                       int32_t *__restrict__ claim, int32_t *__restrict__ front,
+                       int32_t *__restrict__ nf, const int32_t *__restrict__ seeds, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) {
     const int32_t s = seeds[t];
     front[t] = s;
     hops[s] = 0;
     rsdf[s] = 0.0;
+    claim[s] = -1;
   }
   if (t == 0) {
     nf[0] = n;
     nf[1] = 0;
+    nf[2] = 0;
+    nf[3] = 0;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_claim(
-    const int32_t *__restrict__ front, const int32_t *__restrict__ nfp,
-    int32_t *__restrict__ claim, const int32_t *__restrict__ hops,
-    const double *__restrict__ nrm, const int32_t *__restrict__ lab, int nx,
-    int ny, int nz) {
+    const int32_t *__restrict__ front, const int32_t *__restrict__ nf, int cur,
+    int32_t *__restrict__ claim, const uint8_t *__restrict__ emask, int ny, int nz) {
   int start, end;
-  chunk_of(*nfp, start, end);
+  chunk_of(nf[cur], start, end);
+  const int32_t q0 = nf[3];
   for (int p = start + threadIdx.x; p < end; p += kBlock) {
     const int32_t u = front[p];
-    int x, y, z;
-    decode(u, ny, nz, x, y, z);
-    const int32_t lu = lab[u];
-    for (int d = 0; d < 6; ++d) {
-      const int64_t c = neighbour(x, y, z, d, nx, ny, nz);
-      if (c < 0 || hops[c] != -1) continue;
-      if (isnan(nrm[3 * c]) || lab[c] != lu) continue;
-      atomicMin(&claim[c], 6 * p + d);
-    }
+    const unsigned m = at32(emask, u);
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if (m >> d & 1) atomicMin(&at32(claim, u + dir_step(d, ny, nz)), key_of(q0, p, d));
   }
 }
 
@@ -157,35 +214,34 @@ __device__ __forceinline__ int block_sum(int v, int *sh) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_count(
-    const int32_t *__restrict__ front, const int32_t *__restrict__ nfp,
-    const int32_t *__restrict__ claim, const int32_t *__restrict__ hops,
-    uint8_t *__restrict__ pmask, int32_t *__restrict__ bcnt, int nx, int ny,
-    int nz) {
+    const int32_t *__restrict__ front, const int32_t *__restrict__ nf, int cur,
+    const int32_t *__restrict__ claim, const uint8_t *__restrict__ emask,
+    uint8_t *__restrict__ pmask, int32_t *__restrict__ bcnt, int ny, int nz) {
   __shared__ int sh[kBlock / kWave];
   int start, end;
-  chunk_of(*nfp, start, end);
+  chunk_of(nf[cur], start, end);
+  const int32_t q0 = nf[3];
   int cnt = 0;
   for (int p = start + threadIdx.x; p < end; p += kBlock) {
     const int32_t u = front[p];
-    int x, y, z;
-    decode(u, ny, nz, x, y, z);
-    unsigned m = 0;
-    for (int d = 0; d < 6; ++d) {
-      const int64_t c = neighbour(x, y, z, d, nx, ny, nz);
-      if (c < 0 || hops[c] != -1) continue;
-      if (claim[c] == 6 * p + d) m |= 1u << d;
-    }
-    pmask[p] = (uint8_t)m;
-    cnt += __popc(m);
+    const unsigned m = at32(emask, u);
+    unsigned won = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if ((m >> d & 1) && at32(claim, u + dir_step(d, ny, nz)) == key_of(q0, p, d))
+        won |= 1u << d;
+    pmask[p] = (uint8_t)won;
+    cnt += __popc(won);
   }
   const int tot = block_sum(cnt, sh);
   if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
 }
 
-// One block of kGrid threads: exclusive scan of the per-block counts.
+// One block of kGrid threads: exclusive scan of the per-block counts; writes
+// the next frontier size and advances the queue index by this level's size.
 __global__ __launch_bounds__(kGrid) void k_scan(const int32_t *__restrict__ bcnt,
                                                 int32_t *__restrict__ boff,
-                                                int32_t *__restrict__ nf_next) {
+                                                int32_t *__restrict__ nf, int cur) {
   __shared__ int sh[kGrid / kWave];
   const int t = threadIdx.x;
   const int v = bcnt[t];
@@ -200,7 +256,10 @@ __global__ __launch_bounds__(kGrid) void k_scan(const int32_t *__restrict__ bcnt
   int before = 0;
   for (int i = 0; i < w; ++i) before += sh[i];
   boff[t] = before + inc - v;
-  if (t == kGrid - 1) *nf_next = before + inc;
+  if (t == kGrid - 1) {
+    nf[3] += nf[cur];
+    nf[cur ^ 1] = before + inc;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_emit(
@@ -208,8 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     const uint8_t *__restrict__ pmask, const int32_t *__restrict__ boff,
     int32_t *__restrict__ next, int32_t *__restrict__ hops,
     double *__restrict__ rsdf, const double *__restrict__ nrm, double dx,
-    int level, int nx, int ny, int nz) {
-#pragma clang fp contract(off)
+    int level, int ny, int nz) {
   __shared__ int sh[kBlock / kWave];
   int start, end;
   chunk_of(*nfp, start, end);
@@ -235,22 +293,186 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     if (m) {
       int out = base + before + inc - c;
       const int32_t u = front[p];
-      int x, y, z;
-      decode(u, ny, nz, x, y, z);
-      const double prev = rsdf[u];
+      const double prev = at32(rsdf, u);
       for (int d = 0; d < 6; ++d) {
         if (!(m & (1u << d))) continue;
-        const int64_t ch = neighbour(x, y, z, d, nx, ny, nz);
-        next[out++] = (int32_t)ch;
-        hops[ch] = level + 1;
-        // dot(dir * dx, n) as the reference forms it: three products, summed.
-        const double t0 = ((double)kDX[d] * dx) * nrm[3 * ch + 0];
-        const double t1 = ((double)kDY[d] * dx) * nrm[3 * ch + 1];
-        const double t2 = ((double)kDZ[d] * dx) * nrm[3 * ch + 2];
-        rsdf[ch] = prev + ((t0 + t1) + t2);
+        const int32_t ch = u + dir_step(d, ny, nz);
+        const double *nc = nrm + 3 * (int64_t)ch;
+        next[out++] = ch;
+        at32(hops, ch) = level + 1;
+        at32(rsdf, ch) = __dadd_rn(prev, step_dot(d, dx, nc[0], nc[1], nc[2]));
       }
     }
     base += round_tot;
+  }
+}
+
+// Levels level0, level0 + 1, ... in one workgroup while the frontier has
+// 1..cap entries (cap <= 1024 PER).  Writes each level's next size to
+// nf[parity], and at exit the first level it did not run to nf[2] and that
+// level's queue index to nf[3].  The frontier (voxel, rsdf, enterable mask)
+// stays in LDS between levels; thread t owns positions [PER t, PER t + PER)
+// in the claim, count and scan phases (so an exclusive scan over threads
+// keeps position order) and entries t, t + 1024, ... when the next frontier
+// is completed.  Positions past the frontier reread its last entry and are
+// masked off, so every load of a phase is issued before any is used.  claim[]
+// is read with agent-scope atomic loads (the atomics are performed at L2; a
+// plain load could hit an older L1 line).  hops and rsdf are written here
+// and read only by later launches.  A level whose next frontier outgrows the
+// workgroup writes it to global memory for the multi-block kernels and ends
+// the loop.
+template <int PER>
+__global__ __launch_bounds__(kSmallThreads) void k_bfs_small(
+    int32_t *__restrict__ fr0, int32_t *__restrict__ fr1, int32_t *__restrict__ nf,
+    int32_t *__restrict__ claim, int32_t *__restrict__ hops, double *__restrict__ rsdf,
+    const double *__restrict__ nrm, const uint8_t *__restrict__ emask, double dx, int level0,
+    int cap, int ny, int nz) {
+  constexpr int kCap = kSmallThreads * PER;
+  __shared__ int32_t s_u[kCap];  // frontier voxels
+  __shared__ double s_r[kCap];   // their rsdf (parent rsdf while a level is emitted)
+  __shared__ uint8_t s_m[kCap];  // their enterable masks
+  __shared__ uint8_t s_d[kCap];  // direction each next-frontier entry was reached in
+  __shared__ int wsum[kSmallThreads / kWave];
+  const int t = threadIdx.x;
+  const int w = t / kWave, l = t % kWave;
+  const int p0 = t * PER;
+  int level = level0;
+  int n = nf[level & 1];
+  int32_t q0 = nf[3];
+  if (n > 0 && n <= cap) {
+    const int32_t *__restrict__ cur = (level & 1) ? fr1 : fr0;
+    for (int j = t; j < n; j += kSmallThreads) {
+      const int32_t u = cur[j];
+      s_u[j] = u;
+      s_r[j] = at32(rsdf, u);
+      s_m[j] = at32(emask, u);
+    }
+  }
+  __syncthreads();
+  while (n > 0 && n <= cap) {
+    int32_t *__restrict__ nxt = (level & 1) ? fr0 : fr1;
+    int32_t u[PER];
+    unsigned m[PER];
+    double pr[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int pc = min(p0 + q, n - 1);
+      u[q] = s_u[pc];
+      m[q] = p0 + q < n ? (unsigned)s_m[pc] : 0u;
+      pr[q] = s_r[pc];
+    }
+    // claim
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int d = 0; d < 6; ++d)
+        if (m[q] >> d & 1)
+          atomicMin(&at32(claim, u[q] + dir_step(d, ny, nz)), key_of(q0, p0 + q, d));
+    __syncthreads();
+    // count
+    unsigned won[PER];
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      // one request per enterable neighbour: a single CU's memory pipeline,
+      // not latency, bounds a level
+      int32_t k[6];
+#pragma unroll
+      for (int d = 0; d < 6; ++d)
+        k[d] = (m[q] >> d & 1) ? __hip_atomic_load(&at32(claim, u[q] + dir_step(d, ny, nz)),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : -1;
+      unsigned wm = 0;
+#pragma unroll
+      for (int d = 0; d < 6; ++d)
+        if ((m[q] >> d & 1) && k[d] == key_of(q0, p0 + q, d)) wm |= 1u << d;
+      won[q] = wm;
+      cnt += __popc(wm);
+    }
+    // exclusive scan of the threads' counts (thread order = position order)
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int v = __shfl_up(inc, o, kWave);
+      if (l >= o) inc += v;
+    }
+    if (l == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < kSmallThreads / kWave; ++i) {
+      if (i < w) before += wsum[i];
+      total += wsum[i];
+    }
+    int out = before + inc - cnt;
+    if (total <= cap) {
+      // emit: the children, in key order, with their parents' rsdf ...
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+          if (!(won[q] >> d & 1)) continue;
+          const int32_t ch = u[q] + dir_step(d, ny, nz);
+          s_u[out] = ch;
+          s_r[out] = pr[q];
+          s_d[out] = (uint8_t)d;
+          at32(nxt, out) = ch;
+          ++out;
+        }
+      __syncthreads();
+      // ... then each child's normal, rsdf and mask
+      if (total > 0) {
+        int32_t ch[PER];
+        int dd[PER];
+        double n0[PER], n1[PER], n2[PER];
+        unsigned em[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+          const int j = min(t + k * kSmallThreads, total - 1);
+          ch[k] = s_u[j];
+          dd[k] = s_d[j];
+          const double *nc = nrm + 3 * (int64_t)ch[k];
+          n0[k] = nc[0];
+          n1[k] = nc[1];
+          n2[k] = nc[2];
+          em[k] = at32(emask, ch[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+          const int j = t + k * kSmallThreads;
+          if (j >= total) continue;
+          const double r = __dadd_rn(s_r[j], step_dot(dd[k], dx, n0[k], n1[k], n2[k]));
+          s_r[j] = r;
+          s_m[j] = (uint8_t)em[k];
+          at32(hops, ch[k]) = level + 1;
+          at32(rsdf, ch[k]) = r;
+        }
+      }
+    } else {
+      // the next frontier outgrows the workgroup: write it for the multi-block kernels
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (!won[q]) continue;
+        for (int d = 0; d < 6; ++d) {
+          if (!(won[q] >> d & 1)) continue;
+          const int32_t ch = u[q] + dir_step(d, ny, nz);
+          const double *nc = nrm + 3 * (int64_t)ch;
+          at32(nxt, out++) = ch;
+          at32(hops, ch) = level + 1;
+          at32(rsdf, ch) = __dadd_rn(pr[q], step_dot(d, dx, nc[0], nc[1], nc[2]));
+        }
+      }
+    }
+    if (t == 0) nf[(level + 1) & 1] = total;
+    q0 += n;
+    ++level;
+    n = total;
+    // the next level rereads s_u, s_r, s_m written above and rewrites wsum
+    __syncthreads();
+  }
+  if (t == 0) {
+    nf[2] = level;
+    nf[3] = q0;
   }
 }
 
@@ -286,37 +508,71 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   hipLaunchKernelGGL(k_init, dim3(kGrid), dim3(kBlock), 0, st, rsdf, hops,
                      w.claim, V);
   hipLaunchKernelGGL(k_seed, dim3(mmn_blocks(n_seeds > 0 ? n_seeds : 1, kBlock)),
-                     dim3(kBlock), 0, st, rsdf, hops, w.front0, w.nf, seeds,
+                     dim3(kBlock), 0, st, rsdf, hops, w.claim, w.front0, w.nf, seeds,
                      n_seeds);
   int rc = mmn_check_launch("mmn_rsdf_bfs(init)");
   if (rc) return rc;
   if (n_seeds == 0) return 0;
+  hipLaunchKernelGGL(k_emask, dim3(kGrid), dim3(kBlock), 0, st, w.emask, normals, labels,
+                     nx, ny, nz);
 
+  // Positions per thread of the single-workgroup loop (MMN_BFS_PER: 2, 4, 8,
+  // or 0 for the multi-block kernels at every level).
+  static const int per = [] {
+    const char *e = getenv("MMN_BFS_PER");
+    const int v = e ? atoi(e) : 2;
+    return v == 0 || v == 4 || v == 8 ? v : 2;
+  }();
+  // Largest frontier the single workgroup takes (MMN_BFS_CAP, default 1024,
+  // at most 1024 * per).  Its levels are bound by one CU's memory pipeline
+  // (about 1.5 ns per scattered request, some 14 requests per entry), the
+  // multi-block kernels' by four launches (about 17 us a level): on a
+  // 1.5M-voxel field any cap from 512 to 1536 gives the same time.
+  static const int cap_env = [] {
+    const char *e = getenv("MMN_BFS_CAP");
+    return e ? atoi(e) : 1024;
+  }();
+  const int small_cap = std::max(1, std::min(cap_env, kSmallThreads * per));
   int32_t *fr[2] = {w.front0, w.front1};
-  for (int level = 0; level <= V;) {
+  int32_t n_cur = n_seeds;
+  for (int level = 0; level <= V && n_cur > 0;) {
+    if (n_cur <= small_cap) {
+      // small frontiers: one workgroup runs levels until it outgrows it
+      auto small = per == 8 ? k_bfs_small<8> : per == 4 ? k_bfs_small<4> : k_bfs_small<2>;
+      hipLaunchKernelGGL(small, dim3(1), dim3(kSmallThreads), 0, st, w.front0, w.front1, w.nf,
+                         w.claim, hops, rsdf, normals, w.emask, dx, level, small_cap, ny, nz);
+      rc = mmn_check_launch("mmn_rsdf_bfs(small)");
+      if (rc) return rc;
+      int32_t st4[4] = {0, 0, 0, 0};
+      hipError_t e = hipMemcpyAsync(st4, w.nf, sizeof(st4), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) {
+        mmn_set_error("mmn_rsdf_bfs: %s", hipGetErrorString(e));
+        return (int)e;
+      }
+      level = st4[2];
+      n_cur = st4[level & 1];
+      continue;
+    }
     for (int b = 0; b < kLevelsPerBatch; ++b, ++level) {
       const int cur = level & 1, nxt = cur ^ 1;
-      hipLaunchKernelGGL(k_claim, dim3(kGrid), dim3(kBlock), 0, st, fr[cur],
-                         w.nf + cur, w.claim, hops, normals, labels, nx, ny, nz);
-      hipLaunchKernelGGL(k_count, dim3(kGrid), dim3(kBlock), 0, st, fr[cur],
-                         w.nf + cur, w.claim, hops, w.pmask, w.bcnt, nx, ny, nz);
-      hipLaunchKernelGGL(k_scan, dim3(1), dim3(kGrid), 0, st, w.bcnt, w.boff,
-                         w.nf + nxt);
-      hipLaunchKernelGGL(k_emit, dim3(kGrid), dim3(kBlock), 0, st, fr[cur],
-                         w.nf + cur, w.pmask, w.boff, fr[nxt], hops, rsdf,
-                         normals, dx, level, nx, ny, nz);
+      hipLaunchKernelGGL(k_claim, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
+                         w.claim, w.emask, ny, nz);
+      hipLaunchKernelGGL(k_count, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
+                         w.claim, w.emask, w.pmask, w.bcnt, ny, nz);
+      hipLaunchKernelGGL(k_scan, dim3(1), dim3(kGrid), 0, st, w.bcnt, w.boff, w.nf, cur);
+      hipLaunchKernelGGL(k_emit, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf + cur,
+                         w.pmask, w.boff, fr[nxt], hops, rsdf, normals, dx, level, ny, nz);
     }
     rc = mmn_check_launch("mmn_rsdf_bfs(level)");
     if (rc) return rc;
-    int32_t n_next = 0;
-    hipError_t e = hipMemcpyAsync(&n_next, w.nf + (level & 1), sizeof(int32_t),
+    hipError_t e = hipMemcpyAsync(&n_cur, w.nf + (level & 1), sizeof(int32_t),
                                   hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
       mmn_set_error("mmn_rsdf_bfs: %s", hipGetErrorString(e));
       return (int)e;
     }
-    if (n_next == 0) break;
   }
   return 0;
 }

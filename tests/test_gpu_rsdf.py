@@ -39,6 +39,11 @@ def coords_of(shape, pitch):
     ((20, 24, 16), 0.35, 2, True),
     ((16, 16, 16), 0.6, 3, False),
     ((40, 33, 27), 0.3, 4, True),
+    # frontiers above one workgroup's 4096 entries (single seed, peak 9225): the
+    # single-workgroup and multi-block level paths hand over both ways
+    ((96, 96, 64), 0.9, 5, False),
+    # 8387 seeds: the first level already runs on the multi-block path
+    ((64, 64, 64), 0.25, 6, False),
 ])
 def test_bfs_bit_exact_vs_fifo(shape, density, seed, blobs, oracle):
     n, labeled, nlab = random_field(shape, density, seed, blobs)
