@@ -424,16 +424,19 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
 // df / c, the slow factor exp(-+j x_j delta) of each phasor is a power
 // series in xi with argument y_j xi, |y_j| = |x_j| rho <= a:
 //   H_i(d0 + delta) = exp(-j k_c delta) sum_k h_k xi^k,
-//   h_k = sum_j k_j s_ij exp(-j k_j d0) (-j y_j)^k / k!,
+//   h_k = sum_j k_j s_ij exp(-j k_j d0) (-j)^k w_k(y_j),
 //   sum_{v in b} g_v exp(+j k_j d_iv)
 //       = exp(+j k_j d0) sum_k eps_k j^k J_k(y_j) mu_k,
 //   mu_k = sum_{v in b} g_v exp(+j k_c delta_v) T_k(xi_v).
-// The position term keeps K = 16 power-series terms (remainder below
-// 2^16/16! e^2 = 2.3e-8 of sum_j |.| for a <= 2); the adjoint uses the
-// Chebyshev form exp(+j y xi) = sum_k eps_k j^k J_k(y) T_k(xi) (Jacobi-Anger)
-// with 12 terms (remainder ~ 2 J_12(2) = 4e-9), its moments taken against
-// T_k(xi_v).  The host sizes the groups for a <= 2; the factors y_j^k / k! and
-// eps_k J_k(y_j) depend on j only and come from a table (scalar loads).
+// where exp(-j y xi) = sum_k (-j)^k w_k(y) xi^k is the degree-11 polynomial
+// of the Jacobi-Anger sum below turned into powers of xi (w_k(y) = y^k / k!
+// would be the power series, which needs 16 terms for the same accuracy).
+// The adjoint uses the Chebyshev form exp(+j y xi) = sum_k eps_k j^k J_k(y)
+// T_k(xi) (Jacobi-Anger) directly, its moments taken against T_k(xi_v).  Both
+// keep 12 terms: the remainder is below 2 sum_{k >= 12} J_k(2) = 4.2e-9 of
+// sum_j |.| for a <= 2.  The host sizes the groups for a <= 2; the factors
+// w_k(y_j) and eps_k J_k(y_j) depend on j only and come from a table (scalar
+// loads).
 // k_bwd_taylor gives each lane one row (64 rows per wave) and walks the
 // groups of one chunk: per (row, group) the K coefficients h_k from the
 // row's samples, per voxel (wave-uniform, through scalar loads) the FP64
@@ -442,26 +445,26 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
 // row's F sample gradients.  Nothing is shared between lanes: no atomics, no
 // cross-lane sums; the per-chunk partials are reduced in a fixed order.
 // --------------------------------------------------------------------------
-constexpr int kTaylorK = 16;      // power-series terms of the position term
+constexpr int kTaylorK = 12;      // polynomial terms of the position term
 constexpr int kChebK = 12;        // Chebyshev terms of the adjoint
+static_assert(kTaylorK <= kChebK, "the position polynomial is built from the Chebyshev terms");
 constexpr double kTaylorA = 2.0;  // bound on |y_j xi| the groups are sized for
 constexpr int kTJ = 32;           // samples per FP64 re-anchor of exp(-+j k_j d0)
 
-// Per sample j: y_j^k / k! (k < kTaylorK) and eps_k J_k(y_j) (k < kChebK,
-// eps_0 = 1, eps_k = 2), y_j = 2 pi (j - t_c) df rho / c, in FP64 from the
-// power series of J_k (|y| <= 2: 20 terms leave < 1e-20).
+// Per sample j, y_j = 2 pi (j - t_c) df rho / c: eps_k J_k(y_j) (k < kChebK,
+// eps_0 = 1, eps_k = 2) in FP64 from the power series of J_k (|y| <= 2: 20
+// terms leave < 1e-20), and the power-basis coefficients of the same sum,
+//   w_k = sum_{m >= k, m = k mod 2} (-1)^((m - k) / 2) eps_m J_m(y_j) t_mk,
+// T_m(xi) = sum_k t_mk xi^k (k < kTaylorK), so that
+// sum_k (-j)^k w_k xi^k = sum_m eps_m (-j)^m J_m(y_j) T_m(xi).
 __global__ __launch_bounds__(kBlock) void k_bt_tables(float *__restrict__ tay,
                                                       float *__restrict__ bes, int F,
                                                       double ystep, double tc) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= F) return;
   const double y = ((double)j - tc) * ystep;
-  double t = 1.0;
-  for (int k = 0; k < kTaylorK; ++k) {
-    tay[(size_t)j * kTaylorK + k] = (float)t;
-    t *= y / (double)(k + 1);
-  }
   const double h = 0.5 * y, h2 = h * h;
+  double ej[kChebK];
   double hk = 1.0, kf = 1.0;  // (y/2)^k, k!
   for (int k = 0; k < kChebK; ++k) {
     double term = hk / kf, sum = 0.0;
@@ -469,10 +472,25 @@ __global__ __launch_bounds__(kBlock) void k_bt_tables(float *__restrict__ tay,
       sum += term;
       term *= -h2 / ((double)(m + 1) * (double)(m + 1 + k));
     }
-    bes[(size_t)j * kChebK + k] = (float)((k ? 2.0 : 1.0) * sum);
+    ej[k] = (k ? 2.0 : 1.0) * sum;
+    bes[(size_t)j * kChebK + k] = (float)ej[k];
     hk *= h;
     kf *= (double)(k + 1);
   }
+  // coefficients of T_{m-1} and T_m: T_1 = xi, then T_{m+1} = 2 xi T_m - T_{m-1}
+  double tp[kTaylorK] = {}, tm[kTaylorK] = {}, w[kTaylorK] = {};
+  tm[0] = 1.0;
+  for (int m = 0; m < kTaylorK; ++m) {
+    for (int k = m & 1; k <= m; k += 2) w[k] += (((m - k) >> 1) & 1 ? -ej[m] : ej[m]) * tm[k];
+    double tn[kTaylorK];
+    for (int k = 0; k < kTaylorK; ++k)
+      tn[k] = m == 0 ? (k == 1 ? 1.0 : 0.0) : (k ? 2.0 * tm[k - 1] : 0.0) - tp[k];
+    for (int k = 0; k < kTaylorK; ++k) {
+      tp[k] = tm[k];
+      tm[k] = tn[k];
+    }
+  }
+  for (int k = 0; k < kTaylorK; ++k) tay[(size_t)j * kTaylorK + k] = (float)w[k];
 }
 
 // Brick of each listed voxel (the sort key) and its list position.
@@ -607,6 +625,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
     const float *__restrict__ bes, double f0c, double dfc, double tc, double rho, int A,
     int R, int F, int is_ti) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  // a wave whose 64 rows all lie past A has nothing to do (no barriers here)
+  if ((blockIdx.x * 4 + wv) * 64 >= A) return;
   const int i_raw = (blockIdx.x * 4 + wv) * 64 + (int)(threadIdx.x % 64);
   const bool live = i_raw < A;
   const int i = live ? i_raw : A - 1;
@@ -643,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
     float sr, si;  // exp(-j k_j d0) steps by exp(-j 2 pi df d0 / c)
     phasor_quarter(4.0 * d0 * dfc, sr, si);
 
-    // h_k = sum_j k_j s_ij exp(-j k_j d0) (-j)^k y_j^k / k!
+    // h_k = sum_j k_j s_ij exp(-j k_j d0) (-j)^k w_k(y_j)
     float hr[K], hi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) hr[k] = hi[k] = 0.0f;
@@ -707,7 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
         gy = fmaf(w1, py - rf.y, fmaf(w2, qy - rf.y, gy));
         gz = fmaf(w1, pz - rf.z, fmaf(w2, qz - rf.z, gz));
       } else {
-        const float w1 = 2.0f * tt * y1;
+        const float w1 = tt * y1;  // the factor 2 of the two legs is applied at the end
         gx = fmaf(w1, px - rf.x, gx);
         gy = fmaf(w1, py - rf.y, gy);
         gz = fmaf(w1, pz - rf.z, gz);
@@ -735,7 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
       voxel(recd[v], recf[v], gv.x, gv.y);
     }
 
-    // S_ij += exp(+j k_j d0) sum_k (j y_j)^k / k! mu_k  (conjugate steps)
+    // S_ij += exp(+j k_j d0) sum_k eps_k j^k J_k(y_j) mu_k  (conjugate steps)
     float2 *__restrict__ srow = st + (size_t)c * F * A + i;
     for (int j0 = 0; j0 < F; j0 += kTJ) {
       float er, ei;
@@ -772,29 +792,58 @@ __global__ __launch_bounds__(kBlock) void k_bwd_taylor(
     }
   }
   if (live) {
+    const float two_legs = BI ? 1.0f : 2.0f;  // exact: a power of two
     float *__restrict__ o = pp + ((size_t)c * A + i) * 3;
-    o[0] = gx;
-    o[1] = gy;
-    o[2] = gz;
+    o[0] = two_legs * gx;
+    o[1] = two_legs * gy;
+    o[2] = two_legs * gz;
   }
 }
 
 // grad_meas[i][j] = scale sum_c st[c][j][i] (FP64, fixed order).
+// grad[i][j] = scale * sum_q st[q][j][i], q in order (FP64).  A block takes
+// a tile of 64 rows x 8 samples: the chunk partials are read along i
+// (coalesced; each thread keeps two independent sums in flight), the tile is
+// transposed through LDS and written along j.
+constexpr int kRedI = 64, kRedJ = 8;
 __global__ __launch_bounds__(kBlock) void k_bt_reduce_meas(float2 *__restrict__ grad,
                                                            const float2 *__restrict__ st,
                                                            int chunks, int A, int F,
                                                            double scale) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  constexpr int kRows = kBlock / kRedI;  // 4 sample rows of the tile per pass
+  constexpr int kPer = kRedJ / kRows;    // 2 samples per thread
+  __shared__ float2 tile[kRedJ][kRedI + 1];
+  const int i0 = blockIdx.x * kRedI, j0 = blockIdx.y * kRedJ;
   const int64_t AF = (int64_t)A * F;
-  if (t >= AF) return;
-  const int i = (int)(t / F), j = (int)(t % F);
-  double a = 0.0, b = 0.0;
-  for (int q = 0; q < chunks; ++q) {
-    const float2 v = st[(size_t)q * AF + (size_t)j * A + i];
-    a += v.x;
-    b += v.y;
+  const int ii = threadIdx.x % kRedI, jb = threadIdx.x / kRedI;
+  const int i = i0 + ii;
+  double a[kPer] = {}, b[kPer] = {};
+  bool ok[kPer];
+  const float2 *src[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int j = j0 + jb + k * kRows;
+    ok[k] = i < A && j < F;
+    src[k] = st + (size_t)(ok[k] ? j : 0) * A + (ok[k] ? i : 0);
   }
-  grad[t] = make_float2((float)(scale * a), (float)(scale * b));
+  for (int q = 0; q < chunks; ++q) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      if (!ok[k]) continue;
+      const float2 v = src[k][(size_t)q * AF];
+      a[k] += v.x;
+      b[k] += v.y;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    tile[jb + k * kRows][ii] = make_float2((float)(scale * a[k]), (float)(scale * b[k]));
+  __syncthreads();
+  for (int e = threadIdx.x; e < kRedI * kRedJ; e += kBlock) {
+    const int jj = e % kRedJ, iq = e / kRedJ;
+    const int io = i0 + iq, j = j0 + jj;
+    if (io < A && j < F) grad[(size_t)io * F + j] = tile[jj][iq];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_bt_reduce_ant(float *__restrict__ grad,
@@ -1159,7 +1208,10 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
   hipLaunchKernelGGL(k_bt_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)), dim3(kBlock), 0, st,
                      w.mt, (const float2 *)meas, A, F, (float)(two_pi * f0 * inv_c),
                      (float)(two_pi * df * inv_c));
-  // chunks of groups: enough waves for ~16 per CU, at most the workspace's
+  // chunks of groups: ~8 blocks (32 waves) per CU, at most the workspace's.
+  // Measured on MI355X: 128 chunks at config 2 (A = 3600) and 172 at config
+  // 4 (A = 1600) beat 64 / 115 by 5 % / 2 % (smaller tails; the chunk sums
+  // cost ~0.1 ms).
   bool bi = false;
   {
     // any RX offset nonzero -> both legs (host copy of a few floats)
@@ -1179,7 +1231,7 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   (void)hipGetLastError();
   const int64_t rowblocks = (A + 255) / 256;
-  int64_t ch = (4 * (int64_t)cus + rowblocks - 1) / rowblocks;
+  int64_t ch = (8 * (int64_t)cus + rowblocks - 1) / rowblocks;
   const int forced = env_int("MMN_PROFB_CHUNKS", 0);
   if (forced > 0) ch = forced;
   if (ch > sh.ngroups) ch = sh.ngroups;
@@ -1195,7 +1247,7 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
                      rx_off, w.mt, w.tay, w.bes, f0 * inv_c, df * inv_c, tc, rho, A, num_rx, F,
                      is_ti_radar);
   if (grad_meas)
-    hipLaunchKernelGGL(k_bt_reduce_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)),
+    hipLaunchKernelGGL(k_bt_reduce_meas, dim3((A + kRedI - 1) / kRedI, (F + kRedJ - 1) / kRedJ),
                        dim3(kBlock), 0, st, (float2 *)grad_meas, w.st, (int)ch, A, F, scale);
   if (grad_ant)
     hipLaunchKernelGGL(k_bt_reduce_ant, dim3(mmn_blocks(3 * A, kBlock)), dim3(kBlock), 0, st,

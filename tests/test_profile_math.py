@@ -8,10 +8,14 @@ FP64 (NumPy/SciPy), independently of the kernels:
   c / (64 (F-1) df) apart with that cubic reproduces the exact sum to the
   stated bound (Lagrange remainder of the fastest tone, 0.0234 (pi/64)^4
   = 1.4e-7 of sum_j |s_j|);
-* the grouped-series backward's truncations: 16 power-series terms of
-  exp(-j y xi) and 12 Chebyshev terms of exp(+j y xi) (Jacobi-Anger, with
-  scipy's Bessel J_k) stay below 2.3e-8 and 1e-8 for |y| <= 2, |xi| <= 1,
-  and the series k_bt_tables evaluates for J_k agrees with scipy.
+* the grouped-series backward's truncations: 12 Chebyshev terms of
+  exp(+j y xi) (Jacobi-Anger, with scipy's Bessel J_k) stay below 1e-8 for
+  |y| <= 2, |xi| <= 1, and the series k_bt_tables evaluates for J_k agrees
+  with scipy;
+* the position term's table: the power-basis coefficients k_bt_tables forms
+  from the 12 Chebyshev terms (w_k = sum_m (-1)^((m-k)/2) eps_m J_m(y) t_mk)
+  give exp(-j y xi) within 1e-8 in FP64 and 1e-7 once stored as float32, as
+  the 16-term power series y^k / k! does, and reduce to it as y -> 0.
 """
 import math
 
@@ -65,9 +69,6 @@ def test_profile_interpolation_error_bound():
 def test_series_truncations():
     y = np.linspace(-2.0, 2.0, 41)[:, None]
     xi = np.linspace(-1.0, 1.0, 41)[None, :]
-    # power series of exp(-j y xi), 16 terms
-    ps = sum((-1j * y) ** k / math.factorial(k) * xi ** k for k in range(16))
-    assert np.max(np.abs(ps - np.exp(-1j * y * xi))) <= 2.3e-8
     # Jacobi-Anger: exp(+j y cos t) = J0 + 2 sum_k j^k J_k(y) cos(k t), 12 terms
     T = [np.ones_like(xi), xi]
     for k in range(2, 12):
@@ -88,3 +89,46 @@ def test_bessel_series_of_the_tables():
                 tot += term
                 term *= -h * h / ((mm + 1) * (mm + 1 + k))
             assert abs(tot - jv(k, y)) <= 1e-15
+
+
+def power_basis_table(y, K=12):
+    """The position term's table as k_bt_tables forms it: eps_m J_m(y) from
+    20 terms of the Bessel series, then the power-basis coefficients of
+    sum_m eps_m (-j)^m J_m(y) T_m(xi) without the (-j)^k factor."""
+    h = y / 2
+    ej = []
+    for k in range(K):
+        term, tot = h ** k / math.factorial(k), 0.0
+        for mm in range(20):
+            tot += term
+            term *= -h * h / ((mm + 1) * (mm + 1 + k))
+        ej.append((2.0 if k else 1.0) * tot)
+    # T_m coefficients from numpy's Chebyshev-to-power conversion
+    w = np.zeros(K)
+    for m in range(K):
+        t = np.polynomial.chebyshev.cheb2poly([0] * m + [1])
+        for k in range(m % 2, m + 1, 2):
+            w[k] += (-1) ** ((m - k) // 2) * ej[m] * t[k]
+    return w
+
+
+def test_position_table_power_basis():
+    xi = np.linspace(-1.0, 1.0, 201)
+    worst64 = worst32 = 0.0
+    for y in np.linspace(-2.0, 2.0, 41):
+        w = power_basis_table(y)
+        exact = np.exp(-1j * y * xi)
+        for ww, f32 in ((w, False), (w.astype(np.float32).astype(np.float64), True)):
+            P = sum((-1j) ** k * ww[k] * xi ** k for k in range(len(ww)))
+            err = np.max(np.abs(P - exact))
+            if f32:
+                worst32 = max(worst32, err)
+            else:
+                worst64 = max(worst64, err)
+    assert worst64 <= 1e-8
+    assert worst32 <= 1e-7
+    # small y: the coefficients approach the power series y^k / k!
+    y = 1e-3
+    w = power_basis_table(y)
+    taylor = np.array([y ** k / math.factorial(k) for k in range(12)])
+    assert np.max(np.abs(w - taylor)) <= 1e-15
