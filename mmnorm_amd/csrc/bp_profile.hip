@@ -615,8 +615,10 @@ __global__ __launch_bounds__(kBlock) void k_bt_meas(float2 *__restrict__ mt,
 
 // Lane = row.  BI: some row has a nonzero RX offset (then both legs are
 // evaluated for every row; a monostatic row's second leg is its first).
+// Five waves per SIMD (94 VGPRs, no spills) instead of the four that 112
+// VGPRs allow: 5 % faster at config 4.
 template <int K, int KC, bool BI>
-__global__ __launch_bounds__(kBlock) void k_bwd_taylor(
+__global__ __launch_bounds__(kBlock, 5) void k_bwd_taylor(
     float2 *__restrict__ st, float *__restrict__ pp, const double4 *__restrict__ recd,
     const float4 *__restrict__ recf, const float2 *__restrict__ recg,
     const double4 *__restrict__ groupc, int64_t G, const int32_t *__restrict__ goff,
@@ -912,7 +914,7 @@ struct BtWs {
 // Most chunks (partial copies of the sample gradients) a launch may use.
 int bt_chunks_max(int A, int F) {
   int64_t c = ((int64_t)1 << 30) / ((int64_t)A * F * 8);
-  if (c > 256) c = 256;
+  if (c > 512) c = 512;
   return c < 1 ? 1 : (int)c;
 }
 
@@ -1208,10 +1210,11 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
   hipLaunchKernelGGL(k_bt_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)), dim3(kBlock), 0, st,
                      w.mt, (const float2 *)meas, A, F, (float)(two_pi * f0 * inv_c),
                      (float)(two_pi * df * inv_c));
-  // chunks of groups: ~8 blocks (32 waves) per CU, at most the workspace's.
-  // Measured on MI355X: 128 chunks at config 2 (A = 3600) and 172 at config
-  // 4 (A = 1600) beat 64 / 115 by 5 % / 2 % (smaller tails; the chunk sums
-  // cost ~0.1 ms).
+  // chunks of groups: ~16 blocks (64 waves) per CU, at most the workspace's
+  // (1 GiB of chunk partials, 512 chunks).  Short waves keep the tail small:
+  // measured on MI355X, 128 chunks at config 2 (A = 3600; the workspace's
+  // limit is 145) and 343 (one group each) at config 4 (A = 1600) beat
+  // 64 / 115 by 5 % / 10 %; the chunk sums cost ~0.2 ms.
   bool bi = false;
   {
     // any RX offset nonzero -> both legs (host copy of a few floats)
@@ -1231,7 +1234,7 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   (void)hipGetLastError();
   const int64_t rowblocks = (A + 255) / 256;
-  int64_t ch = (8 * (int64_t)cus + rowblocks - 1) / rowblocks;
+  int64_t ch = (16 * (int64_t)cus + rowblocks - 1) / rowblocks;
   const int forced = env_int("MMN_PROFB_CHUNKS", 0);
   if (forced > 0) ch = forced;
   if (ch > sh.ngroups) ch = sh.ngroups;
