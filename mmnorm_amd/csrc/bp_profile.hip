@@ -828,7 +828,24 @@ __global__ __launch_bounds__(kBlock) void k_bt_reduce_meas(float2 *__restrict__ 
     ok[k] = i < A && j < F;
     src[k] = st + (size_t)(ok[k] ? j : 0) * A + (ok[k] ? i : 0);
   }
-  for (int q = 0; q < chunks; ++q) {
+  // four chunks' loads in flight per sample, added in chunk order
+  constexpr int kU = 4;
+  int q = 0;
+  for (; q + kU <= chunks; q += kU) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      if (!ok[k]) continue;
+      float2 v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) v[u] = src[k][(size_t)(q + u) * AF];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        a[k] += v[u].x;
+        b[k] += v[u].y;
+      }
+    }
+  }
+  for (; q < chunks; ++q) {
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       if (!ok[k]) continue;
@@ -854,7 +871,17 @@ __global__ __launch_bounds__(kBlock) void k_bt_reduce_ant(float *__restrict__ gr
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= 3 * A) return;
   double s = 0.0;
-  for (int q = 0; q < chunks; ++q) s += pp[(size_t)q * A * 3 + t];
+  // eight chunks' loads in flight, added in chunk order
+  constexpr int kU = 8;
+  int q = 0;
+  for (; q + kU <= chunks; q += kU) {
+    float v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) v[u] = pp[(size_t)(q + u) * A * 3 + t];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) s += v[u];
+  }
+  for (; q < chunks; ++q) s += pp[(size_t)q * A * 3 + t];
   grad[t] = (float)(scale * s);
 }
 
