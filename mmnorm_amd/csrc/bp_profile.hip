@@ -615,10 +615,11 @@ __global__ __launch_bounds__(kBlock) void k_bt_meas(float2 *__restrict__ mt,
 
 // Lane = row.  BI: some row has a nonzero RX offset (then both legs are
 // evaluated for every row; a monostatic row's second leg is its first).
-// Five waves per SIMD (94 VGPRs, no spills) instead of the four that 112
-// VGPRs allow: 5 % faster at config 4.
+// Monostatic: five waves per SIMD (94 VGPRs, no spills) instead of the four
+// that 112 VGPRs allow, 5 % faster at config 4.  Bistatic (both legs) keeps
+// four: at five it spills.
 template <int K, int KC, bool BI>
-__global__ __launch_bounds__(kBlock, 5) void k_bwd_taylor(
+__global__ __launch_bounds__(kBlock, BI ? 4 : 5) void k_bwd_taylor(
     float2 *__restrict__ st, float *__restrict__ pp, const double4 *__restrict__ recd,
     const float4 *__restrict__ recf, const float2 *__restrict__ recg,
     const double4 *__restrict__ groupc, int64_t G, const int32_t *__restrict__ goff,
