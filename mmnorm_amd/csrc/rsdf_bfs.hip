@@ -29,11 +29,13 @@
 // k_bfs_small, a single 1024-thread workgroup that loops over levels with
 // the frontier, its rsdf and its masks in LDS: no launches, and the
 // frontier's data never goes through global memory, but one CU issues all
-// of a level's scattered requests.  Larger frontiers run as four kernels
+// of a level's scattered requests.  Larger frontiers run as two kernels
 // per level over kGrid blocks, each block owning one contiguous chunk of
-// the frontier (block order is position order); the host enqueues those
-// levels in batches, reads the frontier size back once per batch, and
-// hands back to k_bfs_small when the frontier shrinks.
+// the frontier (block order is position order): k_count and k_emit_claim
+// (each block sums the counts before it for its offset; each child offers
+// the next level's keys at once).  The host enqueues those levels in
+// batches, reads the frontier size back once per batch, and hands back to
+// k_bfs_small when the frontier shrinks.
 #include <stdlib.h>
 
 #include <algorithm>
@@ -55,9 +57,9 @@ struct Ws {
   uint8_t *pmask;   // [V]  won mask of frontier position p (bit d)
   uint8_t *emask;   // [V]  enterable neighbours of each voxel (bit d)
   int32_t *bcnt;    // [kGrid]
-  int32_t *boff;    // [kGrid]
-  int32_t *nf;      // [4]  frontier sizes by level parity; the level k_bfs_small
-                    //      stopped at; queue index of the current level's first entry
+  int32_t *nf;      // [8]  [0,1] frontier sizes by level parity; [2] the level
+                    //      k_bfs_small stopped at; [4,5] queue index of a level's
+                    //      first entry, by level parity
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -75,8 +77,7 @@ size_t ws_layout(int64_t V, Ws *w, char *base) {
   char *pm = take(V);
   char *em = take(V);
   char *bc = take(sizeof(int32_t) * kGrid);
-  char *bo = take(sizeof(int32_t) * kGrid);
-  char *nf = take(sizeof(int32_t) * 4);
+  char *nf = take(sizeof(int32_t) * 8);
   if (w) {
     w->claim = (int32_t *)c;
     w->front0 = (int32_t *)f0;
@@ -84,7 +85,6 @@ size_t ws_layout(int64_t V, Ws *w, char *base) {
     w->pmask = (uint8_t *)pm;
     w->emask = (uint8_t *)em;
     w->bcnt = (int32_t *)bc;
-    w->boff = (int32_t *)bo;
     w->nf = (int32_t *)nf;
   }
   return off;
@@ -183,7 +183,8 @@ __global__ void k_seed(double *__restrict__ rsdf, int32_t *__restrict__ hops,
     nf[0] = n;
     nf[1] = 0;
     nf[2] = 0;
-    nf[3] = 0;
+    nf[4] = 0;
+    nf[5] = 0;
   }
 }
 
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_claim(
     int32_t *__restrict__ claim, const uint8_t *__restrict__ emask, int ny, int nz) {
   int start, end;
   chunk_of(nf[cur], start, end);
-  const int32_t q0 = nf[3];
+  const int32_t q0 = nf[4 + cur];
   for (int p = start + threadIdx.x; p < end; p += kBlock) {
     const int32_t u = front[p];
     const unsigned m = at32(emask, u);
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_count(
   __shared__ int sh[kBlock / kWave];
   int start, end;
   chunk_of(nf[cur], start, end);
-  const int32_t q0 = nf[3];
+  const int32_t q0 = nf[4 + cur];
   int cnt = 0;
   for (int p = start + threadIdx.x; p < end; p += kBlock) {
     const int32_t u = front[p];
@@ -237,41 +238,43 @@ __global__ __launch_bounds__(kBlock) void k_count(
   if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
 }
 
-// One block of kGrid threads: exclusive scan of the per-block counts; writes
-// the next frontier size and advances the queue index by this level's size.
-__global__ __launch_bounds__(kGrid) void k_scan(const int32_t *__restrict__ bcnt,
-                                                int32_t *__restrict__ boff,
-                                                int32_t *__restrict__ nf, int cur) {
-  __shared__ int sh[kGrid / kWave];
-  const int t = threadIdx.x;
-  const int v = bcnt[t];
-  int inc = v;
-  for (int o = 1; o < kWave; o <<= 1) {
-    const int n = __shfl_up(inc, o, kWave);
-    if ((t % kWave) >= o) inc += n;
-  }
-  const int w = t / kWave;
-  if ((t % kWave) == kWave - 1) sh[w] = inc;
-  __syncthreads();
-  int before = 0;
-  for (int i = 0; i < w; ++i) before += sh[i];
-  boff[t] = before + inc - v;
-  if (t == kGrid - 1) {
-    nf[3] += nf[cur];
-    nf[cur ^ 1] = before + inc;
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_emit(
-    const int32_t *__restrict__ front, const int32_t *__restrict__ nfp,
-    const uint8_t *__restrict__ pmask, const int32_t *__restrict__ boff,
+// Emit, and offer each child's keys of the next level at once.  Each block
+// finds its output offset by summing the counts of the blocks before it
+// (k_count wrote them; 4 KB), and block 0 writes the next level's size and
+// queue index into the other parity's slots (this launch reads only its
+// own).  The child at next-frontier position o has queue index q1 + o,
+// q1 = q0 + n.  Other blocks may still be emitting this level; a voxel they
+// reach holds a smaller key, so the early offers change nothing there.
+// (A "last block scans" variant of k_count needs a device-scope fence in
+// every block, which on this part made a level twice as slow.)
+__global__ __launch_bounds__(kBlock) void k_emit_claim(
+    const int32_t *__restrict__ front, int32_t *__restrict__ nf, int cur,
+    const uint8_t *__restrict__ pmask, const int32_t *__restrict__ bcnt,
     int32_t *__restrict__ next, int32_t *__restrict__ hops,
-    double *__restrict__ rsdf, const double *__restrict__ nrm, double dx,
+    double *__restrict__ rsdf, const double *__restrict__ nrm,
+    const uint8_t *__restrict__ emask, int32_t *__restrict__ claim, double dx,
     int level, int ny, int nz) {
   __shared__ int sh[kBlock / kWave];
+  const int n = nf[cur];
   int start, end;
-  chunk_of(*nfp, start, end);
-  int base = boff[blockIdx.x];
+  chunk_of(n, start, end);
+  const int32_t q1 = nf[4 + cur] + n;
+  int before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kGrid / kBlock; ++k) {
+    const int b = k * kBlock + (int)threadIdx.x;
+    const int v = bcnt[b];
+    all += v;
+    if (b < (int)blockIdx.x) before += v;
+  }
+  int base = block_sum(before, sh);
+  if (blockIdx.x == 0) {
+    const int total = block_sum(all, sh);
+    if (threadIdx.x == 0) {
+      nf[cur ^ 1] = total;
+      nf[4 + (cur ^ 1)] = q1;
+    }
+  }
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
   for (int r0 = start; r0 < end; r0 += kBlock) {
     const int p = r0 + threadIdx.x;
@@ -298,9 +301,14 @@ __global__ __launch_bounds__(kBlock) void k_emit(
         if (!(m & (1u << d))) continue;
         const int32_t ch = u + dir_step(d, ny, nz);
         const double *nc = nrm + 3 * (int64_t)ch;
-        next[out++] = ch;
+        const unsigned em = at32(emask, ch);
+        next[out] = ch;
         at32(hops, ch) = level + 1;
         at32(rsdf, ch) = __dadd_rn(prev, step_dot(d, dx, nc[0], nc[1], nc[2]));
+#pragma unroll
+        for (int e = 0; e < 6; ++e)
+          if (em >> e & 1) atomicMin(&at32(claim, ch + dir_step(e, ny, nz)), key_of(q1, out, e));
+        ++out;
       }
     }
     base += round_tot;
@@ -310,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(
 // Levels level0, level0 + 1, ... in one workgroup while the frontier has
 // 1..cap entries (cap <= 1024 PER).  Writes each level's next size to
 // nf[parity], and at exit the first level it did not run to nf[2] and that
-// level's queue index to nf[3].  The frontier (voxel, rsdf, enterable mask)
+// level's queue index to nf[4 + parity].  The frontier (voxel, rsdf, enterable mask)
 // stays in LDS between levels; thread t owns positions [PER t, PER t + PER)
 // in the claim, count and scan phases (so an exclusive scan over threads
 // keeps position order) and entries t, t + 1024, ... when the next frontier
@@ -338,7 +346,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_bfs_small(
   const int p0 = t * PER;
   int level = level0;
   int n = nf[level & 1];
-  int32_t q0 = nf[3];
+  int32_t q0 = nf[4 + (level & 1)];
   if (n > 0 && n <= cap) {
     const int32_t *__restrict__ cur = (level & 1) ? fr1 : fr0;
     for (int j = t; j < n; j += kSmallThreads) {
@@ -472,7 +480,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_bfs_small(
   }
   if (t == 0) {
     nf[2] = level;
-    nf[3] = q0;
+    nf[4 + (level & 1)] = q0;
   }
 }
 
@@ -526,8 +534,8 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   // Largest frontier the single workgroup takes (MMN_BFS_CAP, default 1024,
   // at most 1024 * per).  Its levels are bound by one CU's memory pipeline
   // (about 1.5 ns per scattered request, some 14 requests per entry), the
-  // multi-block kernels' by four launches (about 17 us a level): on a
-  // 1.5M-voxel field any cap from 512 to 1536 gives the same time.
+  // multi-block kernels' by two launches (about 10 us a level): on a
+  // 1.5M-voxel field caps of 512 and 1024 give the same time.
   static const int cap_env = [] {
     const char *e = getenv("MMN_BFS_CAP");
     return e ? atoi(e) : 1024;
@@ -554,15 +562,17 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
       n_cur = st4[level & 1];
       continue;
     }
+    // the batch's first level is claimed here; every emit claims the next
+    // (a level that was already claimed gets the same keys again: no change)
+    hipLaunchKernelGGL(k_claim, dim3(kGrid), dim3(kBlock), 0, st, fr[level & 1], w.nf,
+                       level & 1, w.claim, w.emask, ny, nz);
     for (int b = 0; b < kLevelsPerBatch; ++b, ++level) {
       const int cur = level & 1, nxt = cur ^ 1;
-      hipLaunchKernelGGL(k_claim, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
-                         w.claim, w.emask, ny, nz);
       hipLaunchKernelGGL(k_count, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
                          w.claim, w.emask, w.pmask, w.bcnt, ny, nz);
-      hipLaunchKernelGGL(k_scan, dim3(1), dim3(kGrid), 0, st, w.bcnt, w.boff, w.nf, cur);
-      hipLaunchKernelGGL(k_emit, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf + cur,
-                         w.pmask, w.boff, fr[nxt], hops, rsdf, normals, dx, level, ny, nz);
+      hipLaunchKernelGGL(k_emit_claim, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
+                         w.pmask, w.bcnt, fr[nxt], hops, rsdf, normals, w.emask, w.claim, dx,
+                         level, ny, nz);
     }
     rc = mmn_check_launch("mmn_rsdf_bfs(level)");
     if (rc) return rc;
