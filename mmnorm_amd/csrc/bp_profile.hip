@@ -88,7 +88,7 @@ int seg_cap(int rows) {
 // Per-row constants of k_bp_prof, 12 doubles: u offset, |p|^2, -2p,
 // |p+o|^2, -2(p+o), the distance-to-u scale (doubled for a monostatic row,
 // whose round trip is twice one leg), pad.
-constexpr int kRowC = 12;
+constexpr int kRowC = 16;
 
 struct ProfWs {
   double *dlo;   // [A]
@@ -161,8 +161,14 @@ __global__ __launch_bounds__(kBlock) void k_prof_rows(double *__restrict__ dlo,
   c[7] = -2.0 * qy;
   c[8] = -2.0 * qz;
   c[9] = two * inv_h;
-  c[10] = 0.0;
-  c[11] = 0.0;
+  // the row's flag and the moment's point p + o/2 (FP32, exact in FP64), so
+  // that the voxel loop reads one record per row
+  c[10] = mono ? 1.0 : 0.0;
+  c[11] = (double)__fadd_rn(ant[3 * i + 0], __fmul_rn(ox, 0.5f));
+  c[12] = (double)__fadd_rn(ant[3 * i + 1], __fmul_rn(oy, 0.5f));
+  c[13] = (double)__fadd_rn(ant[3 * i + 2], __fmul_rn(oz, 0.5f));
+  c[14] = 0.0;
+  c[15] = 0.0;
 }
 
 // One workgroup row of lanes per table row (blockIdx.y): the row's samples
@@ -272,9 +278,8 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
     float2 *__restrict__ img, float2 *__restrict__ mom, const float *__restrict__ xl,
     const float *__restrict__ yl, const float *__restrict__ zl,
     const int32_t *__restrict__ vidx, int64_t n, int nx, int ny, int nz,
-    const float *__restrict__ ant, const float *__restrict__ rxo,
     const double *__restrict__ rowc, const float4 *__restrict__ coef, float thf, int M,
-    int rows, int R, int rps) {
+    int rows, int rps) {
   float vx[VPT], vy[VPT], vz[VPT];
   int64_t out[VPT];
   bool ok[VPT];
@@ -322,11 +327,9 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
 
   // Row i's geometry for every voxel of the lane: the interval's two
   // coefficient vectors (fetched) and the fraction.
-  auto fetch = [&](int i, float (&f)[VPT], float4 (&a)[VPT], float4 (&b)[VPT]) {
-    const int rx = i % R;
-    const bool mono = (rxo[3 * rx + 0] == 0.0f) & (rxo[3 * rx + 1] == 0.0f) &
-                      (rxo[3 * rx + 2] == 0.0f);
-    const double *__restrict__ c = rowc + (size_t)kRowC * i;
+  auto fetch = [&](int i, const double *__restrict__ c, float (&f)[VPT], float4 (&a)[VPT],
+                   float4 (&b)[VPT]) {
+    const bool mono = c[10] != 0.0;
     const float4 *__restrict__ crow = coef + (size_t)i * M * 2;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
@@ -346,12 +349,9 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
   // Row i's contribution from its fetched coefficients: the cubic, the
   // carrier's remainder past the interval start (thf f turns), the block
   // sum and the vote moment.
-  auto finish = [&](int i, const float (&f)[VPT], const float4 (&a)[VPT],
+  auto finish = [&](const double *__restrict__ c, const float (&f)[VPT], const float4 (&a)[VPT],
                     const float4 (&b)[VPT], float (&Sr)[VPT], float (&Si)[VPT]) {
-    const int rx = i % R;
-    const float vax = ant[3 * i + 0] + rxo[3 * rx + 0] / 2.0f;
-    const float vay = ant[3 * i + 1] + rxo[3 * rx + 1] / 2.0f;
-    const float vaz = ant[3 * i + 2] + rxo[3 * rx + 2] / 2.0f;
+    const float vax = (float)c[11], vay = (float)c[12], vaz = (float)c[13];
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       const float qr = fmaf(fmaf(fmaf(b[k].z, f[k], b[k].x), f[k], a[k].z), f[k], a[k].x);
@@ -389,10 +389,21 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
 #pragma unroll
   for (int k = 0; k < VPT; ++k) Sr[k] = Si[k] = 0.0f;
   for (int i = row_lo; i < row_hi; ++i) {
+    // the row's constants, all in SGPRs before any is used: one scalar-load
+    // wait per row instead of one per use (the compiler otherwise sinks
+    // some loads past the first leg and waits twice)
+    double rc[14];
+    const double *__restrict__ cg = rowc + (size_t)kRowC * i;
+#pragma unroll
+    for (int k = 0; k < 14; ++k) rc[k] = cg[k];
+    asm volatile(""
+                 : "+s"(rc[0]), "+s"(rc[1]), "+s"(rc[2]), "+s"(rc[3]), "+s"(rc[4]),
+                   "+s"(rc[5]), "+s"(rc[6]), "+s"(rc[7]), "+s"(rc[8]), "+s"(rc[9]),
+                   "+s"(rc[10]), "+s"(rc[11]), "+s"(rc[12]), "+s"(rc[13]));
     float f[VPT];
     float4 a[VPT], b[VPT];
-    fetch(i, f, a, b);
-    finish(i, f, a, b, Sr, Si);
+    fetch(i, rc, f, a, b);
+    finish(rc, f, a, b, Sr, Si);
     if (((i - row_lo) % kRowBlock) == kRowBlock - 1) flush(Sr, Si);
   }
   flush(Sr, Si);
@@ -750,7 +761,8 @@ __global__ __launch_bounds__(kBlock, BI ? 4 : 5) void k_bwd_taylor(
     };
     // Voxels one at a time: measured on MI355X, two per iteration (more
     // independent chains, 156 VGPRs) and prefetching the next record gained
-    // nothing at config 2 (18.6 ms either way).
+    // nothing; pipelining the records' scalar loads one voxel deep is undone
+    // by the scheduler, which sinks the loads back to their use.
     const int64_t v0 = goff ? (int64_t)goff[b] : (int64_t)b * G;
     const int64_t v1 = goff ? (int64_t)goff[b + 1] : v0 + G;
     for (int64_t v = v0; v < v1; ++v) {
@@ -1107,8 +1119,8 @@ extern "C" int mmn_bp_profile(float *img, float *moment, const float *x_locs,
     mo = with_mom ? w.part + (size_t)nseg * n : nullptr;
   }
   hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nseg), dim3(kBlock), 0, st, im, mo, x_locs,
-                     y_locs, z_locs, vox_idx, n, nx, ny, nz, ant, rx_off, w.rowc,
-                     (const float4 *)w.coef, (float)(fcc * g.h), M, A, num_rx, rps);
+                     y_locs, z_locs, vox_idx, n, nx, ny, nz, w.rowc,
+                     (const float4 *)w.coef, (float)(fcc * g.h), M, A, rps);
   if (nseg > 1) mmn_rs_reduce(st, (float2 *)img, (float2 *)moment, im, mo, n, nseg);
   return mmn_check_launch("mmn_bp_profile");
 }
