@@ -11,11 +11,14 @@
 // result is deterministic; the state lives on the device and the host reads
 // it back once per batch of iterations.
 //
-// Labelling: lock-free union-find over the 6-neighbourhood.  Every voxel
-// starts as its own root; each foreground voxel unites with its foreground
-// neighbours at x-1, y-1, z-1; a union hooks the larger root under the
-// smaller with a compare-and-swap, so parents only ever decrease and the
-// final root of a component is its smallest linear index.  Labels are then
+// Labelling: lock-free union-find over the 6-neighbourhood.  Each foreground
+// voxel unites with its foreground neighbours at x-1, y-1, z-1; a union hooks
+// the larger root under the smaller with a compare-and-swap, so parents only
+// ever decrease and the final root of a component is its smallest linear
+// index.  The unions run in two launches: inside 8 x 8 x 4 tiles, in LDS
+// (k_ccl_local, which also writes every parent), then across tile faces on
+// the tiles' roots in global memory (k_ccl_border) -- so a large component
+// merges per tile rather than per voxel at its one root.  Labels are then
 // numbered 1..n by that index, i.e. in the order a C-order raster scan first
 // meets each component -- scipy.ndimage.label's numbering.
 #include "mmn_internal.h"
@@ -295,28 +298,109 @@ __device__ void unite(int32_t *par, int32_t a, int32_t b) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_ccl_init(int32_t *__restrict__ par,
-                                                     const uint8_t *__restrict__ mask,
-                                                     int64_t V) {
-  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (v >= V) return;
-  par[v] = mask[v] ? (int32_t)v : -1;
+// Tiles of 8 x 8 x 4 voxels, one per 256-thread block; local index
+// t = (lx * 8 + ly) * 4 + lz orders a tile's voxels as their linear indices.
+constexpr int kTX = 8, kTY = 8, kTZ = 4;
+static_assert(kTX * kTY * kTZ == kBlock, "one thread per tile voxel");
+
+struct TileVoxel {
+  int lx, ly, lz, x, y, z;
+  int64_t v;
+  bool in;
+};
+
+__device__ __forceinline__ TileVoxel tile_voxel(int nx, int ny, int nz) {
+  const int ntz = (nz + kTZ - 1) / kTZ, nty = (ny + kTY - 1) / kTY;
+  const int64_t b = blockIdx.x;
+  const int tz = (int)(b % ntz), ty = (int)((b / ntz) % nty), tx = (int)(b / ((int64_t)ntz * nty));
+  TileVoxel r;
+  const int t = threadIdx.x;
+  r.lx = t / (kTY * kTZ);
+  r.ly = (t / kTZ) % kTY;
+  r.lz = t % kTZ;
+  r.x = tx * kTX + r.lx;
+  r.y = ty * kTY + r.ly;
+  r.z = tz * kTZ + r.lz;
+  r.in = r.x < nx && r.y < ny && r.z < nz;
+  r.v = r.in ? ((int64_t)r.x * ny + r.y) * nz + r.z : 0;
+  return r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_ccl_hook(int32_t *par,
-                                                     const uint8_t *__restrict__ mask,
-                                                     int nx, int ny, int nz) {
-  const int64_t V = (int64_t)nx * ny * nz;
-  const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (v >= V || !mask[v]) return;
-  const int z = (int)(v % nz);
-  const int64_t xy = v / nz;
-  const int y = (int)(xy % ny);
-  const int x = (int)(xy / ny);
-  const int32_t vi = (int32_t)v;
-  if (z > 0 && mask[v - 1]) unite(par, vi, vi - 1);
-  if (y > 0 && mask[v - nz]) unite(par, vi, vi - nz);
-  if (x > 0 && mask[v - (int64_t)ny * nz]) unite(par, vi, (int32_t)(v - (int64_t)ny * nz));
+// The union-find of unite / find_root on a tile's LDS parents (workgroup
+// scope): parents only decrease, roots are a component's smallest index.
+__device__ int32_t lfind(int32_t *lp, int32_t x) {
+  while (true) {
+    const int32_t p = __hip_atomic_load(lp + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (p == x) return x;
+    const int32_t gp = __hip_atomic_load(lp + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (gp == p) return p;
+    __hip_atomic_store(lp + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = gp;
+  }
+}
+
+__device__ void lunite(int32_t *lp, int32_t a, int32_t b) {
+  int32_t ra = lfind(lp, a), rb = lfind(lp, b);
+  while (ra != rb) {
+    if (ra < rb) {
+      const int32_t t = ra;
+      ra = rb;
+      rb = t;
+    }
+    int32_t expect = ra;
+    if (__hip_atomic_compare_exchange_strong(lp + ra, &expect, rb, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+      return;
+    ra = lfind(lp, expect);
+    rb = lfind(lp, rb);
+  }
+}
+
+// Components inside each tile, in LDS; every voxel's parent becomes its tile
+// component's root (the smallest linear index of that component within the
+// tile, a root: its own parent), background -1.  Writes every voxel of the
+// grid.
+__global__ __launch_bounds__(kBlock) void k_ccl_local(int32_t *__restrict__ par,
+                                                      const uint8_t *__restrict__ mask,
+                                                      int nx, int ny, int nz) {
+  __shared__ int32_t lp[kBlock];
+  __shared__ uint8_t lf[kBlock];
+  const TileVoxel tv = tile_voxel(nx, ny, nz);
+  const int t = threadIdx.x;
+  const bool fg = tv.in && mask[tv.v];
+  lp[t] = fg ? t : -1;
+  lf[t] = fg;
+  __syncthreads();
+  if (fg) {
+    if (tv.lz > 0 && lf[t - 1]) lunite(lp, t, t - 1);
+    if (tv.ly > 0 && lf[t - kTZ]) lunite(lp, t, t - kTZ);
+    if (tv.lx > 0 && lf[t - kTY * kTZ]) lunite(lp, t, t - kTY * kTZ);
+  }
+  __syncthreads();
+  if (!tv.in) return;
+  int32_t g = -1;
+  if (fg) {
+    int32_t r = t;
+    for (int32_t q = lp[r]; q != r; q = lp[r]) r = q;
+    const int rx = tv.x - tv.lx + r / (kTY * kTZ), ry = tv.y - tv.ly + (r / kTZ) % kTY,
+              rz = tv.z - tv.lz + r % kTZ;
+    g = (int32_t)(((int64_t)rx * ny + ry) * nz + rz);
+  }
+  par[tv.v] = g;
+}
+
+// Unions across tile faces, on the tile roots k_ccl_local left (a kernel
+// boundary after it: every parent is written before any is hooked).
+__global__ __launch_bounds__(kBlock) void k_ccl_border(int32_t *par,
+                                                       const uint8_t *__restrict__ mask,
+                                                       int nx, int ny, int nz) {
+  const TileVoxel tv = tile_voxel(nx, ny, nz);
+  if (!tv.in || !mask[tv.v]) return;
+  const int32_t vi = (int32_t)tv.v;
+  if (tv.lz == 0 && tv.z > 0 && mask[tv.v - 1]) unite(par, vi, vi - 1);
+  if (tv.ly == 0 && tv.y > 0 && mask[tv.v - nz]) unite(par, vi, vi - nz);
+  if (tv.lx == 0 && tv.x > 0 && mask[tv.v - (int64_t)ny * nz])
+    unite(par, vi, (int32_t)(tv.v - (int64_t)ny * nz));
 }
 
 // After all unions (a kernel boundary): parent := root.  Every store here
@@ -515,8 +599,10 @@ extern "C" int mmn_ccl6(int32_t *labels, int32_t *n_components, const uint8_t *m
   hipStream_t st = (hipStream_t)stream;
   const dim3 g(mmn_blocks(V, kBlock)), b(kBlock);
   const int64_t nch = ccl_chunks(V);
-  hipLaunchKernelGGL(k_ccl_init, g, b, 0, st, w.par, mask, V);
-  hipLaunchKernelGGL(k_ccl_hook, g, b, 0, st, w.par, mask, nx, ny, nz);
+  const int64_t ntiles =
+      (int64_t)((nx + kTX - 1) / kTX) * ((ny + kTY - 1) / kTY) * ((nz + kTZ - 1) / kTZ);
+  hipLaunchKernelGGL(k_ccl_local, dim3((unsigned)ntiles), b, 0, st, w.par, mask, nx, ny, nz);
+  hipLaunchKernelGGL(k_ccl_border, dim3((unsigned)ntiles), b, 0, st, w.par, mask, nx, ny, nz);
   hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, st, w.par, V);
   hipLaunchKernelGGL(k_ccl_count, dim3((unsigned)nch), b, 0, st, w.bsum, w.par, V);
   hipLaunchKernelGGL(k_ccl_scan, dim3(1), b, 0, st, w.bsum, w.count, nch);
