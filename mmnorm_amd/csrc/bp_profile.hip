@@ -164,9 +164,14 @@ __global__ __launch_bounds__(kBlock) void k_prof_rows(double *__restrict__ dlo,
   // the row's flag and the moment's point p + o/2 (FP32, exact in FP64), so
   // that the voxel loop reads one record per row
   c[10] = mono ? 1.0 : 0.0;
-  c[11] = (double)__fadd_rn(ant[3 * i + 0], __fmul_rn(ox, 0.5f));
-  c[12] = (double)__fadd_rn(ant[3 * i + 1], __fmul_rn(oy, 0.5f));
-  c[13] = (double)__fadd_rn(ant[3 * i + 2], __fmul_rn(oz, 0.5f));
+  // the point's three floats as raw bits in c[11], c[12] (no conversions in
+  // the voxel loop)
+  const float2 va01 = make_float2(__fadd_rn(ant[3 * i + 0], __fmul_rn(ox, 0.5f)),
+                                  __fadd_rn(ant[3 * i + 1], __fmul_rn(oy, 0.5f)));
+  const float2 va2 = make_float2(__fadd_rn(ant[3 * i + 2], __fmul_rn(oz, 0.5f)), 0.0f);
+  c[11] = __builtin_bit_cast(double, va01);
+  c[12] = __builtin_bit_cast(double, va2);
+  c[13] = 0.0;
   c[14] = 0.0;
   c[15] = 0.0;
 }
@@ -351,7 +356,8 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
   // sum and the vote moment.
   auto finish = [&](const double *__restrict__ c, const float (&f)[VPT], const float4 (&a)[VPT],
                     const float4 (&b)[VPT], float (&Sr)[VPT], float (&Si)[VPT]) {
-    const float vax = (float)c[11], vay = (float)c[12], vaz = (float)c[13];
+    const float2 va01 = __builtin_bit_cast(float2, c[11]);
+    const float vax = va01.x, vay = va01.y, vaz = __builtin_bit_cast(float2, c[12]).x;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       const float qr = fmaf(fmaf(fmaf(b[k].z, f[k], b[k].x), f[k], a[k].z), f[k], a[k].x);
@@ -392,14 +398,14 @@ __global__ __launch_bounds__(kBlock) void k_bp_prof(
     // the row's constants, all in SGPRs before any is used: one scalar-load
     // wait per row instead of one per use (the compiler otherwise sinks
     // some loads past the first leg and waits twice)
-    double rc[14];
+    double rc[13];
     const double *__restrict__ cg = rowc + (size_t)kRowC * i;
 #pragma unroll
-    for (int k = 0; k < 14; ++k) rc[k] = cg[k];
+    for (int k = 0; k < 13; ++k) rc[k] = cg[k];
     asm volatile(""
                  : "+s"(rc[0]), "+s"(rc[1]), "+s"(rc[2]), "+s"(rc[3]), "+s"(rc[4]),
                    "+s"(rc[5]), "+s"(rc[6]), "+s"(rc[7]), "+s"(rc[8]), "+s"(rc[9]),
-                   "+s"(rc[10]), "+s"(rc[11]), "+s"(rc[12]), "+s"(rc[13]));
+                   "+s"(rc[10]), "+s"(rc[11]), "+s"(rc[12]));
     float f[VPT];
     float4 a[VPT], b[VPT];
     fetch(i, rc, f, a, b);
