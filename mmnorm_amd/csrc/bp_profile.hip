@@ -768,12 +768,21 @@ __global__ __launch_bounds__(kBlock, BI ? 4 : 5) void k_bwd_taylor(
     // Voxels one at a time: measured on MI355X, two per iteration (more
     // independent chains, 156 VGPRs) and prefetching the next record gained
     // nothing; pipelining the records' scalar loads one voxel deep is undone
-    // by the scheduler, which sinks the loads back to their use.
+    // by the scheduler, which sinks the loads back to their use.  The empty
+    // asm takes the whole record as in/out SGPR operands, so all of it is
+    // loaded before any is used: one scalar-load wait per voxel, not two.
     const int64_t v0 = goff ? (int64_t)goff[b] : (int64_t)b * G;
     const int64_t v1 = goff ? (int64_t)goff[b + 1] : v0 + G;
     for (int64_t v = v0; v < v1; ++v) {
+      const double4 rd = recd[v];
+      const float4 rf = recf[v];
       const float2 gv = recg[v];
-      voxel(recd[v], recf[v], gv.x, gv.y);
+      double cx = rd.x, cy = rd.y, cz = rd.z, cw = rd.w;
+      float fx = rf.x, fy = rf.y, fz = rf.z, gr = gv.x, gi = gv.y;
+      asm volatile(""
+                   : "+s"(cx), "+s"(cy), "+s"(cz), "+s"(cw), "+s"(fx), "+s"(fy), "+s"(fz),
+                     "+s"(gr), "+s"(gi));
+      voxel(make_double4(cx, cy, cz, cw), make_float4(fx, fy, fz, 0.0f), gr, gi);
     }
 
     // S_ij += exp(+j k_j d0) sum_k eps_k j^k J_k(y_j) mu_k  (conjugate steps)
