@@ -15,7 +15,7 @@
 // voxel unites with its foreground neighbours at x-1, y-1, z-1; a union hooks
 // the larger root under the smaller with a compare-and-swap, so parents only
 // ever decrease and the final root of a component is its smallest linear
-// index.  The unions run in two launches: inside 8 x 8 x 4 tiles, in LDS
+// index.  The unions run in two launches: inside 8 x 8 x 8 tiles, in LDS
 // (k_ccl_local, which also writes every parent), then across tile faces on
 // the tiles' roots in global memory (k_ccl_border) -- so a large component
 // merges per tile rather than per voxel at its one root.  Labels are then
@@ -298,10 +298,10 @@ __device__ void unite(int32_t *par, int32_t a, int32_t b) {
   }
 }
 
-// Tiles of 8 x 8 x 4 voxels, one per 256-thread block; local index
-// t = (lx * 8 + ly) * 4 + lz orders a tile's voxels as their linear indices.
-constexpr int kTX = 8, kTY = 8, kTZ = 4;
-static_assert(kTX * kTY * kTZ == kBlock, "one thread per tile voxel");
+// Tiles of 8 x 8 x 8 voxels, one thread per voxel; local index
+// t = (lx * 8 + ly) * 8 + lz orders a tile's voxels as their linear indices.
+constexpr int kTX = 8, kTY = 8, kTZ = 8;
+constexpr int kTile = kTX * kTY * kTZ;
 
 struct TileVoxel {
   int lx, ly, lz, x, y, z;
@@ -360,11 +360,11 @@ __device__ void lunite(int32_t *lp, int32_t a, int32_t b) {
 // component's root (the smallest linear index of that component within the
 // tile, a root: its own parent), background -1.  Writes every voxel of the
 // grid.
-__global__ __launch_bounds__(kBlock) void k_ccl_local(int32_t *__restrict__ par,
+__global__ __launch_bounds__(kTile) void k_ccl_local(int32_t *__restrict__ par,
                                                       const uint8_t *__restrict__ mask,
                                                       int nx, int ny, int nz) {
-  __shared__ int32_t lp[kBlock];
-  __shared__ uint8_t lf[kBlock];
+  __shared__ int32_t lp[kTile];
+  __shared__ uint8_t lf[kTile];
   const TileVoxel tv = tile_voxel(nx, ny, nz);
   const int t = threadIdx.x;
   const bool fg = tv.in && mask[tv.v];
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_ccl_local(int32_t *__restrict__ par,
 
 // Unions across tile faces, on the tile roots k_ccl_local left (a kernel
 // boundary after it: every parent is written before any is hooked).
-__global__ __launch_bounds__(kBlock) void k_ccl_border(int32_t *par,
+__global__ __launch_bounds__(kTile) void k_ccl_border(int32_t *par,
                                                        const uint8_t *__restrict__ mask,
                                                        int nx, int ny, int nz) {
   const TileVoxel tv = tile_voxel(nx, ny, nz);
@@ -601,8 +601,10 @@ extern "C" int mmn_ccl6(int32_t *labels, int32_t *n_components, const uint8_t *m
   const int64_t nch = ccl_chunks(V);
   const int64_t ntiles =
       (int64_t)((nx + kTX - 1) / kTX) * ((ny + kTY - 1) / kTY) * ((nz + kTZ - 1) / kTZ);
-  hipLaunchKernelGGL(k_ccl_local, dim3((unsigned)ntiles), b, 0, st, w.par, mask, nx, ny, nz);
-  hipLaunchKernelGGL(k_ccl_border, dim3((unsigned)ntiles), b, 0, st, w.par, mask, nx, ny, nz);
+  hipLaunchKernelGGL(k_ccl_local, dim3((unsigned)ntiles), dim3(kTile), 0, st, w.par, mask, nx,
+                     ny, nz);
+  hipLaunchKernelGGL(k_ccl_border, dim3((unsigned)ntiles), dim3(kTile), 0, st, w.par, mask, nx,
+                     ny, nz);
   hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, st, w.par, V);
   hipLaunchKernelGGL(k_ccl_count, dim3((unsigned)nch), b, 0, st, w.bsum, w.par, V);
   hipLaunchKernelGGL(k_ccl_scan, dim3(1), b, 0, st, w.bsum, w.count, nch);
