@@ -766,8 +766,9 @@ __global__ __launch_bounds__(kBlock, BI ? 4 : 5) void k_bwd_taylor(
       }
     };
     // Voxels one at a time: measured on MI355X, two per iteration (more
-    // independent chains, 156 VGPRs) and prefetching the next record gained
-    // nothing; pipelining the records' scalar loads one voxel deep is undone
+    // independent chains, 156 VGPRs), even/odd splits of the Horner and
+    // Chebyshev chains (half their depth) and prefetching the next record
+    // gained nothing; pipelining the records' scalar loads one voxel deep is undone
     // by the scheduler, which sinks the loads back to their use.  The empty
     // asm takes the whole record as in/out SGPR operands, so all of it is
     // loaded before any is used: one scalar-load wait per voxel, not two.
