@@ -84,6 +84,18 @@ def test_label_sizes_equal_bincount():
     assert sizes2[1] == ref[1] and sizes2[0] == lab.numel() - ref[1]
 
 
+def test_label_sizes_many_labels():
+    """More components than the kernel's per-block table holds (4096): the
+    labels past it are counted per wave in global memory."""
+    rng = np.random.default_rng(9)
+    mask = rng.uniform(size=(64, 64, 64)) < 0.06
+    lab, n = ops.ccl6(torch.from_numpy(mask).to(dev()))
+    assert n > 4096
+    sizes = ops.label_sizes(lab, n).cpu().numpy()
+    ref = np.bincount(lab.cpu().numpy().ravel(), minlength=n + 1)
+    assert np.array_equal(sizes, ref)
+
+
 @pytest.mark.parametrize("ex", [2, 2.5, 3.5, 4])
 def test_li_threshold_matches_oracle(oracle, ex):
     rng = np.random.default_rng(int(ex * 100))
