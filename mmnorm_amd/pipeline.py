@@ -165,12 +165,11 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
     valid = sdf.size_rule(areas, dx)
     torch.cuda.synchronize()
     rec["ccl_s"] = time.perf_counter() - tc
-    nrm_np = normals.cpu().numpy()
-    labeled = lab_t.cpu().numpy()
-    m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
-    coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
-    seeds = sdf.component_seeds(nrm_np, coords, labeled, valid)
+    # start voxels: the whole-grid masks on the GPU, each component's mean
+    # and nearest voxel on the host (NumPy's sequential sum)
+    seeds = sdf.component_seeds_gpu(normals, lab_t, valid, sc.x_locs, sc.y_locs, sc.z_locs)
     torch.cuda.synchronize()
+    rec["seeds_s"] = time.perf_counter() - tc - rec["ccl_s"]
     tb = time.perf_counter()
     rsdf, hops = ops.rsdf_bfs(normals, lab_t, T(seeds, np.int32), dx)
     torch.cuda.synchronize()
@@ -183,11 +182,17 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
                 "reached": int(reached.sum().item()), "levels": int(hops.max().item()) + 1,
                 "band_voxels": int(bidx.numel())})
     log(f"vote + CCL + BFS: {rec['prep_s']:.2f} s (CCL {rec['ccl_s'] * 1e3:.1f} ms, "
-        f"BFS {rec['bfs_s'] * 1e3:.1f} ms, "
+        f"seeds {rec['seeds_s'] * 1e3:.1f} ms, BFS {rec['bfs_s'] * 1e3:.1f} ms, "
         f"{rec['components']} components, {rec['reached']} voxels reached, "
         f"{rec['levels']} levels); band {rec['band_voxels']} voxels")
     if bidx.numel() == 0:
         raise RuntimeError("empty isosurface band")
+    if out or tau_search:
+        # host copies for the stage file and the tau search (sdf.py:278-279)
+        nrm_np = normals.cpu().numpy()
+        labeled = lab_t.cpu().numpy()
+        m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
+        coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
     if out:
         # the reference's stage file after the RSDF (sdf.py:226-253)
         from . import formats

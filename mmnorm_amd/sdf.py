@@ -145,6 +145,80 @@ def component_seeds(all_normals, coords_full, labeled, valid_labels):
     return np.asarray(seeds, dtype=np.int64)
 
 
+def component_seeds_gpu(normals, labeled, valid_labels, x_locs, y_locs, z_locs):
+    """``component_seeds`` on device fields, for coords_full built as the
+    reference builds it (sdf.py:278-279: voxel (i, j, k) at
+    (x_locs[i], y_locs[j], z_locs[k])).
+
+    The whole-grid part runs on the GPU: the voxels of the listed labels with a
+    normal, in C order (one mask, one compaction), then a stable sort by label,
+    so each component's voxels form a run in the C order of
+    ``np.argwhere``, and their coordinates, gathered from the axes (equal to
+    coords_full[mask]).  Only those runs come to the host, where the start
+    voxel follows sdf.py:182-191 with the same NumPy expressions on the same
+    values: the mean is NumPy's sequential sum in C order, which a parallel
+    reduction would not reproduce bit for bit.
+
+    normals float64 (NX,NY,NZ,3) and labeled int32 (NX,NY,NZ) tensors on one
+    device (the GPU's in the pipeline; the same torch code runs on CPU
+    tensors).  Returns int64 NumPy linear indices in ``valid_labels`` order,
+    repeated labels searched once.
+    """
+    if normals.dim() != 4 or normals.shape[3] != 3 or normals.shape[:3] != labeled.shape:
+        raise ValueError("normals must be (NX,NY,NZ,3) and labeled (NX,NY,NZ)")
+    nx, ny, nz = labeled.shape
+    xl, yl, zl = (np.asarray(a, dtype=np.float64) for a in (x_locs, y_locs, z_locs))
+    if (len(xl), len(yl), len(zl)) != (nx, ny, nz):
+        raise ValueError("axis lengths do not match the grid")
+    order, seen = [], set()
+    for comp in np.asarray(valid_labels).ravel():
+        key = int(comp)
+        if key not in seen:
+            seen.add(key)
+            order.append(key)
+    if not order:
+        return np.zeros(0, np.int64)
+    flat = labeled.reshape(-1)
+    kmin, kmax = min(order), max(order)
+    if kmax - kmin < 1 << 24:
+        # a flag per label of [kmin, kmax]: one gather per voxel
+        lut = torch.zeros(kmax - kmin + 2, dtype=torch.bool, device=labeled.device)
+        lut[torch.tensor(order, device=labeled.device) - kmin] = True
+        rel = flat.long() - kmin
+        # labels outside [kmin, kmax] read the last flag, which stays False
+        rel = torch.where((rel < 0) | (rel > kmax - kmin), kmax - kmin + 1, rel)
+        member = lut[rel]
+    else:
+        member = torch.isin(flat, torch.tensor(order, dtype=labeled.dtype,
+                                               device=labeled.device))
+    sel = member & ~torch.isnan(normals.reshape(-1, 3)[:, 0])
+    lin = torch.nonzero(sel).reshape(-1)
+    labs = flat[lin]
+    if len(order) > 1:
+        labs, perm = torch.sort(labs, stable=True)
+        lin = lin[perm]
+    # each selected voxel's coordinates, gathered from the axes (exact copies)
+    i = torch.div(lin, ny * nz, rounding_mode="floor")
+    r = lin - i * (ny * nz)
+    j = torch.div(r, nz, rounding_mode="floor")
+    k = r - j * nz
+    dev = labeled.device
+    ax = [torch.from_numpy(a).to(dev) for a in (xl, yl, zl)]
+    sel_coords = torch.stack([ax[0][i], ax[1][j], ax[2][k]], dim=1).cpu().numpy()
+    lin = lin.cpu().numpy()
+    labs = labs.cpu().numpy()
+    seeds = []
+    for key in order:
+        lo, hi = np.searchsorted(labs, key, "left"), np.searchsorted(labs, key, "right")
+        if hi == lo:
+            raise ValueError(f"component {key} has no voxel with a valid normal")
+        coords_sel = sel_coords[lo:hi]
+        mean_coord = np.mean(coords_sel, axis=0)[None, :]
+        dists = np.linalg.norm(coords_sel - mean_coord, axis=1)
+        seeds.append(int(lin[lo + int(np.argmin(dists))]))
+    return np.asarray(seeds, dtype=np.int64)
+
+
 def rsdf_components(all_normals, labeled, seeds, delta_x, device=None, return_hops=False):
     """Run the GPU BFS from explicit seeds; NumPy in and out."""
     _lib.require_gpu()

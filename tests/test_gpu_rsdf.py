@@ -26,10 +26,12 @@ def random_field(shape, density, seed, blobs=True):
     return n, labeled, nlab
 
 
+def axes_of(shape, pitch):
+    return [(np.arange(s) - (s - 1) / 2) * pitch for s in shape]
+
+
 def coords_of(shape, pitch):
-    xl = (np.arange(shape[0]) - (shape[0] - 1) / 2) * pitch
-    yl = (np.arange(shape[1]) - (shape[1] - 1) / 2) * pitch
-    zl = (np.arange(shape[2]) - (shape[2] - 1) / 2) * pitch
+    xl, yl, zl = axes_of(shape, pitch)
     m1, m2, m3 = np.meshgrid(yl, xl, zl)
     return np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1), xl
 
@@ -52,6 +54,12 @@ def test_bfs_bit_exact_vs_fifo(shape, density, seed, blobs, oracle):
     valid = np.arange(2, nlab + 2)[sizes >= 2]
     seeds = msdf.component_seeds(n, coords, labeled, valid)
     np.testing.assert_array_equal(seeds, oracle.rsdf_seeds(n, coords, labeled, valid))
+    # the device-side seed search on the GPU's copies of the fields
+    dev = torch.device("cuda", 0)
+    seeds_dev = msdf.component_seeds_gpu(
+        torch.from_numpy(n).to(dev), torch.from_numpy(labeled.astype(np.int32)).to(dev),
+        valid, *axes_of(shape, 1.5e-3))
+    np.testing.assert_array_equal(seeds_dev, seeds)
     got, hops = msdf.rsdf_components(n, labeled, seeds, xl[1] - xl[0], return_hops=True)
     want, whops = oracle.bfs_fifo(n, labeled, seeds, xl[1] - xl[0])
     assert got.view(np.uint64).tobytes() == want.view(np.uint64).tobytes()
