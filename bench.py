@@ -367,7 +367,15 @@ def bfs_stage(img, mom, sc, dev):
     nrm_np = normals.cpu().numpy()
     m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
     coords = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
+    t0 = time.perf_counter()
     seeds = sdf.component_seeds(nrm_np, coords, labeled, valid)
+    seeds_host_ms = (time.perf_counter() - t0) * 1e3
+    # the device-side search on the same fields (a warm-up call, then timed)
+    sdf.component_seeds_gpu(normals, lab_t, valid, sc.x_locs, sc.y_locs, sc.z_locs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seeds_dev = sdf.component_seeds_gpu(normals, lab_t, valid, sc.x_locs, sc.y_locs, sc.z_locs)
+    seeds_ms = (time.perf_counter() - t0) * 1e3
     seeds_t = torch.from_numpy(seeds.astype(np.int32)).to(dev)
     ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
     torch.cuda.synchronize()
@@ -375,7 +383,10 @@ def bfs_stage(img, mom, sc, dev):
     rsdf, hops = ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    res = {"ms": dt * 1e3, "ccl_ms": ccl_ms, "voxels_reached": int((hops >= 0).sum().item()),
+    res = {"ms": dt * 1e3, "ccl_ms": ccl_ms, "seeds_ms": seeds_ms,
+           "seeds_host_ms": seeds_host_ms,
+           "seeds_equal_host": bool(np.array_equal(seeds_dev, seeds)),
+           "voxels_reached": int((hops >= 0).sum().item()),
            "components": int(len(seeds)), "levels": int(hops.max().item()) + 1,
            "valid_voxels": int(vidx.numel())}
     # what the CPU baseline of the BFS needs: the same inputs and the GPU result
