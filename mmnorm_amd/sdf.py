@@ -11,10 +11,11 @@ and out.
   reference's size rule (sdf.py:92-119) is applied to the areas as written.
 * The breadth-first search (sdf.py:195-222, the interpreter-bound loop) runs
   on the GPU (``ops.rsdf_bfs``), bit-exact with the reference's FIFO order.
-  The start voxel of each component is picked on the host with the
-  reference's own NumPy expressions (sdf.py:182-191: mean of the component's
-  coordinates, Euclidean distance, first minimum), so ties resolve
-  identically.
+  The start voxel of each component is picked with the reference's own
+  NumPy expressions (sdf.py:182-191: mean of the component's coordinates,
+  Euclidean distance, first minimum), so ties resolve identically: on the
+  host for NumPy inputs (``component_seeds``), or from device fields with the
+  whole-grid masks and gathers on the GPU (``component_seeds_gpu``).
 """
 from __future__ import annotations
 
