@@ -392,18 +392,36 @@ __global__ __launch_bounds__(kTile) void k_ccl_local(int32_t *__restrict__ par,
   par[tv.v] = g;
 }
 
+// One union per active lane across a tile face, on the parents of its two
+// voxels (ancestors in the same sets).  The face voxels of a tile mostly
+// join the same two tile components, so a lane whose pair of parents equals
+// the previous active lane's in its wave skips its union: that lane joins
+// the same two sets.  On a field with one large component this removes most
+// of the compare-and-swaps that would retry at its root.  All lanes of the
+// wave call this (ballot and shuffle).
+__device__ __forceinline__ void face_union(int32_t *par, bool act, int32_t a, int32_t b) {
+  const int lane = (int)(threadIdx.x % kWave);
+  const unsigned long long m = __ballot(act);
+  const int32_t pa = act ? load_par(par, a) : -1;
+  const int32_t pb = act ? load_par(par, b) : -1;
+  const unsigned long long below = m & ((1ull << lane) - 1ull);
+  const int prev = below ? 63 - __clzll((long long)below) : lane;
+  const int32_t qa = __shfl(pa, prev, kWave), qb = __shfl(pb, prev, kWave);
+  if (act && !(below && qa == pa && qb == pb)) unite(par, pa, pb);
+}
+
 // Unions across tile faces, on the tile roots k_ccl_local left (a kernel
 // boundary after it: every parent is written before any is hooked).
 __global__ __launch_bounds__(kTile) void k_ccl_border(int32_t *par,
                                                        const uint8_t *__restrict__ mask,
                                                        int nx, int ny, int nz) {
   const TileVoxel tv = tile_voxel(nx, ny, nz);
-  if (!tv.in || !mask[tv.v]) return;
+  const bool fg = tv.in && mask[tv.v];
   const int32_t vi = (int32_t)tv.v;
-  if (tv.lz == 0 && tv.z > 0 && mask[tv.v - 1]) unite(par, vi, vi - 1);
-  if (tv.ly == 0 && tv.y > 0 && mask[tv.v - nz]) unite(par, vi, vi - nz);
-  if (tv.lx == 0 && tv.x > 0 && mask[tv.v - (int64_t)ny * nz])
-    unite(par, vi, (int32_t)(tv.v - (int64_t)ny * nz));
+  const int64_t sx = (int64_t)ny * nz;
+  face_union(par, fg && tv.lz == 0 && tv.z > 0 && mask[tv.v - 1], vi, vi - 1);
+  face_union(par, fg && tv.ly == 0 && tv.y > 0 && mask[tv.v - nz], vi, vi - nz);
+  face_union(par, fg && tv.lx == 0 && tv.x > 0 && mask[tv.v - sx], vi, (int32_t)(tv.v - sx));
 }
 
 // After all unions (a kernel boundary): parent := root.  Every store here
