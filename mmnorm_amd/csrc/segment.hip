@@ -476,27 +476,33 @@ __global__ __launch_bounds__(kBlock) void k_ccl_scan(int32_t *__restrict__ bsum,
 }
 
 // Roots get their number 1..n in index order; the rest are written by
-// k_ccl_label after this kernel has finished.
+// k_ccl_label after this kernel has finished.  The flags are 0 or 1, so a
+// lane's rank within its wave is a popcount of the wave's ballot below it,
+// and the waves' totals go through LDS: two barriers per 256 voxels.
 __global__ __launch_bounds__(kBlock) void k_ccl_number(int32_t *__restrict__ labels,
                                                        const int32_t *__restrict__ par,
                                                        const int32_t *__restrict__ boff,
                                                        int64_t V) {
-  __shared__ int32_t sh[kBlock];
+  __shared__ int32_t wsum[kBlock / kWave];
+  const int lane = (int)(threadIdx.x % kWave), w = (int)(threadIdx.x / kWave);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
   const int64_t c0 = (int64_t)blockIdx.x * kScanChunk;
   int32_t carry = boff[blockIdx.x];
   for (int q = 0; q < kScanChunk / kBlock; ++q) {
     const int64_t v = c0 + (int64_t)q * kBlock + threadIdx.x;
-    const int32_t f = (v < V && par[v] == (int32_t)v) ? 1 : 0;
-    sh[threadIdx.x] = f;
+    const bool f = v < V && par[v] == (int32_t)v;
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) wsum[w] = __popcll(m);
     __syncthreads();
-    for (int s = 1; s < kBlock; s <<= 1) {
-      const int32_t y = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += y;
-      __syncthreads();
+    int32_t before = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / kWave; ++i) {
+      if (i < w) before += wsum[i];
+      total += wsum[i];
     }
-    if (f) labels[v] = carry + sh[threadIdx.x];  // inclusive count = 1-based number
-    carry += sh[kBlock - 1];
+    // inclusive count = 1-based number
+    if (f) labels[v] = carry + before + __popcll(m & upto);
+    carry += total;
     __syncthreads();
   }
 }
