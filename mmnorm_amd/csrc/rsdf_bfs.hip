@@ -30,7 +30,7 @@
 // the frontier, its rsdf and its masks in LDS: no launches, and the
 // frontier's data never goes through global memory, but one CU issues all
 // of a level's scattered requests.  Larger frontiers run as two kernels
-// per level over kGrid blocks, each block owning one contiguous chunk of
+// per level over a grid of blocks, each block owning one contiguous chunk of
 // the frontier (block order is position order): k_count and k_emit_claim
 // (each block sums the counts before it for its offset; each child offers
 // the next level's keys at once).  The host enqueues those levels in
@@ -44,7 +44,7 @@
 
 namespace {
 
-constexpr int kGrid = 1024;
+constexpr int kGrid = 1024;      // blocks of the grid-stride kernels; most level blocks
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kSmallThreads = 1024;  // one workgroup; PER positions per thread
@@ -55,7 +55,7 @@ struct Ws {
   int32_t *front1;  // [V]
   uint8_t *pmask;   // [V]  won mask of frontier position p (bit d)
   uint8_t *emask;   // [V]  enterable neighbours of each voxel (bit d)
-  int32_t *bcnt;    // [kGrid]
+  int32_t *bcnt;    // [kGrid] per-block counts of a level
   int32_t *nf;      // [8]  [0,1] frontier sizes by level parity; [2] the level
                     //      k_bfs_small stopped at; [4,5] queue index of a level's
                     //      first entry, by level parity
@@ -90,7 +90,7 @@ size_t ws_layout(int64_t V, Ws *w, char *base) {
 }
 
 __device__ __forceinline__ void chunk_of(int n, int &start, int &end) {
-  const int chunk = (n + kGrid - 1) / kGrid;
+  const int chunk = (n + (int)gridDim.x - 1) / (int)gridDim.x;
   start = min(n, (int)blockIdx.x * chunk);
   end = min(n, start + chunk);
 }
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_count(
 
 // Emit, and offer each child's keys of the next level at once.  Each block
 // finds its output offset by summing the counts of the blocks before it
-// (k_count wrote them; 4 KB), and block 0 writes the next level's size and
+// (k_count wrote them; 1-4 KB), and block 0 writes the next level's size and
 // queue index into the other parity's slots (this launch reads only its
 // own).  The child at next-frontier position o has queue index q1 + o,
 // q1 = q0 + n.  Other blocks may still be emitting this level; a voxel they
@@ -259,9 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_claim(
   chunk_of(n, start, end);
   const int32_t q1 = nf[4 + cur] + n;
   int before = 0, all = 0;
-#pragma unroll
-  for (int k = 0; k < kGrid / kBlock; ++k) {
-    const int b = k * kBlock + (int)threadIdx.x;
+  for (int b = (int)threadIdx.x; b < (int)gridDim.x; b += kBlock) {
     const int v = bcnt[b];
     all += v;
     if (b < (int)blockIdx.x) before += v;
@@ -549,6 +547,16 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     const int v = e ? atoi(e) : 32;
     return v < 1 ? 1 : v > 64 ? 64 : v;
   }();
+  // Blocks of the multi-block level kernels (MMN_BFS_GRID: 256, 512 or
+  // 1024; default 256).  A level's frontier has a few thousand entries, so
+  // fewer blocks each take more of it and read fewer counts: on the
+  // 1.5M-voxel field 256 / 1024 blocks take 1.35 / 1.39 ms, on an 8.4M-voxel
+  // field (307 levels) 3.5-3.6 / 3.65-3.75 ms.
+  static const int lgrid = [] {
+    const char *e = getenv("MMN_BFS_GRID");
+    const int v = e ? atoi(e) : 256;
+    return v == 512 || v == kGrid ? v : 256;
+  }();
   int32_t *fr[2] = {w.front0, w.front1};
   int32_t n_cur = n_seeds;
   for (int level = 0; level <= V && n_cur > 0;) {
@@ -572,13 +580,13 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     }
     // the batch's first level is claimed here; every emit claims the next
     // (a level that was already claimed gets the same keys again: no change)
-    hipLaunchKernelGGL(k_claim, dim3(kGrid), dim3(kBlock), 0, st, fr[level & 1], w.nf,
+    hipLaunchKernelGGL(k_claim, dim3(lgrid), dim3(kBlock), 0, st, fr[level & 1], w.nf,
                        level & 1, w.claim, w.emask, ny, nz);
     for (int b = 0; b < batch; ++b, ++level) {
       const int cur = level & 1, nxt = cur ^ 1;
-      hipLaunchKernelGGL(k_count, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
+      hipLaunchKernelGGL(k_count, dim3(lgrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
                          w.claim, w.emask, w.pmask, w.bcnt, ny, nz);
-      hipLaunchKernelGGL(k_emit_claim, dim3(kGrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
+      hipLaunchKernelGGL(k_emit_claim, dim3(lgrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
                          w.pmask, w.bcnt, fr[nxt], hops, rsdf, normals, w.emask, w.claim, dx,
                          level, ny, nz);
     }
