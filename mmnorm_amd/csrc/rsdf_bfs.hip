@@ -548,10 +548,9 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     return v < 1 ? 1 : v > 64 ? 64 : v;
   }();
   // Blocks of the multi-block level kernels (MMN_BFS_GRID: 256, 512 or
-  // 1024; default 256).  A level's frontier has a few thousand entries, so
-  // fewer blocks each take more of it and read fewer counts: on the
-  // 1.5M-voxel field 256 / 1024 blocks take 1.35 / 1.39 ms, on an 8.4M-voxel
-  // field (307 levels) 3.5-3.6 / 3.65-3.75 ms.
+  // 1024; default 256).  A level's frontier has a few thousand entries; with
+  // 256 or 1024 blocks a level's kernels take the same time (4.7 + 5.7 us on
+  // the 1.5M-voxel field), and 256 reads a quarter of the counts.
   static const int lgrid = [] {
     const char *e = getenv("MMN_BFS_GRID");
     const int v = e ? atoi(e) : 256;
