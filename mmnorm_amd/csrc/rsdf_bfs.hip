@@ -521,12 +521,15 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   hipLaunchKernelGGL(k_emask, dim3(kGrid), dim3(kBlock), 0, st, w.emask, normals, labels,
                      nx, ny, nz);
 
-  // Positions per thread of the single-workgroup loop (MMN_BFS_PER: 2, 4, 8,
-  // or 0 for the multi-block kernels at every level).
+  // Positions per thread of the single-workgroup loop (MMN_BFS_PER: 1, 2, 4,
+  // 8, or 0 for the multi-block kernels at every level).  One position per
+  // thread spreads a level's scattered requests over all 16 waves: on fields
+  // of 57,483 / 392,019 reached voxels (64 / 307 levels) PER 1 takes 0.66 /
+  // 3.3-3.4 ms, PER 2 0.71 / 3.45-3.6 ms.
   static const int per = [] {
     const char *e = getenv("MMN_BFS_PER");
-    const int v = e ? atoi(e) : 2;
-    return v == 0 || v == 4 || v == 8 ? v : 2;
+    const int v = e ? atoi(e) : 1;
+    return v == 0 || v == 2 || v == 4 || v == 8 ? v : 1;
   }();
   // Largest frontier the single workgroup takes (MMN_BFS_CAP, default 1024,
   // at most 1024 * per).  Its levels are bound by one CU's memory pipeline
@@ -561,7 +564,10 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   for (int level = 0; level <= V && n_cur > 0;) {
     if (n_cur <= small_cap) {
       // small frontiers: one workgroup runs levels until it outgrows it
-      auto small = per == 8 ? k_bfs_small<8> : per == 4 ? k_bfs_small<4> : k_bfs_small<2>;
+      auto small = per == 8   ? k_bfs_small<8>
+                   : per == 4 ? k_bfs_small<4>
+                   : per == 2 ? k_bfs_small<2>
+                              : k_bfs_small<1>;
       hipLaunchKernelGGL(small, dim3(1), dim3(kSmallThreads), 0, st, w.front0, w.front1, w.nf,
                          w.claim, hops, rsdf, normals, w.emask, dx, level, small_cap, ny, nz);
       rc = mmn_check_launch("mmn_rsdf_bfs(small)");
