@@ -584,7 +584,8 @@ int dispatch_k(int K, hipStream_t st, const RecArgs &a) {
 
 // Lane shape: VPT voxels per lane, S-way split Horner.  The defaults (2, 1)
 // were chosen by measurement on MI355X (config 2, K = 64): VPT 2 beats 1 by
-// ~8 %; the even/odd split and two blocks side by side give no gain (the
+// ~8 %, and VPT 4 (128 VGPRs, four waves per SIMD, half the scalar-load waits
+// per FMA) is 0.5 % slower; the even/odd split and two blocks side by side give no gain (the
 // latter spills SGPRs); staging rows in LDS with a double-buffered tile, or
 // streaming chunks with explicit prefetch, measured 15-50 % slower (their
 // extra VGPRs cost occupancy).  MMN_REC_VPT / MMN_REC_SPLIT override them.
