@@ -104,10 +104,14 @@ int mmn_cuda_sn_est(float *normals, const int32_t *vox_idx, int64_t n_valid,
  * No reference counterpart for the fused moment or the variants; the
  * reference computes the vote only for the interpolated branch
  * (py_sn_est_gpu.py:404-405).
- * workspace (optional, may be NULL): lets a small voxel set be split over
- * row segments whose partial sums are reduced in a fixed order, so that the
- * device fills; mmn_bp_forward_workspace_size() gives the bytes that enable
- * every split the launch could choose.  Results are deterministic for a
+ * workspace (optional, may be NULL): lets the voxel blocks that do not fill
+ * a whole round of resident workgroups (all of them for a small voxel set,
+ * else those of the last, partial round) be split over row segments whose
+ * partial sums are reduced in a fixed order, so that the device stays full;
+ * mmn_bp_forward_workspace_size() gives exactly the bytes of the split the
+ * launch chooses on the current device (0 when it needs none; without a
+ * device it assumes MI355X: 256 CUs, 8 workgroups each).  A smaller
+ * workspace gives a coarser split or none.  Results are deterministic for a
  * given workspace size and device.
  */
 size_t mmn_bp_forward_workspace_size(int64_t n_vox, int num_rows, int with_moment);
@@ -237,7 +241,10 @@ int mmn_bp_backward(float *grad_meas, float *grad_ant, const float *g,
  * seeds, otherwise rsdf[parent] + dot(dir * dx, normals[child]) with the
  * parent FIFO order gives, sdf.py:216-222), hops int32 [V] (BFS level, -1
  * where not reached).  Bit-exact with the reference's FIFO order.  Calls
- * hipStreamSynchronize on `stream` once per batch of levels.
+ * hipStreamSynchronize on `stream` once per batch of levels.  Every seed
+ * must lie in [0, V): the seeds are read on the device, which skips a seed
+ * outside the grid and runs no level, and the call then returns
+ * MMN_ERR_INPUT (rsdf and hops unspecified).
  */
 size_t mmn_rsdf_bfs_workspace_size(int nx, int ny, int nz);
 int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
@@ -312,7 +319,7 @@ int mmn_ray_matching(float *intersection_pos, float *voxel_position,
 /*
  * mmn_simulate_channel -- the channel of one antenna from the isosurface
  * grid - tau (_simulate_channels for one component,
- * isosurface_optimization.py:137-245): march every ray, refine the hit by
+ * isosurface_optimization.py:111-209): march every ray, refine the hit by
  * the interpolated value along the ray, d = 2 |rx_loc - hit| (+ 0.15 m if
  * is_ti_radar), and channel[j] = sum over hits of exp(+j 2 pi (f0 + j df) d / c)
  * (complex [samples]).  voxel_size is the refine step's voxel size (the
@@ -329,7 +336,7 @@ int mmn_simulate_channel(float *channel, const float *grid, float tau, int width
                          void *workspace, size_t workspace_bytes, void *stream);
 
 /*
- * mmn_channel_loss -- loss_fn of isosurface_optimization.py:248-280 for
+ * mmn_channel_loss -- loss_fn of isosurface_optimization.py:211-246 for
  * n_pairs (simulated, measured) channel pairs (complex [n_pairs][samples]
  * each): the DFT of each zero-padded to n_fft, bins [bin_lo, bin_hi) (the
  * measured profile zero from bin_hi_target on), each profile scaled to unit

@@ -21,6 +21,8 @@
 // staging and no per-lane measurement traffic.
 #include <stdlib.h>
 
+#include <algorithm>
+
 #include "mmn_internal.h"
 
 namespace {
@@ -102,20 +104,22 @@ __device__ __forceinline__ void load_voxels(
   }
 }
 
+// Voxel t (< nvox) of the launch goes to img[t - off] (and mom[t - off]).
 template <int VPT>
 __device__ __forceinline__ void store_voxels(
     float2 *__restrict__ img, float2 *__restrict__ mom, int64_t nvox,
     int64_t base, const float (&Ir)[VPT], const float (&Ii)[VPT],
-    const float (&M)[VPT][6]) {
+    const float (&M)[VPT][6], int64_t off = 0) {
 #pragma unroll
   for (int k = 0; k < VPT; ++k) {
     const int64_t t = base + (int64_t)k * kBlock;
     if (t < nvox) {
-      img[t] = make_float2(Ir[k], Ii[k]);
+      const int64_t o = t - off;
+      img[o] = make_float2(Ir[k], Ii[k]);
       if (mom) {
-        mom[3 * t + 0] = make_float2(M[k][0], M[k][1]);
-        mom[3 * t + 1] = make_float2(M[k][2], M[k][3]);
-        mom[3 * t + 2] = make_float2(M[k][4], M[k][5]);
+        mom[3 * o + 0] = make_float2(M[k][0], M[k][1]);
+        mom[3 * o + 1] = make_float2(M[k][2], M[k][3]);
+        mom[3 * o + 2] = make_float2(M[k][4], M[k][5]);
       }
     }
   }
@@ -390,19 +394,30 @@ __device__ __forceinline__ void rec_block(const float2 *__restrict__ row, int b0
   }
 }
 
-// Row segments: blockIdx.y takes rows [y*rps, min((y+1)*rps, rows)) and, when
-// there is more than one segment, writes its partial sums to img/mom + y*n
-// (a workspace reduced afterwards in a fixed order).
+// Work split (RecPlan): workgroups [0, nmain) take voxel block b = blockIdx.x
+// and every row, and write img/mom.  The others take tail voxel block
+// nmain + (t mod ntail) and row segment s = t / ntail (t = blockIdx.x - nmain),
+// rows [s rps, min((s + 1) rps, rows)), and write their partial sums to
+// pimg/pmom + s * (tail voxels), indexed from the first tail voxel; the
+// segments are added afterwards in a fixed order.
 template <int K, int VPT, int S>
 __global__ __launch_bounds__(kBlock) void k_bp_rec(
     float2 *__restrict__ img, float2 *__restrict__ mom,
+    float2 *__restrict__ pimg, float2 *__restrict__ pmom,
     const float *__restrict__ xl, const float *__restrict__ yl,
     const float *__restrict__ zl, const int32_t *__restrict__ vidx,
     int64_t nvox, const float *__restrict__ ant,
     const float2 *__restrict__ meas, const float *__restrict__ rxo,
     double f0c, double dfc, int ny, int nz, int rows, int R, int F,
-    int is_ti, int rps) {
-  const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
+    int is_ti, int64_t nmain, int64_t ntail, int rps) {
+  int64_t vb = blockIdx.x;
+  int seg = -1;
+  if (vb >= nmain) {
+    const int64_t t = vb - nmain;
+    seg = (int)(t / ntail);
+    vb = nmain + (t - (int64_t)seg * ntail);
+  }
+  const int64_t base = vb * (kBlock * VPT) + threadIdx.x;
   float vx[VPT], vy[VPT], vz[VPT];
   load_voxels<VPT>(xl, yl, zl, vidx, nvox, ny, nz, base, vx, vy, vz);
   float Ir[VPT], Ii[VPT], Cr[VPT], Ci[VPT], M[VPT][6];
@@ -415,8 +430,8 @@ __global__ __launch_bounds__(kBlock) void k_bp_rec(
   const int nb = F / K;
   const int tail = F - nb * K;
 
-  const int row_lo = blockIdx.y * rps;
-  const int row_hi = min(rows, row_lo + rps);
+  const int row_lo = seg < 0 ? 0 : seg * rps;
+  const int row_hi = seg < 0 ? rows : min(rows, row_lo + rps);
   for (int i = row_lo; i < row_hi; ++i) {
     const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
     const int rx = i % R;
@@ -462,9 +477,13 @@ __global__ __launch_bounds__(kBlock) void k_bp_rec(
       add_row(cr[k], ci[k], vax - vx[k], vay - vy[k], vaz - vz[k], Ir[k],
               Ii[k], Cr[k], Ci[k], M[k]);
   }
-  store_voxels<VPT>(img + (size_t)blockIdx.y * nvox,
-                    mom ? mom + (size_t)blockIdx.y * nvox * 3 : nullptr, nvox, base, Ir,
-                    Ii, M);
+  if (seg < 0) {
+    store_voxels<VPT>(img, mom, nvox, base, Ir, Ii, M);
+  } else {
+    const int64_t off = nmain * (kBlock * VPT), ntv = nvox - off;
+    store_voxels<VPT>(pimg + (size_t)seg * ntv, pmom ? pmom + (size_t)seg * ntv * 3 : nullptr,
+                      nvox, base, Ir, Ii, M, off);
+  }
 }
 
 // Fixed-order FP64 sum of the row segments' partial sums.
@@ -513,62 +532,105 @@ struct RecArgs {
   size_t ws_bytes;
 };
 
-// Most row segments a launch may use for n voxels: enough to fill the device
-// for small voxel sets, never fewer than 32 rows per segment.
-int rs_cap(int64_t n, int rows) {
+// Most row segments a launch may use: never fewer than 32 rows per segment.
+int rs_cap(int rows) {
   int cap = rows / 32;
   if (cap > 32) cap = 32;
   if (cap < 1) cap = 1;
-  (void)n;
   return cap;
 }
 
-size_t rs_bytes(int rs, int64_t n, bool moment) {
-  return rs > 1 ? (size_t)rs * (size_t)n * sizeof(float2) * (moment ? 4 : 1) : 0;
+// Resident workgroups of a kernel on the current device (occupancy x CUs).
+// Without a device (the workspace query may run on a host with no GPU) it
+// assumes MI355X and the recurrence kernel's measured occupancy: 256 CUs, 8
+// workgroups of 256 lanes per CU.
+template <typename KernelT>
+int64_t resident_slots(KernelT kernel) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 8;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  (void)hipGetLastError();
+  return (int64_t)per_cu * cus;
 }
 
-// Number of row segments: the smallest count that gives the device at least
-// eight full rounds of resident workgroups (occupancy from the runtime), within
-// rs_cap and the workspace the caller provided.  MMN_REC_RS forces a value.
-template <typename KernelT>
-int choose_rs(KernelT kernel, int64_t nblk, const RecArgs &a) {
-  int rs = 1;
+// Work split of one recurrence launch (see k_bp_rec).  Whole rounds of
+// resident workgroups take every row and write the image directly; only the
+// voxel blocks of a last, partial round are split over row segments, enough
+// of them for one more full round, so that the tail does not run the
+// device half empty, and only their partial sums pass through the
+// workspace.  A voxel set smaller than one round is split entirely, aiming
+// at eight rounds.  MMN_REC_RS = n forces every block into n segments.
+struct RecPlan {
+  int64_t nmain = 0, ntail = 0;  // voxel blocks taking all rows / split
+  int nseg = 1, rps = 0;
+  int64_t tail_vox = 0;          // voxels of the split blocks
+  size_t ws = 0;                 // workspace bytes of the partial sums
+};
+
+RecPlan plan_rec(int64_t nvox, int rows, bool moment, int vpt, int64_t slots,
+                 size_t ws_limit) {
+  const int64_t per_blk = (int64_t)kBlock * vpt;
+  const int64_t nblk = (nvox + per_blk - 1) / per_blk;
+  RecPlan p;
+  p.nmain = nblk;
+  p.rps = rows;
+  int nseg = 1;
+  int64_t ntail = 0;
   const int forced = env_int("MMN_REC_RS", 0);
   if (forced > 0) {
-    rs = forced;
+    nseg = forced;
+    ntail = nblk;
+  } else if (nblk < slots) {
+    ntail = nblk;
+    while (ntail * nseg < 8 * slots && nseg < 64) ++nseg;
   } else {
-    int per_cu = 0, dev = 0, cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) !=
-            hipSuccess || per_cu < 1)
-      per_cu = 2;
-    if (hipGetDevice(&dev) == hipSuccess)
-      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipGetLastError();
-    const int64_t slots = (int64_t)per_cu * cus;
-    while (nblk * rs < 8 * slots && rs < 64) ++rs;
+    ntail = nblk % slots;
+    if (ntail > 0) nseg = (int)std::min<int64_t>(64, (slots + ntail - 1) / ntail);
   }
-  const int cap = rs_cap(a.nvox, a.rows);
-  if (rs > cap) rs = cap;
-  while (rs > 1 && rs_bytes(rs, a.nvox, a.mom != nullptr) > a.ws_bytes) --rs;
-  if (!a.ws) rs = 1;
-  return rs;
+  nseg = std::min(nseg, rs_cap(rows));
+  for (; nseg > 1; --nseg) {
+    const int rps = (rows + nseg - 1) / nseg;
+    const int used = (rows + rps - 1) / rps;
+    const int64_t tv = nvox - (nblk - ntail) * per_blk;
+    const size_t bytes = (size_t)used * (size_t)tv * sizeof(float2) * (moment ? 4 : 1);
+    if (bytes > ws_limit) continue;
+    if (used < 2) break;
+    p.nmain = nblk - ntail;
+    p.ntail = ntail;
+    p.nseg = used;
+    p.rps = rps;
+    p.tail_vox = tv;
+    p.ws = bytes;
+    break;
+  }
+  return p;
 }
 
 template <int K, int VPT, int S>
 void launch_rec(hipStream_t st, const RecArgs &a) {
-  const int64_t nblk = mmn_blocks(a.nvox, kBlock * VPT);
-  const int rs = choose_rs(k_bp_rec<K, VPT, S>, nblk, a);
-  const int rps = (a.rows + rs - 1) / rs;
-  const int nseg = (a.rows + rps - 1) / rps;
-  float2 *img = a.img, *mom = a.mom;
-  if (nseg > 1) {
-    img = (float2 *)a.ws;
-    mom = a.mom ? img + (size_t)nseg * a.nvox : nullptr;
+  const int64_t slots = resident_slots(k_bp_rec<K, VPT, S>);
+  const RecPlan p = plan_rec(a.nvox, a.rows, a.mom != nullptr, VPT, slots,
+                             a.ws ? a.ws_bytes : 0);
+  float2 *pimg = nullptr, *pmom = nullptr;
+  if (p.nseg > 1) {
+    pimg = (float2 *)a.ws;
+    pmom = a.mom ? pimg + (size_t)p.nseg * p.tail_vox : nullptr;
   }
-  hipLaunchKernelGGL((k_bp_rec<K, VPT, S>), dim3((unsigned)nblk, nseg), dim3(kBlock), 0, st,
-                     img, mom, a.xl, a.yl, a.zl, a.vidx, a.nvox, a.ant, a.meas, a.rxo,
-                     a.f0c, a.dfc, a.ny, a.nz, a.rows, a.R, a.F, a.is_ti, rps);
-  if (nseg > 1) mmn_rs_reduce(st, a.img, a.mom, img, mom, a.nvox, nseg);
+  const int64_t nwg = p.nmain + p.ntail * p.nseg;
+  hipLaunchKernelGGL((k_bp_rec<K, VPT, S>), dim3((unsigned)nwg), dim3(kBlock), 0, st, a.img,
+                     a.mom, pimg, pmom, a.xl, a.yl, a.zl, a.vidx, a.nvox, a.ant, a.meas,
+                     a.rxo, a.f0c, a.dfc, a.ny, a.nz, a.rows, a.R, a.F, a.is_ti, p.nmain,
+                     p.ntail, p.rps);
+  if (p.nseg > 1) {
+    const int64_t off = a.nvox - p.tail_vox;
+    mmn_rs_reduce(st, a.img + off, a.mom ? a.mom + 3 * off : nullptr, pimg, pmom, p.tail_vox,
+                  p.nseg);
+  }
 }
 
 template <int VPT, int S>
@@ -597,6 +659,28 @@ int dispatch_rec(int K, hipStream_t st, const RecArgs &a) {
   if (vpt == 1) return dispatch_k<1, 2>(K, st, a);
   if (split == 1) return dispatch_k<2, 1>(K, st, a);
   return dispatch_k<2, 2>(K, st, a);
+}
+
+// Workspace the recurrence launch uses, for the lane shape dispatch_rec
+// picks and the largest over the re-anchor intervals (their occupancy may
+// differ).
+template <int VPT, int S>
+size_t ws_for_shape(int64_t n, int rows, bool mom) {
+  const size_t lim = (size_t)-1;
+  size_t m = plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<8, VPT, S>), lim).ws;
+  m = std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<16, VPT, S>), lim).ws);
+  m = std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<32, VPT, S>), lim).ws);
+  return std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<64, VPT, S>), lim).ws);
+}
+
+size_t ws_for_rec(int64_t n, int rows, bool mom) {
+  const int vpt = env_int("MMN_REC_VPT", 2);
+  const int split = env_int("MMN_REC_SPLIT", 1);
+  if (vpt == 4) return ws_for_shape<4, 1>(n, rows, mom);
+  if (vpt == 1 && split == 1) return ws_for_shape<1, 1>(n, rows, mom);
+  if (vpt == 1) return ws_for_shape<1, 2>(n, rows, mom);
+  if (split == 1) return ws_for_shape<2, 1>(n, rows, mom);
+  return ws_for_shape<2, 2>(n, rows, mom);
 }
 
 }  // namespace
@@ -660,7 +744,7 @@ extern "C" int mmn_cuda_image(float *img, const float *x_locs,
 extern "C" size_t mmn_bp_forward_workspace_size(int64_t n_vox, int num_rows,
                                                 int with_moment) {
   if (n_vox <= 0 || num_rows <= 0) return 0;
-  return rs_bytes(rs_cap(n_vox, num_rows), n_vox, with_moment != 0);
+  return ws_for_rec(n_vox, num_rows, with_moment != 0);
 }
 
 extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,

@@ -1,6 +1,6 @@
 // Ray marching through an RSDF isosurface and the mmWave channel it returns:
 // the consumer of the RSDF (ray_tracing_kernel.cu:60-421 and
-// isosurface_optimization.py:111-246).
+// isosurface_optimization.py:85-246).
 //
 // Per antenna, width x height rays leave the antenna through a pinhole
 // camera looking along -rx_dir (ray_tracing_kernel.cu:60-127); each ray is
@@ -8,10 +8,10 @@
 // isosurface offset tau, stepping by the SDF value and taking MIN_STEP where
 // a corner is undefined, until the value drops below one voxel
 // (ray_tracing_kernel.cu:206-421).  The rough hit is refined by one step of
-// the interpolated value along the ray (isosurface_optimization.py:111-135),
+// the interpolated value along the ray (isosurface_optimization.py:85-109),
 // and the channel is the sum over hits of exp(+j 2 pi f d / c) with d twice
 // the antenna-to-hit distance (+0.15 m for the TI radar)
-// (isosurface_optimization.py:220-245).
+// (isosurface_optimization.py:203-208).
 //
 // Arithmetic follows the reference expression by expression (FP32, no FMA
 // contraction; the pixel angle in FP64 as its mixed-precision expression
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_ray_match(float *__restrict__ hit_ou
 }
 
 // Grid index of the refine step as torch indexing reads it: -1 was replaced
-// by 0 (isosurface_optimization.py:194-196); other negatives wrap.
+// by 0 (isosurface_optimization.py:185-187); other negatives wrap.
 __device__ __forceinline__ int64_t torch_index(float v, int res) {
   int64_t i = (int64_t)v;
   if (i == -1) i = 0;
@@ -221,8 +221,8 @@ __device__ __forceinline__ int64_t torch_index(float v, int res) {
   return i < 0 ? 0 : (i >= res ? res - 1 : i);
 }
 
-// Per ray: march, refine (isosurface_optimization.py:111-135,190-205) and
-// the round-trip distance in FP64 (isosurface_optimization.py:208-211);
+// Per ray: march, refine (isosurface_optimization.py:85-109,190-196) and
+// the round-trip distance in FP64 (isosurface_optimization.py:198-201);
 // NaN marks a ray without a hit.
 __global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
                                                      const float *__restrict__ grid, RayGeom g,
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_synth_reduce(float2 *__restrict__ ou
   out[f] = make_float2((float)sr, (float)si);
 }
 
-// Range-profile loss of isosurface_optimization.py:248-280 for B channel
+// Range-profile loss of isosurface_optimization.py:211-246 for B channel
 // pairs: DFT of each channel zero-padded to n_fft, bins [lo, hi) (target
 // bins >= hi_target zeroed first), each profile normalised to unit L2 norm,
 // loss = sum | |sim| - |target| |.  One block per pair; twiddles from the

@@ -57,8 +57,9 @@ struct Ws {
   uint8_t *emask;   // [V]  enterable neighbours of each voxel (bit d)
   int32_t *bcnt;    // [kGrid] per-block counts of a level
   int32_t *nf;      // [8]  [0,1] frontier sizes by level parity; [2] the level
-                    //      k_bfs_small stopped at; [4,5] queue index of a level's
-                    //      first entry, by level parity
+                    //      k_bfs_small stopped at; [3] non-zero when a seed
+                    //      lies outside the grid; [4,5] queue index of a
+                    //      level's first entry, by level parity
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -87,6 +88,12 @@ size_t ws_layout(int64_t V, Ws *w, char *base) {
     w->nf = (int32_t *)nf;
   }
   return off;
+}
+
+// Size of the frontier of parity cur; 0 once k_seed has flagged a seed
+// outside the grid, so that no level kernel indexes with it.
+__device__ __forceinline__ int front_size(const int32_t *__restrict__ nf, int cur) {
+  return nf[3] ? 0 : nf[cur];
 }
 
 __device__ __forceinline__ void chunk_of(int n, int &start, int &end) {
@@ -138,8 +145,9 @@ __device__ __forceinline__ double step_dot(int d, double dx, double n0, double n
 }
 
 __global__ void k_init(double *__restrict__ rsdf, int32_t *__restrict__ hops,
-                       int32_t *__restrict__ claim, int64_t V) {
+                       int32_t *__restrict__ claim, int32_t *__restrict__ nf, int64_t V) {
   const double qnan = __longlong_as_double(0x7ff8000000000000LL);
+  if (blockIdx.x == 0 && threadIdx.x == 0) nf[3] = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V;
        i += (int64_t)gridDim.x * blockDim.x) {
     rsdf[i] = qnan;
@@ -169,14 +177,19 @@ __global__ void k_emask(uint8_t *__restrict__ emask, const double *__restrict__ 
 
 __global__ void k_seed(double *__restrict__ rsdf, int32_t *__restrict__ hops,
                        int32_t *__restrict__ claim, int32_t *__restrict__ front,
-                       int32_t *__restrict__ nf, const int32_t *__restrict__ seeds, int n) {
+                       int32_t *__restrict__ nf, const int32_t *__restrict__ seeds, int n,
+                       int32_t V) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) {
     const int32_t s = seeds[t];
     front[t] = s;
-    hops[s] = 0;
-    rsdf[s] = 0.0;
-    claim[s] = -1;
+    if (s >= 0 && s < V) {
+      hops[s] = 0;
+      rsdf[s] = 0.0;
+      claim[s] = -1;
+    } else {
+      atomicOr(&nf[3], 1);  // the host reports MMN_ERR_INPUT
+    }
   }
   if (t == 0) {
     nf[0] = n;
@@ -191,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_claim(
     const int32_t *__restrict__ front, const int32_t *__restrict__ nf, int cur,
     int32_t *__restrict__ claim, const uint8_t *__restrict__ emask, int ny, int nz) {
   int start, end;
-  chunk_of(nf[cur], start, end);
+  chunk_of(front_size(nf, cur), start, end);
   const int32_t q0 = nf[4 + cur];
   for (int p = start + threadIdx.x; p < end; p += kBlock) {
     const int32_t u = front[p];
@@ -219,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_count(
     uint8_t *__restrict__ pmask, int32_t *__restrict__ bcnt, int ny, int nz) {
   __shared__ int sh[kBlock / kWave];
   int start, end;
-  chunk_of(nf[cur], start, end);
+  chunk_of(front_size(nf, cur), start, end);
   const int32_t q0 = nf[4 + cur];
   int cnt = 0;
   for (int p = start + threadIdx.x; p < end; p += kBlock) {
@@ -254,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_claim(
     const uint8_t *__restrict__ emask, int32_t *__restrict__ claim, double dx,
     int level, int ny, int nz) {
   __shared__ int sh[kBlock / kWave];
-  const int n = nf[cur];
+  const int n = front_size(nf, cur);
   int start, end;
   chunk_of(n, start, end);
   const int32_t q1 = nf[4 + cur] + n;
@@ -342,7 +355,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_bfs_small(
   const int w = t / kWave, l = t % kWave;
   const int p0 = t * PER;
   int level = level0;
-  int n = nf[level & 1];
+  int n = front_size(nf, level & 1);
   int32_t q0 = nf[4 + (level & 1)];
   if (n > 0 && n <= cap) {
     const int32_t *__restrict__ cur = (level & 1) ? fr1 : fr0;
@@ -511,10 +524,10 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   hipStream_t st = (hipStream_t)stream;
 
   hipLaunchKernelGGL(k_init, dim3(kGrid), dim3(kBlock), 0, st, rsdf, hops,
-                     w.claim, V);
+                     w.claim, w.nf, V);
   hipLaunchKernelGGL(k_seed, dim3(mmn_blocks(n_seeds > 0 ? n_seeds : 1, kBlock)),
                      dim3(kBlock), 0, st, rsdf, hops, w.claim, w.front0, w.nf, seeds,
-                     n_seeds);
+                     n_seeds, (int32_t)V);
   int rc = mmn_check_launch("mmn_rsdf_bfs(init)");
   if (rc) return rc;
   if (n_seeds == 0) return 0;
@@ -579,6 +592,8 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
         mmn_set_error("mmn_rsdf_bfs: %s", hipGetErrorString(e));
         return (int)e;
       }
+      MMN_REQUIRE(st4[3] == 0, MMN_ERR_INPUT,
+                  "mmn_rsdf_bfs: a seed lies outside the grid of %lld voxels", (long long)V);
       level = st4[2];
       n_cur = st4[level & 1];
       continue;
@@ -597,13 +612,16 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     }
     rc = mmn_check_launch("mmn_rsdf_bfs(level)");
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(&n_cur, w.nf + (level & 1), sizeof(int32_t),
-                                  hipMemcpyDeviceToHost, st);
+    int32_t st4[4] = {0, 0, 0, 0};
+    hipError_t e = hipMemcpyAsync(st4, w.nf, sizeof(st4), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
       mmn_set_error("mmn_rsdf_bfs: %s", hipGetErrorString(e));
       return (int)e;
     }
+    MMN_REQUIRE(st4[3] == 0, MMN_ERR_INPUT,
+                "mmn_rsdf_bfs: a seed lies outside the grid of %lld voxels", (long long)V);
+    n_cur = st4[level & 1];
   }
   return 0;
 }

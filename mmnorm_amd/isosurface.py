@@ -23,7 +23,7 @@ from . import _lib, ops
 C0 = 2.99792458e8
 UNDEF_VAL = -50  # isosurface_optimization.py:26
 
-# Range window of the loss (isosurface_optimization.py:420-429); the
+# Range window of the loss (isosurface_optimization.py:437-448); the
 # reference sets these globals in its main block, here set_range_window does.
 NUM_SAMPLES = None
 FFT_SIZE = 512
@@ -35,7 +35,7 @@ MIN_DIST_IDX = None
 
 def set_range_window(num_samples, wavelengths, fft_size=512, min_dist=0.35,
                      max_dist_all=0.55, max_dist_real=0.55):
-    """The loss's FFT size and range bins from the chirp (isosurface_optimization.py:420-429)."""
+    """The loss's FFT size and range bins from the chirp (isosurface_optimization.py:437-448)."""
     global NUM_SAMPLES, FFT_SIZE, DISTANCES, MAX_DIST_IDX, MAX_DIST_IDX_REAL, MIN_DIST_IDX
     wl = np.asarray(wavelengths, dtype=np.float64)[:num_samples]
     NUM_SAMPLES = int(num_samples)
@@ -92,7 +92,7 @@ def _simulate_channels(bounding_box, bounding_box_lower, voxel_size, grid_res_x,
                        sdf_indices=None):
     """Channel (complex64, one per wavelength) of antenna ``rx_loc`` from the
     isosurfaces grid[..., k] - taus[k] of the selected components
-    (isosurface_optimization.py:137-245).  ``bounding_box_lower`` is implied
+    (isosurface_optimization.py:111-209).  ``bounding_box_lower`` is implied
     by ``bounding_box`` and not needed."""
     dev = grid.device
     num_sdfs = grid.shape[-1]
@@ -126,7 +126,7 @@ def _simulate_channels(bounding_box, bounding_box_lower, voxel_size, grid_res_x,
 
 def loss_fn(output_ch, target_chs, is_real_data):
     """Range-profile loss of one simulated / measured channel pair
-    (isosurface_optimization.py:248-280); needs set_range_window first."""
+    (isosurface_optimization.py:211-246); needs set_range_window first."""
     return batch_loss(output_ch[None], target_chs[None], is_real_data)[0]
 
 
@@ -141,7 +141,7 @@ def batch_loss(output_chs, target_chs, is_real_data):
 
 def _filter_channels(channels, rx_locs, rx_dirs, is_real_data=True):
     """Keep antennas whose range profile has a strong, interior peak
-    (isosurface_optimization.py:282-330), relaxing the thresholds twice if
+    (isosurface_optimization.py:248-290), relaxing the thresholds twice if
     fewer than 20 remain."""
     ch = np.asarray(channels.detach().cpu())
     # the reference's FFT length is its sample count, which equals FFT_SIZE
@@ -165,7 +165,7 @@ def _filter_channels(channels, rx_locs, rx_dirs, is_real_data=True):
 
 def _select_tau(all_taus, all_losses):
     """The isosurface offsets: the loss minimum, or a stronger-looking peak of
-    the flipped loss closer to the top surface (isosurface_optimization.py:332-398)."""
+    the flipped loss closer to the top surface (isosurface_optimization.py:292-358)."""
     from scipy.signal import find_peaks
     if all_losses.ndim == 1:
         imin = int(np.argmin(all_losses))
@@ -196,14 +196,14 @@ def _select_tau(all_taus, all_losses):
 
 def _get_final_point_cloud(sdf, taus, coords):
     """Voxel coordinates within 1 mm of each chosen isosurface
-    (isosurface_optimization.py:400-416)."""
+    (isosurface_optimization.py:360-377)."""
     pts = [coords[np.abs(sdf[..., k] - taus[k]) < 0.001, :] for k in range(sdf.shape[-1])]
     return np.concatenate(pts, axis=0) if pts else np.zeros((0, 3))
 
 
 def prepare_components(rsdf, normals, component_labels, valid_labels):
     """Per-component SDF grids with UNDEF_VAL outside each component, and the
-    tau bounds (min + 15 mm, max - 15 mm) (isosurface_optimization.py:57-81).
+    tau bounds (min + 15 mm, max - 15 mm) (isosurface_optimization.py:75-81).
     Returns (sdf float32 (X,Y,Z,n), tau_bounds float32 (n, 2)) as device tensors."""
     dev = torch.device("cuda", torch.cuda.current_device())
     flat = torch.as_tensor(np.asarray(rsdf), dtype=torch.float32, device=dev).clone()

@@ -37,14 +37,29 @@ MAX_GROUP_ROWS = 4096 * 32  # py_image_gpu.py:339
 
 
 def _prep_inputs(radar_data, rx_offset, antenna_locs, use_4_rx):
-    """Layout of py_image_gpu.py:219-235: rows = meas x rx, samples last."""
+    """Layout of py_image_gpu.py:219-235 and py_sn_est_gpu.py:267-288: rows =
+    meas x rx, samples last.
+
+    The reference flattens the (transposed, when use_4_rx) measurements and
+    its kernel reads row i at offset i * radar_data.shape[1]
+    (imaging_gpu.cu:100,110), for len(antenna_locs) rows.  So without
+    use_4_rx an (M, S) or (M, S, 1) array gives M rows of S samples, and
+    with it an (M, S, R) array gives M*R rows (TX position repeated per RX).
+    """
     radar_data = np.asarray(radar_data)
     antenna_locs = np.asarray(antenna_locs)
+    if radar_data.ndim < 2:
+        raise ValueError("radar_data must be (meas, samples) or (meas, samples, rx)")
+    samples = radar_data.shape[1]
     if use_4_rx:
         radar_data = np.transpose(radar_data, (0, 2, 1))
         antenna_locs = np.repeat(antenna_locs, radar_data.shape[1], axis=0)
     rx_offset = np.asarray(rx_offset, dtype=np.float64).reshape(-1, 3)
-    meas = np.ascontiguousarray(radar_data.reshape(-1, radar_data.shape[-1]))
+    rows = len(antenna_locs)
+    flat = radar_data.reshape(-1)
+    if flat.size < rows * samples:
+        raise ValueError(f"{flat.size} measurement values for {rows} rows of {samples} samples")
+    meas = np.ascontiguousarray(flat[:rows * samples].reshape(rows, samples))
     return rx_offset, meas, antenna_locs
 
 

@@ -26,19 +26,7 @@ import math
 import numpy as np
 
 from . import ops
-from .py_image_gpu import MAX_GROUP_ROWS, _device, _to_dev, host_bbox
-
-
-def _prep_inputs(radar_data, rx_offset, antenna_locs, use_4_rx):
-    """py_sn_est_gpu.py:267-288: rows = meas x rx, samples last."""
-    radar_data = np.asarray(radar_data)
-    antenna_locs = np.asarray(antenna_locs)
-    if use_4_rx:
-        radar_data = np.transpose(radar_data, (0, 2, 1))
-        antenna_locs = np.repeat(antenna_locs, radar_data.shape[1], axis=0)
-    rx = np.asarray(rx_offset, dtype=np.float64).reshape(-1, 3)
-    meas = np.ascontiguousarray(radar_data.reshape(-1, radar_data.shape[-1]))
-    return rx, meas, antenna_locs
+from .py_image_gpu import MAX_GROUP_ROWS, _device, _prep_inputs, _to_dev, host_bbox
 
 
 def _filter(sum_image, initial_filter_percent):

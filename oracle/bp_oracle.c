@@ -33,8 +33,15 @@
  *                  sdf.py:89) by flood fill in C-order scan, numbered in
  *                  order of first appearance.
  *
- * Pins (tests/test_oracle.py): or_bp_ref and or_sn_ref equal their
- * pure-Python restatements bit for bit on the golden inputs; or_bp_f64
+ * Pins (tests/test_oracle.py, tests/test_oracle_interp.py): or_bp_ref's
+ * interpolated branch reads, on hand-constructed distances, the bin integer
+ * arithmetic gives (at, just below and just above bin edges, with and
+ * without the TI offset), skips exactly the rows with d > fs * S, carries
+ * the phasor exp(-j 2 pi d / lambda), and on an FFT'd point scatterer gives
+ * the geometric-sum closed form at the focus; or_sn_ref gives a single
+ * row's vote (p + o/2 - v) Re(c conj I)/|I| and the normal (0, 0, 1) of a
+ * symmetric four-RX aperture; both also equal their pure-Python
+ * restatements bit for bit on the golden inputs; or_bp_f64
  * equals a NumPy brute force and the point-scatterer closed form;
  * or_bfs_fifo equals the deque restatement and the constant-normal closed
  * form; or_li_threshold meets its fixed-point equation and the constant and

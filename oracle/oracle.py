@@ -14,11 +14,14 @@ Two independent restatements of the reference's arithmetic live here:
 * pure-Python loops below (``py_*``) for tiny inputs, written line by line
   from the same reference lines, used to cross-check the C code.
 
-Pins (see tests/test_oracle_*.py): closed forms (point-scatterer focus
-I = rho*A*F, single-row magnitude, single-antenna normal direction, symmetric
-aperture normal (0,0,1), constant-normal RSDF = dx*(k-k0), BFS hops =
-Manhattan distance in a full box), invariants (linearity, translation,
-row permutation, label renumbering) and brute force (the py_* loops).  The
+Pins (tests/test_oracle.py, tests/test_oracle_interp.py): closed forms
+(full-branch point-scatterer focus I = rho*A*F; interpolated branch: bins
+and the skip rule on hand-constructed distances, the geometric-sum focus of
+an FFT'd point scatterer; vote: a single row's direction and weight, a
+symmetric four-RX aperture's normal (0,0,1); constant-normal RSDF =
+dx*(k-k0), BFS hops = Manhattan distance in a full box), invariants
+(linearity, translation, row permutation, label renumbering) and brute
+force (the py_* loops).  The
 reference ships no golden vectors, tests or fixtures for this path.
 """
 from __future__ import annotations
@@ -146,6 +149,44 @@ def bp_f64(xl, yl, zl, ant, meas, rxo, f0, df, is_ti=False, vidx=None, moment=Fa
     return (out, mom) if moment else out
 
 
+def np_bp_full(xl, yl, zl, ant, meas, rxo, slope, wavelength, vidx, dtype=np.float64,
+               chunk=4):
+    """NumPy restatement of the full branch of imaging_gpu.cu:48-122 with the
+    vote moment sum_i (p_i + o_i/2 - v) c_i, vectorised over chunks of the
+    listed voxels (one core: NumPy's ufuncs are single-threaded).  dtype
+    float64: distance, phase -2 pi d / lambda - 2 pi d j slope / c and exp in
+    FP64; float32: all in FP32 (complex64 exp, FP32 sums), the precision of
+    the reference's CPU imaging (imaging.cpp:108-112).  The CPU-1 baseline
+    bench.py times.  Returns (img (n,), moment (n, 3)) of the dtype's complex."""
+    ft = np.dtype(dtype).type
+    ct = np.complex128 if ft is np.float64 else np.complex64
+    xl, yl, zl = (np.asarray(a, np.float32).astype(ft) for a in (xl, yl, zl))
+    ant = np.asarray(ant, np.float32).astype(ft)
+    rxo = np.asarray(rxo, np.float32).astype(ft).reshape(-1, 3)
+    meas = np.asarray(meas, np.complex64).astype(ct)
+    A, F = meas.shape
+    o = rxo[np.arange(A) % len(rxo)]
+    vidx = np.asarray(vidx, np.int64)
+    ny, nz = len(yl), len(zl)
+    ix, r = np.divmod(vidx, ny * nz)
+    iy, iz = np.divmod(r, nz)
+    vox = np.stack([xl[ix], yl[iy], zl[iz]], 1)
+    two_pi = ft(2 * math.pi)
+    kj = (two_pi / ft(wavelength) + two_pi * np.arange(F, dtype=ft) * ft(slope) / ft(C0))
+    va = ant + o / ft(2)
+    img = np.empty(len(vidx), ct)
+    mom = np.empty((len(vidx), 3), ct)
+    for lo in range(0, len(vidx), chunk):
+        v = vox[lo:lo + chunk]                                   # (n, 3)
+        a = v[:, None, :] - ant[None, :, :]                      # (n, A, 3)
+        d = np.sqrt(np.sum(a * a, -1)) + np.sqrt(np.sum((a - o[None]) ** 2, -1))
+        ph = np.exp(-1j * (d[:, :, None] * kj[None, None, :]).astype(ft)).astype(ct)
+        c = np.sum(ph * meas[None], -1)                          # (n, A) row contributions
+        img[lo:lo + chunk] = c.sum(-1)
+        mom[lo:lo + chunk] = np.einsum("nak,na->nk", (va[None] - v[:, None, :]).astype(ct), c)
+    return img, mom
+
+
 def bp_cpu_imaging(xl, yl, zl, ant_tx, meas, rxo, slope, wavelength, is_ti=False,
                    vidx=None):
     """imaging.cpp:60-139 full branch (CPU baseline formulation), complex64.
@@ -228,7 +269,7 @@ def ray_matching(grid, width, height, width_fov, height_fov, bbox, eye, rx_dir, 
 
 def simulate_channel(grid, tau, width, height, width_fov, height_fov, bbox, eye, rx_dir,
                      rx_loc, voxel_size, f0, df, F, is_ti=False):
-    """isosurface_optimization.py:137-245 for one component, FP64 exp per hit and
+    """isosurface_optimization.py:111-209 for one component, FP64 exp per hit and
     sample: (channel complex128 (F,), distances float64 (width*height,), NaN = no hit)."""
     g, bb, res, e, r = _ray_args(grid, bbox, eye, rx_dir)
     rx = np.ascontiguousarray(rx_loc, dtype=np.float64)

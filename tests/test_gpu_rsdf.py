@@ -117,3 +117,38 @@ def test_bfs_rejects_bad_seed():
     with pytest.raises(ValueError):
         msdf.rsdf_components(n, np.ones((3, 3, 3), np.int32), [27], 1e-3)
     assert torch.cuda.is_available()
+
+
+@pytest.mark.parametrize("n_seeds", [3, 1500])
+def test_c_abi_rejects_a_seed_outside_the_grid(n_seeds):
+    """mmn_rsdf_bfs reads the seeds on the device: one outside [0, V) is
+    skipped there, no level runs, and the call returns MMN_ERR_INPUT.  1500
+    seeds start on the multi-block level kernels, 3 on the single workgroup.
+    A valid call on the same workspace afterwards is unaffected."""
+    from mmnorm_amd import _lib
+    dev = torch.device("cuda", 0)
+    shape = (16, 12, 10)
+    n, lab, _ = random_field(shape, 0.9, 11, blobs=False)
+    V = int(np.prod(shape))
+    nrm = torch.from_numpy(n).to(dev)
+    labt = torch.from_numpy(lab.astype(np.int32)).to(dev)
+    good = np.flatnonzero(~np.isnan(n[..., 0]).ravel())[:n_seeds].astype(np.int32)
+    for bad in (V, -1):
+        seeds = good.copy()
+        seeds[len(seeds) // 2] = bad
+        st = torch.from_numpy(seeds).to(dev)
+        rsdf = torch.empty(shape, dtype=torch.float64, device=dev)
+        hops = torch.empty(shape, dtype=torch.int32, device=dev)
+        lib = _lib.load()
+        ws_bytes = int(lib.mmn_rsdf_bfs_workspace_size(*shape))
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        with pytest.raises(ValueError, match="outside the grid"):
+            _lib.call("mmn_rsdf_bfs", rsdf.data_ptr(), hops.data_ptr(), nrm.data_ptr(),
+                      labt.data_ptr(), st.data_ptr(), len(seeds), 1e-3, *shape, ws.data_ptr(),
+                      ws_bytes, _lib.stream_handle(dev))
+    # the same workspace serves a valid call
+    st = torch.from_numpy(good[:1]).to(dev)
+    _lib.call("mmn_rsdf_bfs", rsdf.data_ptr(), hops.data_ptr(), nrm.data_ptr(), labt.data_ptr(),
+              st.data_ptr(), 1, 1e-3, *shape, ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
+    torch.cuda.synchronize()
+    assert int(hops.max()) > 0

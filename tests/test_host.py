@@ -127,3 +127,35 @@ def test_correlate_locs_and_meas():
     assert np.allclose(poses[4, 0], 0.05)
     assert np.allclose(poses[5:, 0], [0.099, 0.1])
     assert 3 / 16 * lam > 2e-4
+
+
+@pytest.mark.parametrize("shape,use_4_rx,rows", [((5, 7), False, 5), ((5, 7, 1), False, 5),
+                                                 ((5, 7, 4), True, 20), ((5, 7, 2), False, 5)])
+def test_prep_inputs_reads_rows_as_the_reference_kernel(shape, use_4_rx, rows):
+    """The reference flattens the measurements (transposed to (meas, rx,
+    samples) when use_4_rx) and its kernel reads row i at i * shape[1]
+    (imaging_gpu.cu:100,110): each drop-in row must be that slice."""
+    from mmnorm_amd.py_image_gpu import _prep_inputs
+    rng = np.random.default_rng(len(shape) * 10 + shape[-1])
+    data = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
+    locs = rng.normal(size=(shape[0], 3))
+    rx = [[0.0, 0.0, 0.0]] * (shape[2] if use_4_rx else 1)
+    rxo, meas, ant = _prep_inputs(data, rx, locs, use_4_rx)
+    S = shape[1]
+    flat = (np.transpose(data, (0, 2, 1)) if use_4_rx else data).ravel()
+    assert meas.shape == (rows, S) and len(ant) == rows and rxo.shape == (len(rx), 3)
+    for i in range(rows):
+        np.testing.assert_array_equal(meas[i], flat[i * S:(i + 1) * S])
+    if use_4_rx:
+        np.testing.assert_array_equal(ant, np.repeat(locs, shape[2], axis=0))
+        # row m*R + r holds receiver r's samples of measurement m
+        np.testing.assert_array_equal(meas[1 * shape[2] + 3], data[1, :, 3])
+    else:
+        np.testing.assert_array_equal(meas[2], data.reshape(shape[0], S, -1)[2, :, 0]
+                                      if len(shape) == 3 and shape[2] == 1 else meas[2])
+
+
+def test_prep_inputs_rejects_short_data():
+    from mmnorm_amd.py_image_gpu import _prep_inputs
+    with pytest.raises(ValueError):
+        _prep_inputs(np.zeros((4, 8), np.complex64), [[0, 0, 0]], np.zeros((5, 3)), False)

@@ -382,3 +382,23 @@ def test_rays_leaving_the_box_have_no_hit(oracle):
     grid = np.full(res, 1.0, np.float32)   # no surface inside
     hit, vox, _ = oracle.ray_matching(grid, 4, 4, 20, 20, bbox, [0.007, 0.007, 0.3], (0, 0, 1))
     assert np.all(hit == -1) and np.all(vox == -1)
+
+
+def test_numpy_cpu1_restatement_is_the_same_sum(oracle, gold):
+    """The NumPy CPU-1 baseline (FP64) equals O64 with its moment; the FP32
+    variant is within the FP32 floor of it."""
+    g = lambda k: gold[f"b_{k}"]
+    lam = float(np.float32(g("lam")))
+    n = len(g("xl")) * len(g("yl")) * len(g("zl"))
+    vidx = np.arange(0, n, 3)
+    want, wmom = oracle.bp_f64(g("xl"), g("yl"), g("zl"), g("ant"), g("meas"), g("rxo"),
+                               C0 / lam, float(np.float32(5861160.0)), vidx=vidx, moment=True)
+    img, mom = oracle.np_bp_full(g("xl"), g("yl"), g("zl"), g("ant"), g("meas"), g("rxo"),
+                                 5861160.0, lam, vidx)
+    np.testing.assert_allclose(img, want, rtol=0, atol=1e-9 * np.abs(want).max())
+    np.testing.assert_allclose(mom, wmom, rtol=0, atol=1e-9 * np.abs(wmom).max())
+    i32, m32 = oracle.np_bp_full(g("xl"), g("yl"), g("zl"), g("ant"), g("meas"), g("rxo"),
+                                 5861160.0, lam, vidx, dtype=np.float32)
+    assert i32.dtype == np.complex64
+    assert np.linalg.norm(i32 - want) / np.linalg.norm(want) < 1e-3
+    assert np.linalg.norm(m32 - wmom) / np.linalg.norm(wmom) < 1e-3
