@@ -104,15 +104,15 @@ int mmn_cuda_sn_est(float *normals, const int32_t *vox_idx, int64_t n_valid,
  * No reference counterpart for the fused moment or the variants; the
  * reference computes the vote only for the interpolated branch
  * (py_sn_est_gpu.py:404-405).
- * workspace (optional, may be NULL): lets the voxel blocks that do not fill
- * a whole round of resident workgroups (all of them for a small voxel set,
- * else those of the last, partial round) be split over row segments whose
- * partial sums are reduced in a fixed order, so that the device stays full;
- * mmn_bp_forward_workspace_size() gives exactly the bytes of the split the
- * launch chooses on the current device (0 when it needs none; without a
- * device it assumes MI355X: 256 CUs, 8 workgroups each).  A smaller
- * workspace gives a coarser split or none.  Results are deterministic for a
- * given workspace size and device.
+ * workspace (optional, may be NULL): when the voxel set gives fewer than
+ * eight rounds of resident workgroups, lets every voxel block be split over
+ * row segments (at least 32 rows each) whose partial sums are reduced in a
+ * fixed order, so that the device stays full and each workgroup's window of
+ * measurement rows stays in L2; mmn_bp_forward_workspace_size() gives
+ * exactly the bytes of the split the launch chooses on the current device
+ * (0 when it needs none; without a device it assumes MI355X: 256 CUs, 8
+ * workgroups each).  A smaller workspace gives a coarser split or none.
+ * Results are deterministic for a given workspace size and device.
  */
 size_t mmn_bp_forward_workspace_size(int64_t n_vox, int num_rows, int with_moment);
 int mmn_bp_forward(float *img, float *moment, const float *x_locs,

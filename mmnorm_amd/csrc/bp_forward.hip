@@ -558,13 +558,19 @@ int64_t resident_slots(KernelT kernel) {
   return (int64_t)per_cu * cus;
 }
 
-// Work split of one recurrence launch (see k_bp_rec).  Whole rounds of
-// resident workgroups take every row and write the image directly; only the
-// voxel blocks of a last, partial round are split over row segments, enough
-// of them for one more full round, so that the tail does not run the
-// device half empty, and only their partial sums pass through the
-// workspace.  A voxel set smaller than one round is split entirely, aiming
-// at eight rounds.  MMN_REC_RS = n forces every block into n segments.
+// Work split of one recurrence launch (see k_bp_rec).  When the voxel
+// blocks give fewer than eight rounds of resident workgroups, every block is
+// split over the fewest row segments that give eight (at least 32 rows per
+// segment, at most 32 segments), and the segments' partial sums pass
+// through the workspace.  Measured on MI355X at 128x128x96 voxels and 3600
+// rows (3072 blocks, 1792 resident): every block in 4-9 segments 111-113 ms,
+// only the last partial round split 114.5 ms, no split 121.9 ms.  Splitting
+// every block also keeps each workgroup's window of measurement rows (720
+// rows, 1.5 MB at five segments) inside an XCD's 4 MB L2: FETCH_SIZE read
+// 59 MB per launch, one pass over the rows per XCD, against 378 / 489 MB for
+// the other two.  MMN_REC_RS = n forces n segments.
+// The kernel's mapping also allows whole blocks and split blocks in one
+// launch; the plan uses one or the other.
 struct RecPlan {
   int64_t nmain = 0, ntail = 0;  // voxel blocks taking all rows / split
   int nseg = 1, rps = 0;
@@ -585,12 +591,9 @@ RecPlan plan_rec(int64_t nvox, int rows, bool moment, int vpt, int64_t slots,
   if (forced > 0) {
     nseg = forced;
     ntail = nblk;
-  } else if (nblk < slots) {
+  } else if (nblk < 8 * slots) {
     ntail = nblk;
     while (ntail * nseg < 8 * slots && nseg < 64) ++nseg;
-  } else {
-    ntail = nblk % slots;
-    if (ntail > 0) nseg = (int)std::min<int64_t>(64, (slots + ntail - 1) / ntail);
   }
   nseg = std::min(nseg, rs_cap(rows));
   for (; nseg > 1; --nseg) {

@@ -52,15 +52,14 @@ def test_workspace_queries_and_argument_errors_without_gpu():
     assert lib.mmn_bp_forward_workspace_size(1000, 20, 0) == 0
     # Without a device the plan assumes 256 CUs x 8 resident workgroups of 512
     # voxels.  128x128x96 voxels, 3600 rows, with the moment: 3072 voxel
-    # blocks, one full round of 2048 takes every row, the last 1024 blocks
-    # (524,288 voxels) are split in two row segments of 32-byte partials.
-    assert lib.mmn_bp_forward_workspace_size(128 * 128 * 96, 3600, 1) == 2 * 524288 * 32
-    # 256x256x128 voxels: exactly 8 rounds, no split, no workspace
+    # blocks, fewer than 8 rounds of 2048, so every block is split into the 6
+    # row segments that give 8 rounds, with 32-byte partials per voxel.
+    assert lib.mmn_bp_forward_workspace_size(128 * 128 * 96, 3600, 1) == 6 * 128 * 128 * 96 * 32
+    assert lib.mmn_bp_forward_workspace_size(128 * 128 * 96, 3600, 0) == 6 * 128 * 128 * 96 * 8
+    # 256x256x128 voxels: 16384 blocks are 8 rounds already, no split, no workspace
     assert lib.mmn_bp_forward_workspace_size(256 * 256 * 128, 10000, 1) == 0
     # the 550x600x250 real-data grid with 16 rows: too few rows to split
     assert lib.mmn_bp_forward_workspace_size(550 * 600 * 250, 16, 1) == 0
-    # ... and with 4096 rows one round of 2048 blocks is a multiple: none either
-    assert lib.mmn_bp_forward_workspace_size(2048 * 512 * 40, 4096, 1) == 0
     # argument checks run before any HIP call
     with pytest.raises(ValueError, match="NULL"):
         _lib.call("mmn_bp_forward", None, None, None, None, None, None, 0, None, None, None,

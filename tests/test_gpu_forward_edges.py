@@ -112,39 +112,41 @@ def test_interp_bins_at_bin_boundaries(oracle):
     assert np.max(np.abs(fast - ref) / np.abs(ref)) <= 2e-6
 
 
-def test_tail_round_split_vs_fp64(oracle):
-    """128x128x80 voxels (2560 blocks of 512): one full round of 2048 blocks
-    takes all 64 rows, the last 512 blocks run as two row segments whose
-    partials are added afterwards.  Voxels on both sides of the boundary,
-    with the workspace the size query gives, and with none (no split)."""
+def test_row_split_with_the_queried_workspace_vs_fp64(oracle):
+    """128x128x80 voxels (2560 blocks of 512, fewer than 8 rounds), 64 rows:
+    the launch splits every block into two row segments.  With exactly the
+    workspace the size query gives, with one byte less (no split), and with
+    none, the image and moment agree with O64 on a voxel sample, and the split
+    and unsplit images with each other to rounding."""
     from mmnorm_amd import _lib
     cfg = synth.Config(1, 8, 2e-3, 16, (128, 128, 80), 1e-3, 20.0, "points", 5)
     sc = synth.make_scene(1, cfg=cfg, seed=7)
     n_grid = 128 * 128 * 80
     lib = _lib.load()
     ws_bytes = int(lib.mmn_bp_forward_workspace_size(n_grid, len(sc.ant), 1))
-    args = (T(sc.meas, np.complex64), T(sc.ant, np.float32), T(sc.x_locs, np.float32),
-            T(sc.y_locs, np.float32), T(sc.z_locs, np.float32))
-    img, mom = ops.backproject(*args, f0=sc.f0, df=sc.df, moment=True)
-    off = n_grid - ws_bytes // (2 * 32) if ws_bytes else n_grid
+    assert ws_bytes == 2 * n_grid * 32
+    meas, ant = T(sc.meas, np.complex64), T(sc.ant, np.float32)
+    xl, yl, zl = T(sc.x_locs, np.float32), T(sc.y_locs, np.float32), T(sc.z_locs, np.float32)
+    rx = torch.zeros((1, 3), dtype=torch.float32, device=dev())
     rng = np.random.default_rng(3)
-    vi = np.unique(np.concatenate([np.arange(off - 3, off + 3), [0, n_grid - 1],
-                                   rng.choice(n_grid, 200, replace=False)])).astype(np.int64)
+    vi = np.unique(np.concatenate([[0, n_grid - 1], rng.choice(n_grid, 200, replace=False)]))
     want, wmom = oracle.bp_f64(sc.x_locs, sc.y_locs, sc.z_locs, sc.ant, sc.meas,
                                np.zeros((1, 3)), sc.f0, sc.df, vidx=vi, moment=True)
-    got = img.reshape(-1)[torch.from_numpy(vi).to(dev())].cpu().numpy()
-    gmom = mom.reshape(-1, 3)[torch.from_numpy(vi).to(dev())].cpu().numpy()
-    assert rel(got, want) <= 1e-5 and rel(gmom, wmom) <= 1e-5
-    # no workspace: every block takes every row; same sums to rounding
-    img0 = torch.empty_like(img)
-    xl, yl, zl = args[2:]
-    _lib.call("mmn_bp_forward", img0.data_ptr(), None, xl.data_ptr(), yl.data_ptr(),
-              zl.data_ptr(), None, 0, args[1].data_ptr(), args[0].data_ptr(),
-              torch.zeros((1, 3), dtype=torch.float32, device=dev()).data_ptr(), float(sc.f0),
-              float(sc.df), 128, 128, 80, len(sc.ant), 1, sc.meas.shape[1], 0, 2, 64, None, 0,
-              _lib.stream_handle(dev()))
-    torch.cuda.synchronize()
-    assert rel(img0.cpu().numpy(), img.cpu().numpy()) <= 1e-6
-    # the main-round voxels are computed the same way in both: bit-identical
-    main = min(off, n_grid)
-    assert torch.equal(img0.reshape(-1)[:main], img.reshape(-1)[:main])
+    sel = torch.from_numpy(vi).to(dev())
+    imgs = []
+    for nbytes in (ws_bytes, ws_bytes - 1, 0):
+        img = torch.empty(n_grid, dtype=torch.complex64, device=dev())
+        mom = torch.empty((n_grid, 3), dtype=torch.complex64, device=dev())
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev())
+        _lib.call("mmn_bp_forward", img.data_ptr(), mom.data_ptr(), xl.data_ptr(),
+                  yl.data_ptr(), zl.data_ptr(), None, 0, ant.data_ptr(), meas.data_ptr(),
+                  rx.data_ptr(), float(sc.f0), float(sc.df), 128, 128, 80, len(sc.ant), 1,
+                  sc.meas.shape[1], 0, 2, 64, ws.data_ptr() if nbytes else None, nbytes,
+                  _lib.stream_handle(dev()))
+        torch.cuda.synchronize()
+        assert rel(img[sel].cpu().numpy(), want) <= 1e-5
+        assert rel(mom[sel].cpu().numpy(), wmom) <= 1e-5
+        imgs.append(img.cpu().numpy())
+    assert rel(imgs[0], imgs[2]) <= 1e-6
+    # too small a workspace and none both run unsplit: the same bits
+    assert np.array_equal(imgs[1], imgs[2])
