@@ -253,6 +253,26 @@ int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
                  size_t workspace_bytes, void *stream);
 
 /*
+ * mmn_rsdf_bfs_noalloc -- mmn_rsdf_bfs with no workspace argument and no
+ * allocation of any kind: its scratch is a device array of
+ * MMN_RSDF_BFS_SCRATCH_BYTES (128 MiB) built into the library, one per GPU,
+ * held from the first call on that GPU until the process ends.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (serving loops and harnesses that forbid hipMalloc).  Same
+ * arguments, outputs, bit-exact results and stream synchronisation as
+ * mmn_rsdf_bfs; the stream must belong to the current device.  A grid whose
+ * scratch exceeds the array (mmn_rsdf_bfs_workspace_size(nx, ny, nz) >
+ * MMN_RSDF_BFS_SCRATCH_BYTES, about 9.5 M voxels) returns MMN_ERR_ARG with
+ * nothing launched.  All calls on one GPU share its array: calls that
+ * overlap in time on one GPU (two host threads, or two streams) race and
+ * must not be made.
+ */
+#define MMN_RSDF_BFS_SCRATCH_BYTES ((size_t)128 << 20)
+int mmn_rsdf_bfs_noalloc(double *rsdf, int32_t *hops, const double *normals,
+                         const int32_t *labels, const int32_t *seeds, int n_seeds,
+                         double dx, int nx, int ny, int nz, void *stream);
+
+/*
  * mmn_li_threshold -- replaces skimage.filters.threshold_li as called by
  * _separate_binary_image (sdf.py:46-70) on the image |img|^exponent.
  * img: complex [n] (finite values).  With a = |img|^exponent (FP64) and

@@ -64,6 +64,8 @@ SIGNATURES = {
     "mmn_rsdf_bfs_workspace_size": (_sz, [_c_int, _c_int, _c_int]),
     "mmn_rsdf_bfs": (_c_int, [_p, _p, _p, _p, _p, _c_int, _c_d, _c_int, _c_int,
                               _c_int, _p, _sz, _p]),
+    "mmn_rsdf_bfs_noalloc": (_c_int, [_p, _p, _p, _p, _p, _c_int, _c_d, _c_int, _c_int,
+                                      _c_int, _p]),
     "mmn_li_threshold_workspace_size": (_sz, [_c_i64]),
     "mmn_li_threshold": (_c_int, [_p, _p, _p, _p, _p, _c_i64, _c_d, _c_int, _p, _sz,
                                   _p]),

@@ -39,6 +39,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "mmn_internal.h"
 
@@ -624,4 +625,76 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     n_cur = st4[level & 1];
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation variant.  The BFS's scratch (claims, the two frontiers, the
+// won and enterable masks, the per-block counts and the level counters) sits
+// in one file-scope device array instead of a caller workspace.  Every GPU
+// the library runs on holds its own copy of the array until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device
+// memory: the device address comes from hipGetSymbolAddress, resolved once
+// per device on the first call made with that device current, and only that
+// address is passed to the kernels.  Everything the kernels read is written
+// first by k_init / k_seed / k_emask / the level kernels within the same
+// call, so the contents a previous call left behind never reach a result.
+// ---------------------------------------------------------------------------
+static constexpr size_t kBfsScratchBytes = MMN_RSDF_BFS_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kBfsScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (resolved on first use).
+int scratch_on_current_device(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    mmn_set_error("mmn_rsdf_bfs_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  MMN_REQUIRE(dev >= 0 && dev < kMaxDevices, MMN_ERR_ARG,
+              "mmn_rsdf_bfs_noalloc: device %d beyond the %d this library tracks", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      mmn_set_error("mmn_rsdf_bfs_noalloc: hipGetSymbolAddress(scratch): %s",
+                    hipGetErrorString(e));
+      return e != hipSuccess ? (int)e : (int)hipErrorInvalidSymbol;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int mmn_rsdf_bfs_noalloc(double *rsdf, int32_t *hops, const double *normals,
+                                    const int32_t *labels, const int32_t *seeds,
+                                    int n_seeds, double dx, int nx, int ny, int nz,
+                                    void *stream) {
+  MMN_REQUIRE(nx > 0 && ny > 0 && nz > 0, MMN_ERR_ARG,
+              "mmn_rsdf_bfs_noalloc: sizes must be positive");
+  const int64_t V = (int64_t)nx * ny * nz;
+  MMN_REQUIRE(V <= (int64_t)(INT32_MAX / 6), MMN_ERR_ARG,
+              "mmn_rsdf_bfs_noalloc: grid of %lld voxels exceeds the int32 key range",
+              (long long)V);
+  const size_t need = ws_layout(V, nullptr, nullptr);
+  MMN_REQUIRE(need <= kBfsScratchBytes, MMN_ERR_ARG,
+              "mmn_rsdf_bfs_noalloc: a grid of %lld voxels needs %zu bytes of scratch, more "
+              "than the %zu this entry point holds (about %lld voxels); use mmn_rsdf_bfs "
+              "with a workspace",
+              (long long)V, need, kBfsScratchBytes, (long long)(kBfsScratchBytes / 14));
+  void *ws = nullptr;
+  const int rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  return mmn_rsdf_bfs(rsdf, hops, normals, labels, seeds, n_seeds, dx, nx, ny, nz, ws,
+                      kBfsScratchBytes, stream);
 }

@@ -78,6 +78,9 @@ def test_workspace_queries_and_argument_errors_without_gpu():
                   1, 1, 1, 1, 1, 1, 0, 32, None, 0, None)
     with pytest.raises(ValueError, match="workspace"):
         _lib.call("mmn_rsdf_bfs", 8, 8, 8, 8, 8, 1, 1e-3, 4, 4, 4, None, 0, None)
+    # the no-allocation BFS refuses a grid beyond its 128 MiB scratch before any HIP call
+    with pytest.raises(ValueError, match="scratch"):
+        _lib.call("mmn_rsdf_bfs_noalloc", 8, 8, 8, 8, 8, 1, 1e-3, 2048, 2048, 4, None)
     assert lib.mmn_ccl6_workspace_size(4, 4, 4) >= 4 * 4 * 4 * 4
     assert lib.mmn_li_threshold_workspace_size(100) >= 800
     with pytest.raises(ValueError, match="workspace"):
