@@ -39,7 +39,9 @@ extern "C" {
 /* Backprojection variants accepted by mmn_bp_forward. */
 #define MMN_BP_PLAIN 1 /* direct FP32 sincosf per (voxel, row, sample)          */
 #define MMN_BP_REC   2 /* FP32 rotation (Horner) over samples, FP64 re-anchor
-                          every K samples, compensated FP32 accumulation       */
+                          every K samples; plain FP32 sums within a row (its
+                          K-sample blocks), Kahan-compensated FP32 sum over
+                          rows                                                 */
 #define MMN_BP_PLAIN_FAST 3 /* direct FP32: phase in revolutions, hardware sine
                                and cosine of its fraction                     */
 
