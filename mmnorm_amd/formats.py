@@ -96,3 +96,92 @@ def read_adc_bin(src, num_lanes=4):
         raise ValueError(f"{raw.size} values is not a multiple of {per}")
     rows = raw.reshape(raw.size // per, per)
     return rows[:, :num_lanes] + 1j * rows[:, num_lanes:]
+
+
+# ---------------------------------------------------------------------------
+# The dataset's raw captures, as GenericLoader reads them (generic_loader.py).
+# ---------------------------------------------------------------------------
+def dataset_path(root, obj_id, name, radar_type, is_sim=False, is_los=True, exp_num=1,
+                 angles=(0, 0, 0), is_processed=False, is_json=False,
+                 path_to_data="sample_data"):
+    """The folder GenericLoader._get_path names (generic_loader.py:36-67):
+    <root>/<path_to_data>/<id>_<name>/<simulation|robot_collected>/<x>_<y>_<z>/
+    <exp>/<los|nlos>/<processed|unprocessed>/radars/<radar_type> (the camera
+    folder and the experiment JSON folder as there)."""
+    if radar_type is None:
+        raise ValueError("You should not use Radar_type = None")
+    obj = f"{obj_id}_{name}"
+    sim = "simulation" if is_sim else "robot_collected"
+    ang = "_".join(str(a) for a in angles)
+    los = "" if is_sim or is_los is None else ("los" if is_los else "nlos")
+    proc = "processed" if is_processed else "unprocessed"
+    if exp_num in ("None", None):
+        exp_num = 1
+    exp = f"exp{exp_num}" if not is_sim else ""
+    base = f"{root}/{path_to_data}/{obj}/{sim}/{ang}"
+    if is_json:
+        return f"{base}/{exp}"
+    if radar_type == "camera":
+        return f"{base}/{exp}/{los}/{proc}/camera"
+    return f"{base}/{exp}/{los}/{proc}/radars/{radar_type}"
+
+
+def radar_file_timestamp(filename):
+    """The capture timestamp in a 77 GHz file name: characters 8-17, or 9-20
+    when those are not an integer (generic_loader.py:522-525)."""
+    try:
+        return int(filename[8:18])
+    except ValueError:
+        return int(filename[9:21])
+
+
+def load_radar_files(path, radar_type="77_ghz"):
+    """{timestamp: complex (samples, 4)} of every .bin capture in `path`, in
+    sorted file-name order (generic_loader.py:484-536, the 77 GHz branch)."""
+    if radar_type != "77_ghz":
+        raise ValueError("only the 77 GHz captures are read")
+    out = {}
+    for filename in sorted(os.listdir(path)):
+        if filename[-4:] != ".bin":
+            continue
+        out[radar_file_timestamp(filename)] = read_adc_bin(os.path.join(path, filename))
+    return out
+
+
+def load_robot_loc_files(path):
+    """The robot trajectory files of `path` (names starting 'antenna', numeric
+    .npz), concatenated key by key in sorted name order (generic_loader.py:
+    538-575).  A file whose time and position arrays differ in length is cut,
+    as the reference cuts it: once per key of the file, both to the shorter
+    length minus 20."""
+    combined = {}
+    for filename in sorted(f for f in os.listdir(path) if f[:7] == "antenna"):
+        with np.load(os.path.join(path, filename), allow_pickle=False) as f:
+            cur = {k: f[k] for k in f.files}
+        for tag in ("77", "24"):
+            tk, lk = f"times_{tag}", f"tx_{tag}_locs"
+            if tk in cur and lk in cur and cur[tk].shape[0] != cur[lk].shape[0]:
+                for _ in cur:
+                    m = min(cur[tk].shape[0], cur[lk].shape[0])
+                    cur[tk] = cur[tk][:m - 20]
+                    cur[lk] = cur[lk][:m - 20]
+        for k, v in cur.items():
+            combined[k] = np.concatenate((combined[k], v)) if k in combined else v
+    return combined
+
+
+def load_exp_json(path):
+    """experiment_info.json of an experiment folder (generic_loader.py:614-630)."""
+    import json
+    with open(os.path.join(path, "experiment_info.json")) as f:
+        return json.load(f)
+
+
+def write_adc_bin(path, samples):
+    """Write complex (n, 4) samples as a 77 GHz capture: int16, lanes' I then
+    Q per sample (the layout read_adc_bin reads); values are rounded to int16."""
+    s = np.asarray(samples)
+    raw = np.empty((s.shape[0], 2 * s.shape[1]), "<i2")
+    raw[:, :s.shape[1]] = np.clip(np.rint(s.real), -32768, 32767)
+    raw[:, s.shape[1]:] = np.clip(np.rint(s.imag), -32768, 32767)
+    raw.tofile(path)

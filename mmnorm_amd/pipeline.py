@@ -113,9 +113,14 @@ def tau_search_stage(sc, p0, rsdf, labeled, valid, coords, rays, fov, num_rx, se
         f"{rec['best_grid_surface_err_mm']:.2f} mm)")
 
 
-def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
-        ckpt_every=10, resume=False, seed=7, objective="energy", tau_search=False, rays=512,
-        fov=20.0, num_rx=50, variant="rec", repeat_prep=0):
+def prepare(cid=5, sigma=3e-4, K=64, seed=7, variant="rec", repeat_prep=0, rec=None):
+    """Steps 1-3 for configuration `cid`: the scene, the perturbed start
+    positions p0, the full-grid forward at p0 with the vote moment, the unit
+    normals on the 5 % filter without the voxels below the Li threshold of
+    |I|^2, the 6-connected labels and the size rule, the start voxels and
+    the RSDF BFS.  Returns a dict of the scene, the device fields and the
+    host seeds; `rec` (a dict) receives the timings."""
+    rec = {} if rec is None else rec
     dev = torch.device("cuda", torch.cuda.current_device())
     t0 = time.perf_counter()
     sc = synth.make_scene(cid, device=dev)
@@ -128,8 +133,6 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
     rng = np.random.default_rng(seed)
     p_true = sc.ant.astype(np.float32)
     p0 = (sc.ant + rng.normal(scale=sigma, size=sc.ant.shape)).astype(np.float32)
-    A = cfg.rows
-    rec = {"config": cid, "sigma_m": sigma, "steps": steps, "K": K, "variant": variant}
     bb = ops.grid_bbox(xl, yl, zl)
 
     # 2. full-grid forward with the fused vote moment
@@ -199,6 +202,26 @@ def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
             rec[f"{k}_warm_ms"] = float(np.median(v)) * 1e3
         log(f"again ({repeat_prep} times, median): CCL {rec['ccl_warm_ms']:.2f} ms, "
             f"seeds {rec['seeds_warm_ms']:.2f} ms, BFS {rec['bfs_warm_ms']:.2f} ms")
+    return {"sc": sc, "meas": meas, "axes": (xl, yl, zl), "bbox": bb, "p_true": p_true,
+            "p0": p0, "img": img, "mom": mom, "vidx": vidx, "normals": normals,
+            "labels": lab_t, "valid": valid, "seeds": seeds, "rsdf": rsdf, "hops": hops,
+            "dx": dx, "t0": t0}
+
+
+def run(cid=5, steps=50, sigma=3e-4, tau_vox=1.5, lr=2e-6, K=64, out=None,
+        ckpt_every=10, resume=False, seed=7, objective="energy", tau_search=False, rays=512,
+        fov=20.0, num_rx=50, variant="rec", repeat_prep=0):
+    rec = {"config": cid, "sigma_m": sigma, "steps": steps, "K": K, "variant": variant}
+    st = prepare(cid, sigma, K, seed, variant, repeat_prep, rec)
+    sc, meas, (xl, yl, zl), bb = st["sc"], st["meas"], st["axes"], st["bbox"]
+    p_true, p0, img, vidx, normals = st["p_true"], st["p0"], st["img"], st["vidx"], st["normals"]
+    lab_t, valid, seeds, rsdf, hops, dx = (st["labels"], st["valid"], st["seeds"], st["rsdf"],
+                                           st["hops"], st["dx"])
+    t0 = st["t0"]
+    cfg = sc.cfg
+    A = cfg.rows
+    dev = meas.device
+    T = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x, dtype=dt)).to(dev)
     reached = hops >= 0
     band = reached & (rsdf.abs() <= tau_vox * dx)
     bidx = torch.nonzero(band.reshape(-1)).reshape(-1).to(torch.int32)

@@ -218,3 +218,36 @@ def test_backward_ragged_bistatic(n_list):
     gm, ga = gm.cpu().numpy().astype(np.complex128), ga.cpu().numpy().astype(np.float64)
     assert np.linalg.norm(gm - gm64) <= 1e-4 * np.linalg.norm(gm64)
     assert np.linalg.norm(ga - ga64) <= 1e-3 * np.linalg.norm(ga64)
+
+
+@pytest.mark.parametrize("variant", ["rec", "profile"])
+def test_position_gradient_config4_geometry_vs_fd(oracle, variant):
+    """Config 4 itself (40x40 aperture at 2 mm, 128 samples 77-81 GHz, 96^3
+    voxels at 2 mm): dL/dp of L = sum_{v in S} w_v |I_v|^2 over a subset S of
+    the 128 strongest and 128 random voxels, against FP64 central differences
+    (h = 1e-6 m) on 64 random coordinates.  Tolerance from FP32: each
+    coordinate's error at most 1e-5 of the sum of the magnitudes of its terms
+    (tests/fd_subset.py), ten times the recurrence's measured relative error."""
+    import fd_subset
+    sc = synth.make_scene(4)
+    ant = sc.ant.astype(np.float32).astype(np.float64)
+    meas = T(sc.meas, np.complex64)
+    xl, yl, zl = (T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs))
+    a0 = T(ant, np.float32)
+    img = ops.backproject(meas, a0, xl, yl, zl, f0=sc.f0, df=sc.df).reshape(-1)
+    rng = np.random.default_rng(44)
+    top = torch.topk(img.abs(), 128).indices.cpu().numpy()
+    vidx = np.unique(np.concatenate([top, rng.choice(img.numel(), 128, replace=False)]))
+    w = rng.uniform(size=len(vidx))
+    scale = 1.0 / len(ant)
+    a = a0.clone().requires_grad_(True)
+    bb = ops.grid_bbox(xl, yl, zl)
+    I = ops.backproject_ad(meas, a, xl, yl, zl, f0=sc.f0, df=sc.df, scale=scale,
+                           vox_idx=T(vidx, np.int32), variant=variant, bbox=bb)
+    (T(w, np.float32) * I.abs() ** 2).sum().backward()
+    grad = a.grad.cpu().numpy().astype(np.float64)
+    coords = [(int(rng.integers(len(ant))), int(rng.integers(3))) for _ in range(64)]
+    fd, bound = fd_subset.fd_and_bound(oracle, sc, ant, vidx, w, scale, coords)
+    got = np.array([grad[i, k] for (i, k) in coords])
+    assert np.all(np.abs(got - fd) <= 1e-5 * bound), np.max(np.abs(got - fd) / bound)
+    assert np.linalg.norm(got - fd) <= 1e-3 * np.linalg.norm(fd)

@@ -128,3 +128,33 @@ def test_normal_estimates_full_frequency(oracle, variant):
     cos = np.sum(got * wn, 1) / np.linalg.norm(got, axis=1) / np.linalg.norm(wn, axis=1)
     assert np.degrees(np.arccos(np.clip(cos.min(), -1, 1))) < 0.05
     assert np.all(np.isnan(nrm[~filt]))
+
+
+@pytest.mark.parametrize("shape3", [False, True])
+@pytest.mark.parametrize("variant", ["fast", "ref"])
+def test_single_rx_layout_use_4_rx_false(oracle, variant, shape3):
+    """use_4_rx=False (py_sn_est_gpu.py:276-278, py_image_gpu.py:228-235): the
+    measurements are flattened and read as len(antenna_locs) rows of
+    radar_data.shape[1] samples, the offsets wrapped in a list; both a
+    (meas, samples) and a (meas, samples, 1) array, one RX offset, the
+    interpolated image and vote against the oracle."""
+    xl, yl, zl, locs, data, _, lam = real_like_inputs(seed=13, n_meas=24, spectra=True)
+    d1 = data[:, :, 1:2] if shape3 else data[:, :, 1]
+    rxo = [[-0.0051, 0.0, 0.0]]
+    fs = np.float32(3e8 / (2 * 3.0009e9) * 512 / 4096)
+    img = py_image_gpu.image_cuda(xl, yl, zl, locs, d1, rxo, 5.86116e6, lam, 3.0009e9, 512,
+                                  use_4_rx=False, is_ti_radar=True,
+                                  use_interpolated_processing=True, return_phase_centers=False)
+    meas = data[:, :, 1]
+    raw = oracle.bp_ref(xl, yl, zl, locs, meas, np.array(rxo), 5.86116e6, lam, fs, is_ti=True,
+                        interp=True).reshape(img.shape).astype(np.complex64)
+    raw /= len(locs)
+    assert np.max(np.abs(img - raw)) <= 1e-6 * np.max(np.abs(raw))
+    nrm = py_sn_est_gpu.normal_estimates_cuda(img, xl, yl, zl, locs, d1, rxo, 5.86116e6, lam,
+                                              3.0009e9, 512, use_4_rx=False, is_ti_radar=True,
+                                              initial_filter_percent=0.3, variant=variant)
+    filt = np.abs(img) > np.max(np.abs(img)) * 0.3
+    want = oracle.sn_ref(np.flatnonzero(filt), img.ravel(), xl, yl, zl, locs, meas,
+                         np.array(rxo), lam, fs, is_ti=True)
+    assert np.all(np.isnan(nrm[~filt]))
+    assert np.max(np.abs(nrm[filt] - want)) <= 1e-5 * np.max(np.abs(want))

@@ -100,6 +100,8 @@ def test_read_adc_bin():
 
 
 def test_correlate_locs_and_meas():
+    """ImageProcessor.correlate_locs_and_meas with the reference's argument
+    list and loader-shaped dicts (image_processor.py:74-154)."""
     from mmnorm_amd import image_processor as ip
     lam = 2.99792458e8 / 77e9
     # the robot moves along x at 1 mm per time unit from t = 100 to t = 200,
@@ -107,13 +109,18 @@ def test_correlate_locs_and_meas():
     t = np.array([100.0, 200.0, 300.0])
     locs = np.array([[0.0, 0.0, 0.0, 7.0], [0.1, 0.0, 0.0, 7.0], [0.1, 0.0, 0.0, 7.0]])
     spc, chirps, frames, sweep, per = 4, 2, 3, 1.0, 1.0
+    params = ip.params_with(num_frames=frames, sweep_time=sweep, periodicity=per,
+                            num_samples=spc, num_chirps=chirps, min_f=77e9)
     rd = {1: np.arange(frames * spc * chirps * 4).reshape(-1, 4) + 0j,
           2: np.zeros((frames * spc * chirps, 4)) + 0j,
           3: np.zeros((spc * chirps, 4)) + 0j,     # one frame's worth only
-          4: np.zeros((frames * spc * chirps, 4)) + 0j}
-    robot = {"all_radar_filenames": ["1", "2", "3", "4", "5"],
-             "all_radar_created_ts": [120.0, 250.0, 150.0, 199.0, 160.0]}
-    out = ip.correlate_locs_and_meas(robot, rd, t, locs, frames, sweep, per, spc, chirps, 77e9)
+          4: np.zeros((frames * spc * chirps, 4)) + 0j,
+          6: np.zeros((frames * spc * chirps, 4)) + 0j}
+    robot = {"all_radar_filenames": ["1", "2", "3", "4", "5", "6", "7"],
+             "all_radar_created_ts": [120.0, 250.0, 150.0, 199.0, 160.0, 99.5, -1]}
+    proc = ip.ImageProcessor(False, params=params)
+    out = proc.correlate_locs_and_meas("77_ghz", robot, rd, {"aperture size": "almost-square"},
+                                       {"times_77": t, "tx_77_locs": locs}, speed="speed_8")
     poses = out["poses"]
     # file 1: three frames 2 mm apart (> 3/16 lambda = 0.73 mm), poses interpolated
     assert np.allclose(poses[:3, 0], [0.020, 0.022, 0.024]) and np.all(poses[:3, 3] == 7.0)
@@ -122,11 +129,76 @@ def test_correlate_locs_and_meas():
     assert np.allclose(poses[3, 0], 0.1)
     # file 3: frame 0 kept, later frames lack samples; file 4: frames at
     # t = 199 and 201 are 1 mm apart (kept), the one at 203 repeats 201
-    # (dropped); file 5 has no radar file
+    # (dropped); file 5 has no radar file; file 6 starts before the
+    # trajectory (frame 0 dropped) so its later frames are dropped too, as in
+    # the reference; file 7 has no creation time
     assert len(poses) == 3 + 1 + 1 + 2
     assert np.allclose(poses[4, 0], 0.05)
     assert np.allclose(poses[5:, 0], [0.099, 0.1])
     assert 3 / 16 * lam > 2e-4
+    # the motion-capture frame swaps axes (image_processor.py:64-72)
+    opt = ip.ImageProcessor(True, params=params).correlate_locs_and_meas(
+        "77_ghz", robot, rd, {"aperture size": "almost-square"},
+        {"times_77": t, "tx_77_locs": locs[:, :3]})
+    assert np.allclose(opt["poses"][0], [-0.020, 0.0, 0.0])
+    with pytest.raises(AssertionError):
+        proc.correlate_locs_and_meas("10_ghz", robot, rd, {"aperture size": "almost-square"},
+                                     {"times_77": t, "tx_77_locs": locs})
+
+
+def test_get_radar_parameters():
+    from mmnorm_amd import image_processor as ip
+    p = ip.get_radar_parameters("77_ghz", False, "almost-square")
+    assert p["min_f"] == 77.5001e9 and p["num_samples"] == 512 and p["slope"] == 5861160
+    # slope * samples is the bandwidth the chirp sweeps (image_processor.py:359)
+    assert abs(ip.SLOPE_REAL * 512 - p["bandwidth"]) < 1e-5 * p["bandwidth"]
+    # 'large' looks up 'normal', which the reference's table lacks for 77 GHz
+    with pytest.raises(KeyError):
+        ip.get_radar_parameters("77_ghz", False, "large")
+    with pytest.raises(KeyError):
+        ip.get_radar_parameters("77_ghz", True)
+    assert ip.ImageProcessor().use_interpolated_processing is True
+
+
+def test_dataset_path_and_capture_names(tmp_path):
+    from mmnorm_amd import formats
+    got = formats.dataset_path("/r", "001", "cup", "77_ghz", is_los=False, exp_num="2",
+                               angles=(0, 90, 0))
+    assert got == "/r/sample_data/001_cup/robot_collected/0_90_0/exp2/nlos/unprocessed/radars/77_ghz"
+    assert formats.dataset_path("/r", "001", "cup", "77_ghz", exp_num=None, is_json=True) == \
+        "/r/sample_data/001_cup/robot_collected/0_0_0/exp1"
+    assert formats.radar_file_timestamp("adc_data1712345678.bin") == 1712345678
+    assert formats.radar_file_timestamp("adc_data_171234567890.bin") == 171234567890
+    # captures round-trip through the int16 I/Q layout, keyed by timestamp
+    rng = np.random.default_rng(1)
+    a = np.rint(rng.normal(size=(6, 4)) * 900) + 1j * np.rint(rng.normal(size=(6, 4)) * 900)
+    formats.write_adc_bin(str(tmp_path / "adc_data1700000001.bin"), a)
+    formats.write_adc_bin(str(tmp_path / "adc_data1700000002.bin"), 2 * a)
+    (tmp_path / "notes.txt").write_text("x")
+    rd = formats.load_radar_files(str(tmp_path))
+    assert sorted(rd) == [1700000001, 1700000002]
+    assert np.array_equal(rd[1700000001], a) and np.array_equal(rd[1700000002], 2 * a)
+    raw = np.fromfile(str(tmp_path / "adc_data1700000001.bin"), "<i2").reshape(6, 8)
+    assert np.array_equal(raw[:, :4], a.real) and np.array_equal(raw[:, 4:], a.imag)
+
+
+def test_robot_loc_files_trim_rule(tmp_path):
+    """A trajectory file whose times and positions differ in length is cut to
+    the shorter length minus 20, once per key of the file (generic_loader.py:
+    554-563): four keys, so 80 samples go."""
+    from mmnorm_amd import formats
+    np.savez(tmp_path / "antenna_1.npz", times_77=np.arange(200.0),
+             tx_77_locs=np.ones((205, 3)), times_24=np.arange(10.0), tx_24_locs=np.ones((10, 3)))
+    np.savez(tmp_path / "antenna_2.npz", times_77=np.arange(200.0, 210.0),
+             tx_77_locs=np.zeros((10, 3)), times_24=np.arange(3.0), tx_24_locs=np.ones((3, 3)))
+    np.savez(tmp_path / "other.npz", times_77=np.arange(3.0))
+    c = formats.load_robot_loc_files(str(tmp_path))
+    assert c["times_77"].shape == (120 + 10,) and c["tx_77_locs"].shape == (130, 3)
+    assert c["times_77"][119] == 119.0 and c["times_77"][120] == 200.0
+    assert c["times_24"].shape == (13,)
+    import json
+    (tmp_path / "experiment_info.json").write_text(json.dumps({"aperture size": "almost-square"}))
+    assert formats.load_exp_json(str(tmp_path))["aperture size"] == "almost-square"
 
 
 @pytest.mark.parametrize("shape,use_4_rx,rows", [((5, 7), False, 5), ((5, 7, 1), False, 5),
