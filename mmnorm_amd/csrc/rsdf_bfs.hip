@@ -29,13 +29,16 @@
 // k_bfs_small, a single 1024-thread workgroup that loops over levels with
 // the frontier, its rsdf and its masks in LDS: no launches, and the
 // frontier's data never goes through global memory, but one CU issues all
-// of a level's scattered requests.  Larger frontiers run as two kernels
-// per level over a grid of blocks, each block owning one contiguous chunk of
-// the frontier (block order is position order): k_count and k_emit_claim
-// (each block sums the counts before it for its offset; each child offers
-// the next level's keys at once).  The host enqueues those levels in
-// batches of 32, reads the frontier size back once per batch, and hands back
-// to k_bfs_small when the frontier shrinks.
+// of a level's scattered requests.  Larger frontiers run as one kernel per
+// level over a grid of blocks, each block owning one contiguous chunk of the
+// frontier in scan order: k_level counts each entry's won children, finds
+// its output offset by a single-pass scan across blocks (decoupled
+// look-back), emits, and has each child offer the next level's keys at
+// once.  (The earlier form, two kernels per level -- k_count, then
+// k_emit_claim summing the counts before it -- stays selectable for
+// measurement.)  The host enqueues those levels in batches of 32, reads the
+// frontier size back once per batch, and hands back to k_bfs_small when the
+// frontier shrinks.
 #include <stdlib.h>
 
 #include <algorithm>
@@ -57,10 +60,13 @@ struct Ws {
   uint8_t *pmask;   // [V]  won mask of frontier position p (bit d)
   uint8_t *emask;   // [V]  enterable neighbours of each voxel (bit d)
   int32_t *bcnt;    // [kGrid] per-block counts of a level
+  uint32_t *scan;   // [2][kGrid] k_level's look-back status words, by level parity
+  uint32_t *ctr;    // [2] k_level's block-id counters, by level parity
   int32_t *nf;      // [8]  [0,1] frontier sizes by level parity; [2] the level
                     //      k_bfs_small stopped at; [3] non-zero when a seed
                     //      lies outside the grid; [4,5] queue index of a
-                    //      level's first entry, by level parity
+                    //      level's first entry, by level parity; [6]
+                    //      non-zero when a look-back wait ran out
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -78,6 +84,8 @@ size_t ws_layout(int64_t V, Ws *w, char *base) {
   char *pm = take(V);
   char *em = take(V);
   char *bc = take(sizeof(int32_t) * kGrid);
+  char *sc = take(sizeof(uint32_t) * 2 * kGrid);
+  char *ct = take(sizeof(uint32_t) * 2);
   char *nf = take(sizeof(int32_t) * 8);
   if (w) {
     w->claim = (int32_t *)c;
@@ -86,15 +94,18 @@ size_t ws_layout(int64_t V, Ws *w, char *base) {
     w->pmask = (uint8_t *)pm;
     w->emask = (uint8_t *)em;
     w->bcnt = (int32_t *)bc;
+    w->scan = (uint32_t *)sc;
+    w->ctr = (uint32_t *)ct;
     w->nf = (int32_t *)nf;
   }
   return off;
 }
 
 // Size of the frontier of parity cur; 0 once k_seed has flagged a seed
-// outside the grid, so that no level kernel indexes with it.
+// outside the grid or a k_level wait has run out, so that no level kernel
+// indexes with it.
 __device__ __forceinline__ int front_size(const int32_t *__restrict__ nf, int cur) {
-  return nf[3] ? 0 : nf[cur];
+  return (nf[3] | nf[6]) ? 0 : nf[cur];
 }
 
 __device__ __forceinline__ void chunk_of(int n, int &start, int &end) {
@@ -148,7 +159,7 @@ __device__ __forceinline__ double step_dot(int d, double dx, double n0, double n
 __global__ void k_init(double *__restrict__ rsdf, int32_t *__restrict__ hops,
                        int32_t *__restrict__ claim, int32_t *__restrict__ nf, int64_t V) {
   const double qnan = __longlong_as_double(0x7ff8000000000000LL);
-  if (blockIdx.x == 0 && threadIdx.x == 0) nf[3] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) nf[3] = nf[6] = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V;
        i += (int64_t)gridDim.x * blockDim.x) {
     rsdf[i] = qnan;
@@ -203,7 +214,12 @@ __global__ void k_seed(double *__restrict__ rsdf, int32_t *__restrict__ hops,
 
 __global__ __launch_bounds__(kBlock) void k_claim(
     const int32_t *__restrict__ front, const int32_t *__restrict__ nf, int cur,
-    int32_t *__restrict__ claim, const uint8_t *__restrict__ emask, int ny, int nz) {
+    int32_t *__restrict__ claim, const uint8_t *__restrict__ emask, int ny, int nz,
+    uint32_t *__restrict__ scan, uint32_t *__restrict__ ctr) {
+  if (blockIdx.x == 0) {  // a clean scan state for the k_level launches that follow
+    for (int b = threadIdx.x; b < 2 * kGrid; b += kBlock) scan[b] = 0u;
+    if (threadIdx.x < 2) ctr[threadIdx.x] = 0u;
+  }
   int start, end;
   chunk_of(front_size(nf, cur), start, end);
   const int32_t q0 = nf[4 + cur];
@@ -320,6 +336,240 @@ __global__ __launch_bounds__(kBlock) void k_emit_claim(
         for (int e = 0; e < 6; ++e)
           if (em >> e & 1) atomicMin(&at32(claim, ch + dir_step(e, ny, nz)), key_of(q1, out, e));
         ++out;
+      }
+    }
+    base += round_tot;
+  }
+}
+
+// One level in one launch: k_count's won masks, the offsets of k_emit_claim
+// from a single-pass scan across blocks (decoupled look-back), and its
+// emission with the next level's claims.
+//
+// A block takes its position in the scan from a counter (atomicAdd at its
+// start), not from blockIdx, so every block it waits for has already
+// started and, waiting only for blocks before it, finishes: the waits end.
+// Block id takes frontier chunk id.  After counting its children it
+// publishes the count in its status word (state "aggregate"), then sums the
+// words of the blocks before it, nearest first, 64 at a time (one per lane
+// of wave 0), until it meets a block whose word holds an inclusive prefix;
+// then it publishes its own inclusive prefix.  A status word is one 32-bit
+// value, state in bits 30-31 (0 none, 1 aggregate, 2 inclusive prefix) and
+// the count in bits 0-29 (a level has fewer than 2^30 children), written
+// and read with device-scope atomic read-modify-writes, the operations the
+// claims already rely on across XCDs; no other data crosses blocks inside
+// the launch.  A wait that exceeds spin_limit reads sets nf[6] and the block
+// emits nothing; the host reports it.  The last block in scan order writes
+// the next level's size and queue index into the other parity's slots, and
+// block 0 clears the other parity's scan state for the next launch.  Won
+// masks of a block's first kLevelLds positions stay in LDS between the two
+// passes; later positions are tested again, which gives the same answer:
+// this level's keys are final, and the next level's offers made meanwhile
+// are larger and cannot replace them.
+constexpr int kLevelLds = 4096;
+constexpr uint32_t kScanAgg = 1u << 30, kScanPre = 2u << 30, kScanVal = (1u << 30) - 1u;
+
+// The directions of an entry that hold its keys.  All six claims are read
+// unconditionally (a direction that is not enterable reads u's own claim,
+// which is never this level's key) so that the six loads issue together
+// instead of one per branch.
+__device__ __forceinline__ unsigned won_mask(int32_t u, unsigned m, int32_t q0, int p,
+                                             const int32_t *__restrict__ claim, int ny,
+                                             int nz) {
+  int32_t k[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) k[d] = at32(claim, (m >> d & 1) ? u + dir_step(d, ny, nz) : u);
+  unsigned won = 0;
+#pragma unroll
+  for (int d = 0; d < 6; ++d)
+    won |= ((m >> d & 1) && k[d] == key_of(q0, p, d)) ? 1u << d : 0u;
+  return won;
+}
+
+__global__ __launch_bounds__(kBlock) void k_level(
+    const int32_t *__restrict__ front, int32_t *__restrict__ nf, int cur,
+    uint32_t *__restrict__ scan, uint32_t *__restrict__ ctr, int32_t *__restrict__ next,
+    int32_t *__restrict__ hops, double *__restrict__ rsdf, const double *__restrict__ nrm,
+    const uint8_t *__restrict__ emask, int32_t *__restrict__ claim, double dx, int level,
+    int ny, int nz, int spin_limit) {
+  __shared__ int sh[kBlock / kWave];
+  __shared__ int s_id, s_excl;
+  __shared__ uint8_t s_won[kLevelLds];
+  const int nxt = cur ^ 1;
+  const int G = (int)gridDim.x;
+  // the frontier size first: it does not depend on the block's scan position
+  const int n = front_size(nf, cur);
+  const int32_t q0 = nf[4 + cur];
+  if (n == 0) {  // past the end of the search (or a flagged error): nothing to scan
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      nf[nxt] = 0;
+      nf[4 + nxt] = q0;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) s_id = (int)atomicAdd(&ctr[cur], 1u);
+  __syncthreads();
+  const int id = s_id;
+  if (id == 0) {
+    for (int b = threadIdx.x; b < G; b += kBlock) scan[nxt * kGrid + b] = 0u;
+    if (threadIdx.x == 0) ctr[nxt] = 0u;
+  }
+  const int chunk = (n + G - 1) / G;
+  const int start = min(n, id * chunk), end = min(n, start + chunk);
+  // pass 1: won masks and the block's child count.  For the first round of
+  // positions the children's data (parent rsdf, child normal and mask) are
+  // loaded after the count's barrier (which waits for outstanding loads)
+  // and before the scan, so that they overlap it; only the stores and the
+  // next level's offers wait for the scan.
+  int cnt = 0;
+  int32_t u0 = 0;
+  unsigned won0 = 0u;
+  double prev0 = 0.0, c0[6], c1[6], c2[6];
+  unsigned em0[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    c0[d] = c1[d] = c2[d] = 0.0;
+    em0[d] = 0u;
+  }
+  for (int r0 = start; r0 < end; r0 += kBlock) {
+    const int p = r0 + (int)threadIdx.x;
+    if (p >= end) continue;
+    const int32_t u = front[p];
+    const unsigned won = won_mask(u, at32(emask, u), q0, p, claim, ny, nz);
+    if (p - start < kLevelLds) s_won[p - start] = (uint8_t)won;
+    cnt += __popc(won);
+    if (r0 == start) {
+      u0 = u;
+      won0 = won;
+    }
+  }
+  const int tot = block_sum(cnt, sh);
+  // unconditional loads (a direction not won reads u0's own entries): one
+  // burst of requests in flight during the scan
+  prev0 = at32(rsdf, u0);
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const int32_t ch = (won0 >> d & 1) ? u0 + dir_step(d, ny, nz) : u0;
+    const double *nc = nrm + 3 * (int64_t)ch;
+    c0[d] = nc[0];
+    c1[d] = nc[1];
+    c2[d] = nc[2];
+    em0[d] = at32(emask, ch);
+  }
+  // the scan across blocks (wave 0)
+  uint32_t *__restrict__ st = scan + cur * kGrid;
+  if (threadIdx.x < kWave) {
+    const int l = (int)threadIdx.x;
+    int excl = 0;
+    bool ok = true;
+    if (id == 0) {
+      if (l == 0) atomicExch(&st[0], kScanPre | (uint32_t)tot);
+    } else {
+      if (l == 0) atomicExch(&st[id], kScanAgg | (uint32_t)tot);
+      int base = id - 1, spins = 0;
+      for (;;) {
+        const int j = base - l;
+        const uint32_t wv = j >= 0 ? atomicOr(&st[j], 0u) : kScanPre;
+        const uint32_t state = wv & ~kScanVal;
+        const unsigned long long ready = __ballot(state != 0u);
+        const unsigned long long pre = __ballot(state == kScanPre);
+        // lanes up to the nearest inclusive prefix (or all 64) must be ready
+        const int upto = pre ? __ffsll((long long)pre) - 1 : kWave - 1;
+        const unsigned long long need =
+            upto == kWave - 1 ? ~0ull : ((1ull << (upto + 1)) - 1ull);
+        if ((ready & need) == need) {
+          int v = l <= upto ? (int)(wv & kScanVal) : 0;
+          for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+          excl += v;
+          if (pre) break;
+          base -= kWave;
+          continue;
+        }
+        if (++spins > spin_limit) {
+          ok = false;
+          break;
+        }
+      }
+      if (l == 0) {
+        if (ok)
+          atomicExch(&st[id], kScanPre | (uint32_t)(excl + tot));
+        else
+          atomicOr((uint32_t *)&nf[6], 1u);
+      }
+    }
+    if (l == 0) {
+      s_excl = ok ? excl : -1;
+      if (ok && id == G - 1) {  // the next level's size and queue index
+        nf[nxt] = excl + tot;
+        nf[4 + nxt] = q0 + n;
+      }
+    }
+  }
+  __syncthreads();
+  if (s_excl < 0) return;
+  // pass 2: emit in position order and offer the children's next-level keys
+  const int32_t q1 = q0 + n;
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  int base = s_excl;
+  for (int r0 = start; r0 < end; r0 += kBlock) {
+    const int p = r0 + (int)threadIdx.x;
+    const bool first = r0 == start;
+    unsigned m = 0u;
+    int32_t u = 0;
+    if (first) {
+      m = won0;
+      u = u0;
+    } else if (p < end) {
+      u = front[p];
+      m = p - start < kLevelLds ? (unsigned)s_won[p - start]
+                                : won_mask(u, at32(emask, u), q0, p, claim, ny, nz);
+    }
+    const int c = __popc(m);
+    int inc = c;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int t = __shfl_up(inc, o, kWave);
+      if (l >= o) inc += t;
+    }
+    if (l == kWave - 1) sh[w] = inc;
+    __syncthreads();
+    int before = 0, round_tot = 0;
+    for (int i = 0; i < kBlock / kWave; ++i) {
+      if (i < w) before += sh[i];
+      round_tot += sh[i];
+    }
+    __syncthreads();
+    if (m) {
+      int out = base + before + inc - c;
+      if (first) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+          if (!(m >> d & 1)) continue;
+          const int32_t ch = u + dir_step(d, ny, nz);
+          next[out] = ch;
+          at32(hops, ch) = level + 1;
+          at32(rsdf, ch) = __dadd_rn(prev0, step_dot(d, dx, c0[d], c1[d], c2[d]));
+#pragma unroll
+          for (int e = 0; e < 6; ++e)
+            if (em0[d] >> e & 1)
+              atomicMin(&at32(claim, ch + dir_step(e, ny, nz)), key_of(q1, out, e));
+          ++out;
+        }
+      } else {
+        const double prev = at32(rsdf, u);
+        for (int d = 0; d < 6; ++d) {
+          if (!(m & (1u << d))) continue;
+          const int32_t ch = u + dir_step(d, ny, nz);
+          const double *nc = nrm + 3 * (int64_t)ch;
+          const unsigned em = at32(emask, ch);
+          next[out] = ch;
+          at32(hops, ch) = level + 1;
+          at32(rsdf, ch) = __dadd_rn(prev, step_dot(d, dx, nc[0], nc[1], nc[2]));
+#pragma unroll
+          for (int e = 0; e < 6; ++e)
+            if (em >> e & 1)
+              atomicMin(&at32(claim, ch + dir_step(e, ny, nz)), key_of(q1, out, e));
+          ++out;
+        }
       }
     }
     base += round_tot;
@@ -556,13 +806,14 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   }();
   const int small_cap = std::max(1, std::min(cap_env, kSmallThreads * per));
   // Multi-block levels enqueued between two reads of the frontier size
-  // (MMN_BFS_BATCH, default 32).  A read costs more than the empty levels a
-  // long batch may run past the end: on the 1.5M-voxel field 4 / 16 / 32
-  // levels per batch take 1.78 / 1.39 / 1.30 ms.
+  // (MMN_BFS_BATCH, default 64, at most 256).  A read costs about as much as
+  // 6-10 empty levels (an empty k_level returns at once, ~3 us): with k_level
+  // on the 1.5M-voxel field 32 / 64 / 128 levels per batch take 1.02 / 1.00 /
+  // 1.20 ms, on the 8.4M-voxel one 2.89 / 2.92 / 2.85 ms (256: 2.67).
   static const int batch = [] {
     const char *e = getenv("MMN_BFS_BATCH");
-    const int v = e ? atoi(e) : 32;
-    return v < 1 ? 1 : v > 64 ? 64 : v;
+    const int v = e ? atoi(e) : 64;
+    return v < 1 ? 1 : v > 256 ? 256 : v;
   }();
   // Blocks of the multi-block level kernels (MMN_BFS_GRID: 256, 512 or
   // 1024; default 256).  A level's frontier has a few thousand entries; with
@@ -573,6 +824,23 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     const int v = e ? atoi(e) : 256;
     return v == 512 || v == kGrid ? v : 256;
   }();
+  // One launch per multi-block level (k_level, MMN_BFS_FUSED=1, the default)
+  // or two (k_count, k_emit_claim; MMN_BFS_FUSED=0, kept for measurement).
+  static const bool fused = [] {
+    const char *e = getenv("MMN_BFS_FUSED");
+    return !(e && atoi(e) == 0);
+  }();
+  // Blocks of k_level (MMN_BFS_FGRID, 16-1024, default 64): the look-back
+  // covers 64 blocks per round trip.
+  static const int fgrid = [] {
+    const char *e = getenv("MMN_BFS_FGRID");
+    const int v = e ? atoi(e) : 64;
+    return v < 16 ? 16 : v > kGrid ? kGrid : v;
+  }();
+  // Reads of a status word one k_level block makes before giving up (about
+  // a second); the blocks it waits for have all started, so a well-formed
+  // launch never comes near it.
+  constexpr int kSpinLimit = 1 << 20;
   int32_t *fr[2] = {w.front0, w.front1};
   int32_t n_cur = n_seeds;
   for (int level = 0; level <= V && n_cur > 0;) {
@@ -586,7 +854,7 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
                          w.claim, hops, rsdf, normals, w.emask, dx, level, small_cap, ny, nz);
       rc = mmn_check_launch("mmn_rsdf_bfs(small)");
       if (rc) return rc;
-      int32_t st4[4] = {0, 0, 0, 0};
+      int32_t st4[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       hipError_t e = hipMemcpyAsync(st4, w.nf, sizeof(st4), hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) {
@@ -595,6 +863,10 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
       }
       MMN_REQUIRE(st4[3] == 0, MMN_ERR_INPUT,
                   "mmn_rsdf_bfs: a seed lies outside the grid of %lld voxels", (long long)V);
+      if (st4[6]) {
+        mmn_set_error("mmn_rsdf_bfs: a level's scan wait ran out");
+        return (int)hipErrorLaunchTimeOut;
+      }
       level = st4[2];
       n_cur = st4[level & 1];
       continue;
@@ -602,9 +874,15 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     // the batch's first level is claimed here; every emit claims the next
     // (a level that was already claimed gets the same keys again: no change)
     hipLaunchKernelGGL(k_claim, dim3(lgrid), dim3(kBlock), 0, st, fr[level & 1], w.nf,
-                       level & 1, w.claim, w.emask, ny, nz);
+                       level & 1, w.claim, w.emask, ny, nz, w.scan, w.ctr);
     for (int b = 0; b < batch; ++b, ++level) {
       const int cur = level & 1, nxt = cur ^ 1;
+      if (fused) {
+        hipLaunchKernelGGL(k_level, dim3(fgrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
+                           w.scan, w.ctr, fr[nxt], hops, rsdf, normals, w.emask, w.claim, dx,
+                           level, ny, nz, kSpinLimit);
+        continue;
+      }
       hipLaunchKernelGGL(k_count, dim3(lgrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
                          w.claim, w.emask, w.pmask, w.bcnt, ny, nz);
       hipLaunchKernelGGL(k_emit_claim, dim3(lgrid), dim3(kBlock), 0, st, fr[cur], w.nf, cur,
@@ -613,7 +891,7 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     }
     rc = mmn_check_launch("mmn_rsdf_bfs(level)");
     if (rc) return rc;
-    int32_t st4[4] = {0, 0, 0, 0};
+    int32_t st4[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipError_t e = hipMemcpyAsync(st4, w.nf, sizeof(st4), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -622,6 +900,10 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
     }
     MMN_REQUIRE(st4[3] == 0, MMN_ERR_INPUT,
                 "mmn_rsdf_bfs: a seed lies outside the grid of %lld voxels", (long long)V);
+    if (st4[6]) {
+      mmn_set_error("mmn_rsdf_bfs: a level's scan wait ran out");
+      return (int)hipErrorLaunchTimeOut;
+    }
     n_cur = st4[level & 1];
   }
   return 0;

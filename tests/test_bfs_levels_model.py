@@ -13,6 +13,10 @@ own).  This model runs those steps with NumPy, in the same order and with the
 same hand-over rule, and must give the oracle's rsdf and hops bit for bit for
 every cap and batch size.  Each planted mistake (on this CPU model only) is a
 plausible slip in that design and must make some field differ.
+
+The multi-block levels compute their output offsets with a single-pass scan
+across blocks (k_level); its protocol is simulated at the end of this file
+under random schedules, dispatch orders and residency limits.
 """
 import numpy as np
 import pytest
@@ -142,4 +146,99 @@ def test_planted_slips_are_caught(cases, plant):
     # the early next-level offers exist only on the multi-block path (cap 0)
     caught = any(not same(level_search(n, lab, seeds, 1e-3, 0, 4, plant=plant), want)
                  for n, lab, seeds, want in cases)
+    assert caught, plant
+
+
+# ---------------------------------------------------------------------------
+# The single-pass scan of k_level (decoupled look-back), on the CPU.
+# ---------------------------------------------------------------------------
+AGG, PRE, VAL = 1 << 30, 2 << 30, (1 << 30) - 1
+
+
+def lookback_scan(counts, resident, window, rng, plant=None, max_steps=200000):
+    """Run k_level's scan for blocks whose child counts are `counts` (indexed
+    by blockIdx) under a random schedule: at most `resident` blocks run at a
+    time, a block takes its scan position from a counter when it starts
+    (plant "blockidx": its blockIdx instead), publishes its aggregate, then
+    reads the status words of the `window` positions before it, nearest
+    first, and either sums up to the nearest inclusive prefix (all nearer
+    words being ready), moves one window back when all are aggregates, or
+    waits (reads again later).  Returns each scan position's exclusive
+    prefix, or None if the schedule stops making progress."""
+    G = len(counts)
+    status = [0] * G
+    counter = 0
+    # the hardware need not start blocks in blockIdx order: a shuffled order
+    waiting = [int(b) for b in rng.permutation(G)]
+    running = {}                      # blockIdx -> state dict
+    done = {}
+    steps = 0
+    while len(done) < G:
+        steps += 1
+        if steps > max_steps:
+            return None
+        while waiting and len(running) < resident:
+            b = waiting.pop(0)
+            pos = b if plant == "blockidx" else counter
+            counter += 1
+            running[b] = {"pos": pos, "phase": 0, "base": pos - 1, "excl": 0}
+        b = rng.choice(sorted(running))
+        s = running[b]
+        pos, tot = s["pos"], counts[s["pos"]]
+        if s["phase"] == 0:                       # pass 1 done: publish
+            if pos == 0 or plant == "early_prefix":
+                status[pos] = PRE | tot
+                if pos == 0:
+                    done[pos] = 0
+                    del running[b]
+                    continue
+            else:
+                status[pos] = AGG | tot
+            s["phase"] = 1
+            continue
+        lanes = [status[j] if j >= 0 else PRE for j in range(s["base"], s["base"] - window, -1)]
+        states = [w & ~VAL for w in lanes]
+        pre = [i for i, st in enumerate(states) if st == PRE]
+        if plant == "stop_at_agg":
+            pre = [i for i, st in enumerate(states) if st != 0]
+        upto = pre[0] if pre else window - 1
+        if all(st != 0 for st in states[:upto + 1]):
+            s["excl"] += sum(w & VAL for w in lanes[:upto + 1])
+            if pre:
+                status[pos] = PRE | (s["excl"] + tot)
+                done[pos] = s["excl"]
+                del running[b]
+            else:
+                s["base"] -= window
+        # else: not ready, read again on a later turn
+    return [done[p] for p in range(G)]
+
+
+@pytest.mark.parametrize("G,resident,window", [(1, 1, 4), (13, 13, 4), (13, 3, 4), (64, 7, 64),
+                                               (70, 9, 8), (40, 1, 4)])
+def test_lookback_scan_every_schedule(G, resident, window):
+    rng = np.random.default_rng(G * 100 + resident)
+    for trial in range(30):
+        counts = rng.integers(0, 50, size=G)
+        counts[rng.uniform(size=G) < 0.3] = 0
+        got = lookback_scan(list(counts), resident, window, rng)
+        assert got is not None
+        assert got == list(np.concatenate([[0], np.cumsum(counts)[:-1]]))
+
+
+@pytest.mark.parametrize("plant", ["blockidx", "stop_at_agg", "early_prefix"])
+def test_lookback_planted_slips_are_caught(plant):
+    """Scan order by blockIdx can wait on a block that has not started
+    (dispatch order is not scan order when blocks are taken out of order, and
+    with few resident slots the wait never ends); stopping at an aggregate or
+    publishing an inclusive prefix before the look-back gives wrong offsets."""
+    rng = np.random.default_rng(5)
+    caught = False
+    for trial in range(40):
+        counts = list(rng.integers(1, 50, size=12))
+        got = lookback_scan(counts, 3, 4, rng, plant=plant, max_steps=20000)
+        want = list(np.concatenate([[0], np.cumsum(counts)[:-1]]))
+        if got != want:
+            caught = True
+            break
     assert caught, plant
