@@ -205,6 +205,10 @@ def main(argv=None):
                     help="timed launches of the range-profile forward (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="forward only")
+    ap.add_argument("--bfs-only", type=int, default=0, metavar="N",
+                    help="build the configuration's BFS field (range-profile forward with "
+                         "the moment, 5%% filter, vote, labelling, start voxels), time N "
+                         "calls of the RSDF BFS and print them as the JSON line")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="after the timed region, copy a BYTES-sized buffer once (a copy "
                          "kernel of known size for scaling rocprofv3 FETCH/WRITE_SIZE)")
@@ -230,6 +234,8 @@ def main(argv=None):
     xl, yl, zl = T(sc.x_locs, np.float32), T(sc.y_locs, np.float32), T(sc.z_locs, np.float32)
     V, A, F = cfg.nvox, cfg.rows, cfg.n_freq
     W = float(V) * A * F
+    if a.bfs_only > 0:
+        return bfs_only(sc, meas, ant, xl, yl, zl, dev, a.bfs_only)
 
     def fwd(variant):
         return ops.backproject(meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, variant=variant,
@@ -500,9 +506,34 @@ def interp_stage(sc, ant, xl, yl, zl, dev):
             "speedup": ref_ms / fast_ms, "rel_l2_fast_vs_ref": diff, "bins": S}
 
 
-def bfs_stage(img, mom, sc, dev):
-    """5% filter -> votes -> 6-connected components and the size rule (GPU) ->
-    GPU FIFO BFS, timed."""
+def bfs_only(sc, meas, ant, xl, yl, zl, dev, n):
+    """Time n calls of the RSDF BFS on the configuration's field."""
+    import torch
+    from mmnorm_amd import ops
+    img, mom = ops.backproject(meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, moment=True,
+                               variant="profile", bbox=ops.grid_bbox(xl, yl, zl))
+    normals, lab_t, valid, seeds, dx, _ = bfs_field(img, mom, sc, dev)
+    seeds_t = torch.from_numpy(np.asarray(seeds, np.int32)).to(dev)
+    ms = []
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rsdf, hops = ops.rsdf_bfs(normals, lab_t, seeds_t, dx)
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    rest = ms[1:] if n > 1 else ms
+    print(json.dumps({"metric": "RSDF BFS wall time per call", "value": float(np.median(rest)),
+                      "unit": "ms", "higher_is_better": False, "calls": n,
+                      "first_call_ms": ms[0], "min_ms": float(min(rest)),
+                      "voxels_reached": int((hops >= 0).sum().item()),
+                      "levels": int(hops.max().item()) + 1,
+                      "config": {"workload": f"config {sc.cfg.cid} field"}}), flush=True)
+    return 0
+
+
+def bfs_field(img, mom, sc, dev):
+    """The BFS's inputs from an image and its vote moment: unit normals on the
+    5 % filter, GPU 6-connected labels, the size rule and the start voxels."""
     import torch
     from mmnorm_amd import ops, sdf
     mag = img.abs().reshape(-1)
@@ -522,6 +553,16 @@ def bfs_stage(img, mom, sc, dev):
     torch.cuda.synchronize()
     ccl_ms = (time.perf_counter() - tc) * 1e3
     valid = sdf.size_rule(areas, dx)
+    seeds = sdf.component_seeds_gpu(normals, lab_t, valid, sc.x_locs, sc.y_locs, sc.z_locs)
+    return normals, lab_t, valid, seeds, dx, (vidx, ccl_ms)
+
+
+def bfs_stage(img, mom, sc, dev):
+    """5% filter -> votes -> 6-connected components and the size rule (GPU) ->
+    GPU FIFO BFS, timed."""
+    import torch
+    from mmnorm_amd import ops, sdf
+    normals, lab_t, valid, _, dx, (vidx, ccl_ms) = bfs_field(img, mom, sc, dev)
     labeled = lab_t.cpu().numpy()
     nrm_np = normals.cpu().numpy()
     m1, m2, m3 = np.meshgrid(sc.y_locs, sc.x_locs, sc.z_locs)
