@@ -36,7 +36,7 @@
 // look-back), emits, and has each child offer the next level's keys at
 // once.  (The earlier form, two kernels per level -- k_count, then
 // k_emit_claim summing the counts before it -- stays selectable for
-// measurement.)  The host enqueues those levels in batches of 32, reads the
+// measurement.)  The host enqueues those levels in batches of 64, reads the
 // frontier size back once per batch, and hands back to k_bfs_small when the
 // frontier shrinks.
 #include <stdlib.h>
