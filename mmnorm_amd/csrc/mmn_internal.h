@@ -96,10 +96,13 @@ __device__ __forceinline__ void phasor_rev64(double t, float &re, float &im) {
 // swap and two sign flips:
 //   2 pi t = q pi/2 + theta,  q mod 4 = 0: ( c,  s)  1: (-s,  c)
 //                                       2: (-c, -s)  3: ( s, -c)  as (cos, sin).
+// q is converted to int32 directly (one instruction): |t4| < 2^31, which
+// holds for every step phase here (t4 = 4 d df/c is below 10^3 for d < 1 km
+// at df = 75 MHz).
 __device__ __forceinline__ void phasor_quarter(double t4, float &re, float &im) {
   const double qd = rint(t4);
   const float y = (float)(t4 - qd);
-  const int q = (int)(long long)qd;
+  const int q = (int)qd;
   const float th = y * 1.57079632679489662f;
   const float t2 = th * th;
   float ps = fmaf(t2, 2.75573192e-6f, -1.98412698e-4f);
