@@ -360,7 +360,14 @@ def main(argv=None):
         b_s = float(np.mean(bk)) * 1e-3
         rec["backward"] = {"ms": b_s * 1e3, "over_forward": b_s / k_avg,
                            "tflops": V * A * (16.0 * F + 30.0) / b_s / 1e12}
-        log(f"backward: {b_s * 1e3:.2f} ms")
+        # the position gradient alone (what autograd asks for when the
+        # measurements take no gradient, as in the pipeline's steps)
+        _, pk2, _ = timed(lambda: ops.backproject_backward(
+            g, meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, scale=1.0 / A, K=32,
+            need_meas=False), 2, 1)
+        rec["backward"]["ant_only_ms"] = float(np.mean(pk2))
+        log(f"backward: {b_s * 1e3:.2f} ms (position gradient alone "
+            f"{rec['backward']['ant_only_ms']:.2f} ms)")
         rec["rsdf_bfs"], bfs_state = bfs_stage(img, mom, sc, dev)
         log(f"rsdf bfs: {rec['rsdf_bfs']}")
         # the drop-in end to end: NumPy in, NumPy out (host-device copies included)

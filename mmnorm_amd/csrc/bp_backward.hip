@@ -187,12 +187,17 @@ __device__ __forceinline__ VoxStart vox_start(const VoxRec &r, float px, float p
   return v;
 }
 
-// One wave per (row a, sample block fb, voxel chunk c).  At 4 waves per SIMD
-// the K = 32 kernel fits in 128 VGPRs (64 of them the adjoint accumulators);
-// the register cap costs a few spills in the epilogue only, and the extra
-// wave per SIMD is worth ~10 % (measured against 3 waves at 148 VGPRs).
-template <int K>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bwd(
+// One wave per (row a, sample block fb, voxel chunk c).  MEAS: the adjoint
+// (dL/dmeas) is accumulated; ANT: the position term.  At 4 waves per SIMD
+// the K = 32 kernel with the adjoint fits in 128 VGPRs (64 of them the
+// adjoint accumulators); the register cap costs a few spills in the epilogue
+// only, and the extra wave per SIMD is worth ~10 % (measured against 3
+// waves at 148 VGPRs).  Without the adjoint (position gradient only, as
+// when the measurements take no gradient) there are no accumulators and
+// the kernel runs at 8 waves per SIMD.
+template <int K, bool MEAS, bool ANT>
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(MEAS ? 4 : 7))) void k_bwd(
     float2 *__restrict__ gs, float *__restrict__ gp,
     const float4 *__restrict__ rec4, const float *__restrict__ recg, int nvp,
     int chunk, const float *__restrict__ ant, const float2 *__restrict__ coef,
@@ -218,9 +223,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
   const double fc = fma((double)f_lo, dfc, f0c);
 
-  float accr[K], acci[K];
+  static_assert(MEAS || ANT, "nothing to compute");
+  float accr[MEAS ? K : 1], acci[MEAS ? K : 1];
 #pragma unroll
-  for (int t = 0; t < K; ++t) accr[t] = acci[t] = 0.0f;
+  for (int t = 0; t < (MEAS ? K : 1); ++t) accr[t] = acci[t] = 0.0f;
   float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
 
   // chunk and the padded voxel count are multiples of kPad, so every voxel
@@ -245,55 +251,67 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     float2 sp[8];
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-      if (t % 8 == 0) {
+      if (ANT && t % 8 == 0) {
         // 8 coefficients at a time through scalar loads (cache hits), so
         // that K complex values never sit in scalar registers at once
         asm volatile("" ::: "memory");
 #pragma unroll
         for (int q = 0; q < 8; ++q) sp[q] = cf[t + q];
       }
-      float sr = r0[0], si = i0[0];
+      if constexpr (MEAS) {
+        float sr = r0[0], si = i0[0];
 #pragma unroll
-      for (int u = 1; u < kVV; ++u) {
-        sr += r0[u];
-        si += i0[u];
+        for (int u = 1; u < kVV; ++u) {
+          sr += r0[u];
+          si += i0[u];
+        }
+        accr[t] += sr;
+        acci[t] += si;
       }
-      accr[t] += sr;
-      acci[t] += si;
 #pragma unroll
       for (int u = 0; u < kVV; ++u) {
-        tt[u] = fmaf(sp[t % 8].x, r0[u], tt[u]);
-        tt[u] = fmaf(sp[t % 8].y, i0[u], tt[u]);
+        if constexpr (ANT) {
+          tt[u] = fmaf(sp[t % 8].x, r0[u], tt[u]);
+          tt[u] = fmaf(sp[t % 8].y, i0[u], tt[u]);
+        }
         const float r2 = fmaf(p[u].c2, r1[u], -r0[u]);
         const float i2 = fmaf(p[u].c2, i1[u], -i0[u]);
         r0[u] = r1[u]; i0[u] = i1[u]; r1[u] = r2; i1[u] = i2;
       }
     }
+    if constexpr (ANT) {
 #pragma unroll
-    for (int u = 0; u < kVV; ++u) {
-      gpx = fmaf(-tt[u], p[u].gx, gpx);
-      gpy = fmaf(-tt[u], p[u].gy, gpy);
-      gpz = fmaf(-tt[u], p[u].gz, gpz);
+      for (int u = 0; u < kVV; ++u) {
+        gpx = fmaf(-tt[u], p[u].gx, gpx);
+        gpy = fmaf(-tt[u], p[u].gy, gpy);
+        gpz = fmaf(-tt[u], p[u].gz, gpz);
+      }
     }
   }
   // butterfly over the wave: every lane ends with the wave's totals
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) {
+    if constexpr (MEAS) {
 #pragma unroll
-    for (int t = 0; t < K; ++t) {
-      accr[t] += __shfl_xor(accr[t], o, kWave);
-      acci[t] += __shfl_xor(acci[t], o, kWave);
+      for (int t = 0; t < K; ++t) {
+        accr[t] += __shfl_xor(accr[t], o, kWave);
+        acci[t] += __shfl_xor(acci[t], o, kWave);
+      }
     }
-    gpx += __shfl_xor(gpx, o, kWave);
-    gpy += __shfl_xor(gpy, o, kWave);
-    gpz += __shfl_xor(gpz, o, kWave);
+    if constexpr (ANT) {
+      gpx += __shfl_xor(gpx, o, kWave);
+      gpy += __shfl_xor(gpy, o, kWave);
+      gpz += __shfl_xor(gpz, o, kWave);
+    }
   }
   if (live) {
-    float2 *__restrict__ out = gs + ((size_t)c * A + a) * F + f_lo;
+    if constexpr (MEAS) {
+      float2 *__restrict__ out = gs + ((size_t)c * A + a) * F + f_lo;
 #pragma unroll
-    for (int t = 0; t < K; ++t)
-      if (lane == t && t < nf) out[t] = make_float2(accr[t], acci[t]);
-    if (lane == 0) {
+      for (int t = 0; t < K; ++t)
+        if (lane == t && t < nf) out[t] = make_float2(accr[t], acci[t]);
+    }
+    if (ANT && lane == 0) {
       float *__restrict__ o = gp + (((size_t)c * nfbmax + fb) * A + a) * 3;
       o[0] = gpx;
       o[1] = gpy;
@@ -377,9 +395,19 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   const int64_t npad = pad_vox(nvox);
   const int nfb = (F + K - 1) / K;
   const int nfbmax = (F + kMinK - 1) / kMinK;
-  void (*kern)(float2 *, float *, const float4 *, const float *, int, int, const float *,
-               const float2 *, const float *, double, float, double, int, int, int, int,
-               int, int, int64_t) = K == 16 ? k_bwd<16> : k_bwd<32>;
+  // the gradients asked for select the kernel: both, the adjoint alone, or
+  // the position gradient alone (no adjoint accumulators, twice the waves)
+  using KernT = void (*)(float2 *, float *, const float4 *, const float *, int, int,
+                         const float *, const float2 *, const float *, double, float, double,
+                         int, int, int, int, int, int, int64_t);
+  const bool gm_on = grad_meas != nullptr, ga_on = grad_ant != nullptr;
+  KernT kern;
+  if (K == 16)
+    kern = gm_on && ga_on ? k_bwd<16, true, true> : gm_on ? k_bwd<16, true, false>
+                                                         : k_bwd<16, false, true>;
+  else
+    kern = gm_on && ga_on ? k_bwd<32, true, true> : gm_on ? k_bwd<32, true, false>
+                                                         : k_bwd<32, false, true>;
   int per_cu = 0, dev = 0, cus = 256;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
       per_cu < 1)

@@ -220,6 +220,50 @@ def test_backward_ragged_bistatic(n_list):
     assert np.linalg.norm(ga - ga64) <= 1e-3 * np.linalg.norm(ga64)
 
 
+@pytest.mark.parametrize("K", [16, 32])
+@pytest.mark.parametrize("n_list", [77, None])
+def test_backward_single_outputs(K, n_list):
+    """The kernels for one gradient alone (the adjoint without the position
+    term; the position gradient without the adjoint accumulators, at twice
+    the waves) against the kernel for both and against the FP64 NumPy
+    backward, on the ragged bistatic TI case."""
+    sc = small_case(seed=6, n_ap=6, F=40, grid=(9, 7, 5))
+    rng = np.random.default_rng(12)
+    rxo = np.array([[0.0, 0.0, 0.0], [3e-3, 2e-3, 0.0]])
+    meas = np.repeat(sc.meas, 2, axis=0)[: 2 * len(sc.ant)]
+    meas = meas * np.exp(1j * rng.uniform(0, 2 * np.pi, size=meas.shape))
+    ant = np.repeat(sc.ant, 2, axis=0).astype(np.float32).astype(np.float64)
+    X, Y, Z = np.meshgrid(sc.x_locs.astype(np.float32).astype(np.float64),
+                          sc.y_locs.astype(np.float32).astype(np.float64),
+                          sc.z_locs.astype(np.float32).astype(np.float64), indexing="ij")
+    vall = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    if n_list is None:
+        idx, v = None, vall
+    else:
+        sel = np.sort(rng.choice(len(vall), size=n_list, replace=False)).astype(np.int32)
+        idx, v = T(sel, np.int32), vall[sel]
+    g = rng.normal(size=len(v)) + 1j * rng.normal(size=len(v))
+    scale = 1.0 / len(ant)
+    xl, yl, zl = (T(q, np.float32) for q in (sc.x_locs, sc.y_locs, sc.z_locs))
+    args = (T(g, np.complex64), T(meas, np.complex64), T(ant, np.float32), xl, yl, zl,
+            T(rxo, np.float32))
+    kw = dict(f0=sc.f0, df=sc.df, scale=scale, vox_idx=idx, is_ti=True, K=K)
+    gm, ga = ops.backproject_backward(*args, **kw)
+    gm1, none_a = ops.backproject_backward(*args, need_ant=False, **kw)
+    none_m, ga1 = ops.backproject_backward(*args, need_meas=False, **kw)
+    assert none_a is None and none_m is None
+    # the same sums per wave; only the chunking (occupancy) can differ
+    assert (torch.linalg.norm(gm1 - gm) / torch.linalg.norm(gm)).item() <= 1e-6
+    assert (torch.linalg.norm(ga1 - ga) / torch.linalg.norm(ga)).item() <= 1e-6
+    gm64, ga64 = _backward64(g, meas.astype(np.complex64).astype(np.complex128), ant,
+                             rxo.astype(np.float32).astype(np.float64), v, sc.f0, sc.df,
+                             scale, True)
+    gm1 = gm1.cpu().numpy().astype(np.complex128)
+    ga1 = ga1.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(gm1 - gm64) <= 1e-4 * np.linalg.norm(gm64)
+    assert np.linalg.norm(ga1 - ga64) <= 1e-3 * np.linalg.norm(ga64)
+
+
 @pytest.mark.parametrize("variant", ["rec", "profile"])
 def test_position_gradient_config4_geometry_vs_fd(oracle, variant):
     """Config 4 itself (40x40 aperture at 2 mm, 128 samples 77-81 GHz, 96^3
