@@ -378,14 +378,12 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   hipStream_t st = (hipStream_t)stream;
   const int A = num_rows, F = samples;
   if (nvox == 0 || (!grad_meas && !grad_ant)) {
-    hipError_t e = hipSuccess;
-    if (grad_meas) e = hipMemsetAsync(grad_meas, 0, sizeof(float2) * (size_t)A * F, st);
-    if (e == hipSuccess && grad_ant) e = hipMemsetAsync(grad_ant, 0, sizeof(float) * 3 * (size_t)A, st);
-    if (e != hipSuccess) {
-      mmn_set_error("mmn_bp_backward: %s", hipGetErrorString(e));
-      return (int)e;
-    }
-    return 0;
+    int rc = 0;
+    if (grad_meas)
+      rc = mmn_zero_async(grad_meas, sizeof(float2) * (size_t)A * F, st, "mmn_bp_backward");
+    if (rc == 0 && grad_ant)
+      rc = mmn_zero_async(grad_ant, sizeof(float) * 3 * (size_t)A, st, "mmn_bp_backward");
+    return rc;
   }
   const size_t need = bwd_layout(A, F, nvox, nullptr, nullptr);
   MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,

@@ -1177,15 +1177,14 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
   const int A = num_rows, F = samples;
   hipStream_t st = (hipStream_t)stream;
   if (n == 0 || (!grad_meas && !grad_ant)) {
-    hipError_t e = hipSuccess;
-    if (grad_meas) e = hipMemsetAsync(grad_meas, 0, sizeof(float2) * (size_t)A * F, st);
-    if (e == hipSuccess && grad_ant)
-      e = hipMemsetAsync(grad_ant, 0, sizeof(float) * 3 * (size_t)A, st);
-    if (e != hipSuccess) {
-      mmn_set_error("mmn_bp_profile_backward: %s", hipGetErrorString(e));
-      return (int)e;
-    }
-    return 0;
+    int rc = 0;
+    if (grad_meas)
+      rc = mmn_zero_async(grad_meas, sizeof(float2) * (size_t)A * F, st,
+                          "mmn_bp_profile_backward");
+    if (rc == 0 && grad_ant)
+      rc = mmn_zero_async(grad_ant, sizeof(float) * 3 * (size_t)A, st,
+                          "mmn_bp_profile_backward");
+    return rc;
   }
   const size_t need = mmn_bp_profile_backward_workspace_size(n, A, F, f0, df, bbox);
   MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
@@ -1237,7 +1236,9 @@ extern "C" int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const 
       hipLaunchKernelGGL(k_bt_pack, dim3(mmn_blocks(sh.total, kBlock)), dim3(kBlock), 0, st,
                          w.recd, w.recf, w.recg, (const float2 *)g, x_locs, y_locs, z_locs,
                          vox_idx, w.v_out, n, nx, ny, nz, sh.ex, sh.ey, sh.ez, sh.G, sh.total);
-      e = hipMemsetAsync(w.rmax, 0, sizeof(unsigned long long), st);
+      const int zr =
+          mmn_zero_async(w.rmax, sizeof(unsigned long long), st, "mmn_bp_profile_backward");
+      if (zr != 0) return zr;
     }
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_bt_groups, dim3((unsigned)sh.ngroups), dim3(kBlock), 0, st,

@@ -25,6 +25,11 @@ void mmn_set_error(const char *fmt, ...);
 // Checks the launch that was just enqueued; returns the hipError_t (> 0).
 int mmn_check_launch(const char *what);
 
+// Zeroes bytes (whole 32-bit words, 4-byte aligned) at p on stream st with a
+// fill kernel, not hipMemsetAsync, so that callers may capture the entry
+// points in a HIP graph; returns 0 or an error code (message under what).
+int mmn_zero_async(void *p, size_t bytes, hipStream_t st, const char *what);
+
 // ------------------------------------------------------------- geometry --
 // Linear voxel index -> (x, y, z) with z fastest (imaging_gpu.cu:48-54).
 struct VoxelGrid {

@@ -678,11 +678,9 @@ extern "C" int mmn_label_sizes(int64_t *sizes, const int32_t *labels, int64_t n_
   MMN_REQUIRE(n_vox >= 0 && n_labels >= 0, MMN_ERR_ARG,
               "mmn_label_sizes: negative size");
   hipStream_t st = (hipStream_t)stream;
-  hipError_t e = hipMemsetAsync(sizes, 0, sizeof(int64_t) * ((size_t)n_labels + 1), st);
-  if (e != hipSuccess) {
-    mmn_set_error("mmn_label_sizes: %s", hipGetErrorString(e));
-    return (int)e;
-  }
+  const int rc = mmn_zero_async(sizes, sizeof(int64_t) * ((size_t)n_labels + 1), st,
+                                "mmn_label_sizes");
+  if (rc != 0) return rc;
   if (n_vox > 0)
     hipLaunchKernelGGL(k_label_sizes, dim3(mmn_blocks(n_vox, kSizeThreads * kSizePer)),
                        dim3(kSizeThreads), 0, st, sizes, labels, n_vox, n_labels);

@@ -264,6 +264,33 @@ def test_backward_single_outputs(K, n_list):
     assert np.linalg.norm(ga1 - ga64) <= 1e-3 * np.linalg.norm(ga64)
 
 
+@pytest.mark.parametrize("entry", ["mmn_bp_backward", "mmn_bp_profile_backward"])
+def test_backward_of_no_voxels_is_zero(entry):
+    """An empty voxel list through the C ABI: both gradients are filled with
+    zeros (by a fill kernel on the stream; the outputs start as ones)."""
+    from mmnorm_amd import _lib
+    sc = small_case(F=32, grid=(4, 4, 4))
+    meas, a = T(sc.meas, np.complex64), T(sc.ant, np.float32)
+    xl, yl, zl = (T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs))
+    A, F = meas.shape
+    gm = torch.ones((A, F), dtype=torch.complex64, device=dev())
+    ga = torch.ones((A, 3), dtype=torch.float32, device=dev())
+    g1 = torch.zeros(1, dtype=torch.complex64, device=dev())      # not read
+    idx1 = torch.zeros(1, dtype=torch.int32, device=dev())        # not read
+    rxo = torch.zeros((1, 3), dtype=torch.float32, device=dev())
+    common = (_lib.ptr(gm), _lib.ptr(ga), g1.data_ptr(), 1.0, xl.data_ptr(), yl.data_ptr(),
+              zl.data_ptr(), idx1.data_ptr(), 0, a.data_ptr(), meas.data_ptr(),
+              rxo.data_ptr(), float(sc.f0), float(sc.df), 4, 4, 4, A, 1, F, 0)
+    if entry == "mmn_bp_backward":
+        _lib.call(entry, *common, 32, None, 0, _lib.stream_handle(dev()))
+    else:
+        import ctypes
+        bb = (ctypes.c_double * 6)(*ops.grid_bbox(xl, yl, zl))
+        _lib.call(entry, *common, ctypes.addressof(bb), None, 0, _lib.stream_handle(dev()))
+    torch.cuda.synchronize()
+    assert int(torch.count_nonzero(gm).item()) == 0 and int(torch.count_nonzero(ga).item()) == 0
+
+
 @pytest.mark.parametrize("variant", ["rec", "profile"])
 def test_position_gradient_config4_geometry_vs_fd(oracle, variant):
     """Config 4 itself (40x40 aperture at 2 mm, 128 samples 77-81 GHz, 96^3
