@@ -1,10 +1,12 @@
 """Benchmark of the image-formation hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--K 32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--K 256]
 
 A step is one full-frequency backprojection of the configuration's synthetic
-scene with the fused normal-vote moment (recurrence kernel), inputs resident
-in HBM.  Prints ONE JSON line:
+scene with the fused normal-vote moment (recurrence kernel, FP64 re-anchor
+every --K samples: 256 by default, as 64-sample FP32 Horner chains combined
+in r^64; the library's own default is 64), inputs resident in HBM.  Prints
+ONE JSON line:
 
 * value      W / t  in G voxel*row*sample/s (W = V*A*F, t = mean step time)
 * roofline   FP32 VALU: algorithmic FLOPs V*A*(8F+14) per launch over the
@@ -199,7 +201,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--K", type=int, default=64)
+    ap.add_argument("--K", type=int, default=256,
+                    help="FP64 re-anchor interval of the recurrence (8-64: one FP32 Horner "
+                         "chain; 128, 256: chains of 64 combined in r^64)")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--profile-steps", type=int, default=5,
                     help="timed launches of the range-profile forward (0 = skip)")

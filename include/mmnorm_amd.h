@@ -102,7 +102,10 @@ int mmn_cuda_sn_est(float *normals, const int32_t *vox_idx, int64_t n_valid,
  * surface_normal_est.cu:128-144 is then Re(conj(I_v) moment[v]) / |I_v|
  * (mmn_sn_from_moment).  variant: MMN_BP_PLAIN, MMN_BP_PLAIN_FAST or
  * MMN_BP_REC; K (REC only)
- * is the re-anchor interval in samples, one of 8, 16, 32, 64 (or 0 = 64).
+ * is the FP64 re-anchor interval in samples, one of 8, 16, 32, 64, 128, 256
+ * (or 0 = 64).  Up to 64 samples an interval is one FP32 Horner chain; 128 and
+ * 256 are chains of 64 combined by a second Horner level in r^64, whose step
+ * also comes from the FP64 range reduction (default lane shape only).
  * No reference counterpart for the fused moment or the variants; the
  * reference computes the vote only for the interpolated branch
  * (py_sn_est_gpu.py:404-405).

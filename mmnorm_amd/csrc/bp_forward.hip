@@ -343,6 +343,44 @@ __device__ __forceinline__ void anchor_add(const double (&d)[VPT], double fb,
 // r.  S = 2: the even and odd coefficients as two Horner chains in r^2,
 // h = E(r^2) + r O(r^2) -- the same 4 FMAs per sample with twice the
 // independent chains per lane.
+// h = sum_t blk[t] r^t over KB samples, by Horner from the top coefficient.
+template <int KB, int VPT>
+__device__ __forceinline__ void block_h(const float2 *__restrict__ blk,
+                                        const float (&rr)[VPT], const float (&ri)[VPT],
+                                        float (&hr)[VPT], float (&hi)[VPT]) {
+  const float2 top = blk[KB - 1];
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    hr[k] = top.x;
+    hi[k] = top.y;
+  }
+#pragma unroll
+  for (int t = KB - 2; t >= 0; --t) {
+    const float2 s = blk[t];
+    horner_step<VPT>(s.x, s.y, rr, ri, hr, hi);
+  }
+}
+
+// acc = acc * R + h (complex, per voxel): the second Horner level of a
+// re-anchor interval longer than 64 samples, R = r^64.
+template <int VPT>
+__device__ __forceinline__ void outer_step(const float (&Rr)[VPT], const float (&Ri)[VPT],
+                                           const float (&hr)[VPT], const float (&hi)[VPT],
+                                           float (&ar)[VPT], float (&ai)[VPT]) {
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    const float tr = fmaf(ar[k], Rr[k], fmaf(-ai[k], Ri[k], hr[k]));
+    const float ti = fmaf(ar[k], Ri[k], fmaf(ai[k], Rr[k], hi[k]));
+    ar[k] = tr;
+    ai[k] = ti;
+  }
+}
+
+// One re-anchor interval of K samples starting at sample b0.  K <= 64: one
+// Horner block.  K > 64 (S = 1 only): K / 64 Horner blocks of 64 samples,
+// combined from the top one down by a Horner in R = r^64 (R from the FP64
+// range reduction of d 64 df / c), so every chain stays 64 steps long while
+// the FP64 anchor comes once per K samples.
 template <int K, int VPT, int S>
 __device__ __forceinline__ void rec_block(const float2 *__restrict__ row, int b0,
                                           double f0c, double dfc,
@@ -351,21 +389,24 @@ __device__ __forceinline__ void rec_block(const float2 *__restrict__ row, int b0
                                           const float (&ri)[VPT],
                                           const float (&r2r)[VPT],
                                           const float (&r2i)[VPT],
+                                          const float (&Rr)[VPT],
+                                          const float (&Ri)[VPT],
                                           float (&cr)[VPT], float (&ci)[VPT]) {
   const float2 *__restrict__ blk = row + b0;
-  if constexpr (S == 1) {
+  if constexpr (K > 64) {
+    static_assert(S == 1 && K % 64 == 0, "long re-anchor intervals: plain Horner, 64-blocks");
+    float ar[VPT], ai[VPT];
+    block_h<64, VPT>(blk + (K - 64), rr, ri, ar, ai);
+#pragma unroll
+    for (int q = K / 64 - 2; q >= 0; --q) {
+      float hr[VPT], hi[VPT];
+      block_h<64, VPT>(blk + 64 * q, rr, ri, hr, hi);
+      outer_step<VPT>(Rr, Ri, hr, hi, ar, ai);
+    }
+    anchor_add<VPT>(d, fma((double)b0, dfc, f0c), ar, ai, cr, ci);
+  } else if constexpr (S == 1) {
     float hr[VPT], hi[VPT];
-    const float2 top = blk[K - 1];
-#pragma unroll
-    for (int k = 0; k < VPT; ++k) {
-      hr[k] = top.x;
-      hi[k] = top.y;
-    }
-#pragma unroll
-    for (int t = K - 2; t >= 0; --t) {
-      const float2 s = blk[t];
-      horner_step<VPT>(s.x, s.y, rr, ri, hr, hi);
-    }
+    block_h<K, VPT>(blk, rr, ri, hr, hi);
     anchor_add<VPT>(d, fma((double)b0, dfc, f0c), hr, hi, cr, ci);
   } else {
     static_assert(K % 2 == 0, "split Horner needs an even block");
@@ -452,7 +493,7 @@ __attribute__((amdgpu_waves_per_eu(VPT >= 4 ? 3 : VPT * S <= 2 ? 7 : 4, 8))) voi
   const int row_hi = seg < 0 ? rows : min(rows, row_lo + rps);
   // d * 4 df/c equals 4 d * df/c exactly (a power-of-two scale), and adding
   // 0.0 leaves d unchanged: uniform constants instead of per-voxel selects.
-  const double dfc4 = 4.0 * dfc, d_off = is_ti ? 0.15 : 0.0;
+  const double dfc4 = 4.0 * dfc, dfc64 = 64.0 * dfc, d_off = is_ti ? 0.15 : 0.0;
   int rx = __builtin_amdgcn_readfirstlane(row_lo % R);
   for (int i = row_lo; i < row_hi; ++i, rx = rx + 1 == R ? 0 : rx + 1) {
     const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
@@ -461,19 +502,49 @@ __attribute__((amdgpu_waves_per_eu(VPT >= 4 ? 3 : VPT * S <= 2 ? 7 : 4, 8))) voi
     const float2 *__restrict__ row = meas + (size_t)i * F;
 
     double d[VPT];
-    float rr[VPT], ri[VPT], r2r[VPT], r2i[VPT], cr[VPT], ci[VPT];
+    float rr[VPT], ri[VPT], r2r[VPT], r2i[VPT], Rr[VPT], Ri[VPT], cr[VPT], ci[VPT];
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       d[k] = roundtrip_dist64(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono) + d_off;
       phasor_quarter(d[k] * dfc4, rr[k], ri[k]);
       r2r[k] = fmaf(rr[k], rr[k], -ri[k] * ri[k]);
       r2i[k] = 2.0f * rr[k] * ri[k];
+      if constexpr (K > 64) phasor_hw(d[k] * dfc64, Rr[k], Ri[k]);
+      else Rr[k] = Ri[k] = 0.0f;
       cr[k] = ci[k] = 0.0f;
     }
 
     for (int b = 0; b < nb; ++b)
-      rec_block<K, VPT, S>(row, b * K, f0c, dfc, d, rr, ri, r2r, r2i, cr, ci);
-    if (tail) {
+      rec_block<K, VPT, S>(row, b * K, f0c, dfc, d, rr, ri, r2r, r2i, Rr, Ri, cr, ci);
+    if (K > 64 && tail) {
+      // the last, partial interval: its 64-blocks and the remainder above
+      // them, combined from the top down as in rec_block
+      const int g0 = nb * K, nfull = tail / 64, rem = tail - 64 * nfull;
+      float ar[VPT], ai[VPT];
+      int q = nfull - 1;
+      if (rem) {
+        const float2 *__restrict__ blk = row + g0 + 64 * nfull;
+        const float2 top = blk[rem - 1];
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+          ar[k] = top.x;
+          ai[k] = top.y;
+        }
+        for (int t = rem - 2; t >= 0; --t) {
+          const float2 s = blk[t];
+          horner_step<VPT>(s.x, s.y, rr, ri, ar, ai);
+        }
+      } else {
+        block_h<64, VPT>(row + g0 + 64 * q, rr, ri, ar, ai);
+        --q;
+      }
+      for (; q >= 0; --q) {
+        float hr[VPT], hi[VPT];
+        block_h<64, VPT>(row + g0 + 64 * q, rr, ri, hr, hi);
+        outer_step<VPT>(Rr, Ri, hr, hi, ar, ai);
+      }
+      anchor_add<VPT>(d, fma((double)g0, dfc, f0c), ar, ai, cr, ci);
+    } else if (tail) {
       const float2 *__restrict__ blk = row + nb * K;
       float hr[VPT], hi[VPT];
       {
@@ -663,8 +734,13 @@ int dispatch_k(int K, hipStream_t st, const RecArgs &a) {
     case 16: launch_rec<16, VPT, S>(st, a); return 0;
     case 32: launch_rec<32, VPT, S>(st, a); return 0;
     case 64: launch_rec<64, VPT, S>(st, a); return 0;
-    default: return -1;
+    default: break;
   }
+  if constexpr (S == 1) {  // intervals longer than one Horner block
+    if (K == 128) { launch_rec<128, VPT, S>(st, a); return 0; }
+    if (K == 256) { launch_rec<256, VPT, S>(st, a); return 0; }
+  }
+  return -1;
 }
 
 // Lane shape: VPT voxels per lane, S-way split Horner.  The defaults (2, 1)
@@ -693,7 +769,12 @@ size_t ws_for_shape(int64_t n, int rows, bool mom) {
   size_t m = plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<8, VPT, S>), lim).ws;
   m = std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<16, VPT, S>), lim).ws);
   m = std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<32, VPT, S>), lim).ws);
-  return std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<64, VPT, S>), lim).ws);
+  m = std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<64, VPT, S>), lim).ws);
+  if constexpr (S == 1) {
+    m = std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<128, VPT, S>), lim).ws);
+    m = std::max(m, plan_rec(n, rows, mom, VPT, resident_slots(k_bp_rec<256, VPT, S>), lim).ws);
+  }
+  return m;
 }
 
 size_t ws_for_rec(int64_t n, int rows, bool mom) {
@@ -789,8 +870,9 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
               MMN_ERR_ARG,
               "mmn_bp_forward: unknown variant %d", variant);
   if (K == 0) K = 64;
-  MMN_REQUIRE(variant != MMN_BP_REC || K == 8 || K == 16 || K == 32 || K == 64,
-              MMN_ERR_ARG, "mmn_bp_forward: K must be 8, 16, 32 or 64 (got %d)",
+  MMN_REQUIRE(variant != MMN_BP_REC || K == 8 || K == 16 || K == 32 || K == 64 ||
+                  K == 128 || K == 256,
+              MMN_ERR_ARG, "mmn_bp_forward: K must be 8, 16, 32, 64, 128 or 256 (got %d)",
               K);
   const int64_t ngrid = (int64_t)nx * ny * nz;
   int64_t nvox = ngrid;

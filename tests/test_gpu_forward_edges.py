@@ -52,6 +52,30 @@ def test_ragged_bistatic_ti_vs_fp64(scene, oracle, K, is_ti):
     assert rel(mom.cpu().numpy().reshape(-1, 3), wmom) <= 1e-5
 
 
+@pytest.mark.parametrize("K", [128, 256])
+@pytest.mark.parametrize("F", [77, 192, 256, 320, 333])
+def test_long_reanchor_intervals_vs_fp64(oracle, K, F):
+    """Re-anchor intervals of 128 and 256 samples (64-sample Horner chains
+    combined in r^64): sample counts below one interval, exact multiples,
+    and partial last intervals whose remainder is empty or not, bistatic
+    with the TI offset."""
+    cfg = synth.Config(1, 10, 2e-3, F, (10, 9, 7), 2e-3, 20.0, "points", 5)
+    sc = synth.make_scene(1, cfg=cfg, seed=41)
+    rxo = np.array([[0.0, 0.0, 0.0], [-3.1e-3, 0.0, 0.0], [0.0, 2.6e-3, 0.0]])
+    rng = np.random.default_rng(F)
+    rows = np.repeat(sc.ant, 3, axis=0)
+    meas = (rng.normal(size=(len(rows), F)) + 1j * rng.normal(size=(len(rows), F))
+            ).astype(np.complex64)
+    img, mom = ops.backproject(T(meas, np.complex64), T(rows, np.float32),
+                               T(sc.x_locs, np.float32), T(sc.y_locs, np.float32),
+                               T(sc.z_locs, np.float32), T(rxo, np.float32), f0=sc.f0,
+                               df=sc.df, variant="rec", K=K, moment=True, is_ti=True)
+    want, wmom = oracle.bp_f64(sc.x_locs, sc.y_locs, sc.z_locs, rows, meas, rxo, sc.f0, sc.df,
+                               is_ti=True, moment=True)
+    assert rel(img.cpu().numpy(), want) <= 1e-5
+    assert rel(mom.cpu().numpy().reshape(-1, 3), wmom) <= 1e-5
+
+
 def test_single_and_repeated_voxels(scene, oracle):
     sc, rows, meas, rxo = scene
     args = (T(meas, np.complex64), T(rows, np.float32), T(sc.x_locs, np.float32),

@@ -87,15 +87,22 @@ def test_precision_report_config3(cfg3):
     assert table["profile"]["max_normal_deg"] <= 0.01
 
 
-def test_config2_full_size_sampled_parity(oracle):
-    """Config 2 at full size: rec and plain on 384 seeded voxels vs the FP64 oracle."""
+@pytest.fixture(scope="module")
+def scene2():
+    return synth.make_scene(2, device=torch.device("cuda", 0))
+
+
+@pytest.mark.parametrize("K", [64, 256])
+def test_config2_full_size_sampled_parity(oracle, scene2, K):
+    """Config 2 at full size: rec (re-anchor every 64 samples, and every 256 with
+    64-sample chains) and plain on 384 seeded voxels vs the FP64 oracle."""
     dev = torch.device("cuda", 0)
-    sc = synth.make_scene(2, device=dev)
+    sc = scene2
     V = sc.cfg.nvox
     rng = np.random.default_rng(17)
     full = ops.backproject(T(sc.meas, np.complex64), T(sc.ant, np.float32),
                            *(T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs)),
-                           f0=sc.f0, df=sc.df).reshape(-1)
+                           f0=sc.f0, df=sc.df, K=K).reshape(-1)
     mag = full.abs()
     strong = torch.topk(mag, 192).indices.cpu().numpy()
     vidx = np.unique(np.concatenate([strong, rng.choice(V, 192, replace=False)]))
@@ -103,6 +110,10 @@ def test_config2_full_size_sampled_parity(oracle):
                         sc.f0, sc.df, vidx=vidx)
     got = full[torch.from_numpy(vidx).to(dev)].cpu().numpy().astype(np.complex128)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-4
+    s = np.isin(vidx, strong)   # the strongest voxels one by one
+    assert np.max(np.abs(got[s] - ref[s]) / np.abs(ref[s])) <= 1e-5
+    if K != 64:
+        return
     sub = ops.backproject(T(sc.meas, np.complex64), T(sc.ant, np.float32),
                           *(T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs)),
                           f0=sc.f0, df=sc.df, variant="plain",
