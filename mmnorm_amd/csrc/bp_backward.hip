@@ -9,16 +9,22 @@
 //   dL/dp_i  = scale * sum_v Im( sum_j k_j s_ij conj(z_ijv) ) grad_p d_iv,
 //              z_ijv = g_v conj(phi_ijv),  k_j = 2 pi f_j / c,
 //              grad_p d = -(v-p)/|v-p| - (v-p-o)/|v-p-o|.
-// Both reduce over voxels.  A wave owns one (row, block of K samples, voxel
-// chunk): its lanes walk the chunk's voxels with coalesced record loads and
-// keep lane-local sums, which one butterfly across the wave reduces at the
-// end.  The block's K position-term coefficients are wave-uniform and come
-// through scalar loads as SGPR operands.  Per (lane, voxel) the K conjugate
-// phasors come from one FP64-reduced anchor and a two-term FP32 recurrence;
-// per sample the lane does the recurrence (2 FMA), one accumulate (2 add)
-// and the position term (2 FMA).  Each wave writes its partial sums to the
-// workspace; a second kernel reduces the partials in a fixed order (FP64),
-// so results are bitwise reproducible.
+// Both reduce over voxels.  A workgroup of four waves owns one row, four
+// blocks of K samples (one per wave) and one voxel chunk; the chunk is walked
+// in steps of 256 voxels.  In each step every thread first forms one voxel's
+// block-independent set-up -- the FP64 round-trip distance, the position
+// gradient of the distance, the step phasor -- into LDS (double-buffered, one
+// barrier per step), so that the four waves share it; then each wave runs its
+// block over the step's voxels, two per lane at a time, with only its block's
+// FP64-reduced anchor computed per wave.  Lanes keep lane-local sums, which
+// one butterfly across the wave reduces at the end.  The block's K
+// position-term coefficients are wave-uniform and come through scalar loads
+// as SGPR operands.  Per (lane, voxel) the K conjugate phasors come from the
+// anchor and a two-term FP32 recurrence; per sample the lane does the
+// recurrence (2 FMA), one accumulate (2 add) and the position term (2 FMA).
+// Each wave writes its partial sums to the workspace; a second kernel
+// reduces the partials in a fixed order (FP64), so results are bitwise
+// reproducible.
 #include "mmn_internal.h"
 
 namespace {
@@ -26,18 +32,17 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kVV = 2;                 // voxels per lane per step
-constexpr int kPad = kWave * kVV;      // voxel records are padded to this
+constexpr int kVV = 2;                 // voxels per lane at a time
+constexpr int kStep = kBlock;          // voxels per workgroup step (one per thread)
+constexpr int kPass = kWave * kVV;     // voxels of one wave pass over a step
+constexpr int kPad = kStep;            // voxel records are padded to this
 constexpr int kMinK = 16;              // smallest sample block (workspace sizing)
 constexpr int kMaxK = 32;
 
-// Per-voxel record: position and Re(g) in one 16-byte load, Im(g) in a
-// second array (20 bytes per voxel in all).  Records past the voxel count
-// (up to a multiple of kPad) repeat the last voxel's position with g = 0.
-struct VoxRec {
-  float4 p;  // vx, vy, vz, Re g
-  float gi;  // Im g
-};
+// Per-voxel record: position and Re(g) in one 16-byte array element (rec4),
+// Im(g) in a second array (recg), 20 bytes per voxel in all.  Records past
+// the voxel count (up to a multiple of kPad) repeat the last voxel's
+// position with g = 0.
 
 int64_t pad_vox(int64_t nvox) { return (nvox + kPad - 1) / kPad * kPad; }
 
@@ -122,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_coef(float2 *__restrict__ coef,
   coef[t] = c;
 }
 
-// Per (row a, voxel v) state of one sample block: the conjugate phasors
+// Per (row a, voxel v) and sample block the conjugate phasors
 // z_t = g_v exp(+j 2 pi f_t d_av / c) for t = 0..K-1 satisfy, because the
 // step exp(+j w) has modulus one, the real-coefficient recurrence
 //     z_{t+1} = 2 cos(w) z_t - z_{t-1}
@@ -131,70 +136,24 @@ __global__ __launch_bounds__(kBlock) void k_coef(float2 *__restrict__ coef,
 // linearly, amplified by 1/sin(w); for the bands and ranges here
 // (w = 2 pi df d / c between 0.05 and 1 rad) that stays below 1e-5 over a
 // 32-sample block.
-struct VoxStart {
-  float z0r, z0i, z1r, z1i, c2;  // c2 = 2 cos(w)
-  float gx, gy, gz;              // grad_p d (sign applied by the caller)
+// Shared row-level set-up of one voxel: FP64 round-trip distance, the
+// position gradient of the distance, the step phasor exp(+j w) and g_v.
+struct VoxShared {
+  double d;
+  float gx, gy, gz, rr, ri, gr, gi, pad;
 };
 
-__device__ __forceinline__ VoxStart vox_start(const VoxRec &r, float px, float py,
-                                              float pz, float ox, float oy, float oz,
-                                              bool mono, int is_ti, double fc,
-                                              float dfc) {
-  VoxStart v;
-  // round-trip distance in FP64 (as roundtrip_dist64)
-  const double dx = (double)r.p.x - (double)px, dy = (double)r.p.y - (double)py,
-               dz = (double)r.p.z - (double)pz;
-  const double s1 = fma(dz, dz, fma(dy, dy, dx * dx));
-  const float y1 = __builtin_amdgcn_rsqf((float)s1);
-  const double y1d = (double)y1;
-  const double d0 = s1 * y1d;
-  const double d1 = fma(0.5 * y1d, fma(-d0, d0, s1), d0);
-  // first-leg difference in FP32: the correctly rounded FP32 difference is
-  // the exact (FP64) one rounded once
-  const float fx = r.p.x - px, fy = r.p.y - py, fz = r.p.z - pz;
-  double d;
-  if (mono) {
-    d = d1 + d1;
-    const float y2 = y1 + y1;
-    v.gx = fx * y2;
-    v.gy = fy * y2;
-    v.gz = fz * y2;
-  } else {
-    const double ex = dx - (double)ox, ey = dy - (double)oy, ez = dz - (double)oz;
-    d = d1 + sqrt_nr64(fma(ez, ez, fma(ey, ey, ex * ex)));
-    const float bx = (float)ex, by = (float)ey, bz = (float)ez;
-    const float n2 = rsqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
-    v.gx = fmaf(bx, n2, fx * y1);
-    v.gy = fmaf(by, n2, fy * y1);
-    v.gz = fmaf(bz, n2, fz * y1);
-  }
-  if (is_ti) d += 0.15;
-  float ar, ai;
-  phasor_hw(d * fc, ar, ai);  // exp(-j ...); conjugated below
-  // the step phase df d / c is a few hundredths of a turn: FP32 keeps it to
-  // ~1e-9 turns, and a block's K steps to ~1e-7
-  const float ts = (float)d * dfc;
-  const float fr = ts - rintf(ts);
-  const float rr = __builtin_amdgcn_cosf(fr);
-  const float ri = __builtin_amdgcn_sinf(fr);  // exp(+j w)
-  ai = -ai;
-  const float gr = r.p.w, gi = r.gi;
-  v.z0r = gr * ar - gi * ai;
-  v.z0i = gr * ai + gi * ar;
-  v.z1r = v.z0r * rr - v.z0i * ri;
-  v.z1i = v.z0r * ri + v.z0i * rr;
-  v.c2 = rr + rr;
-  return v;
-}
-
-// One wave per (row a, sample block fb, voxel chunk c).  MEAS: the adjoint
-// (dL/dmeas) is accumulated; ANT: the position term.  At 4 waves per SIMD
-// the K = 32 kernel with the adjoint fits in 128 VGPRs (64 of them the
-// adjoint accumulators); the register cap costs a few spills in the epilogue
-// only, and the extra wave per SIMD is worth ~10 % (measured against 3
-// waves at 148 VGPRs).  Without the adjoint (position gradient only, as
-// when the measurements take no gradient) there are no accumulators and
-// the kernel runs at 8 waves per SIMD.
+// One workgroup per (row a, group fbg of 4 sample blocks, voxel chunk c);
+// wave w takes sample block 4 fbg + w (a wave past the row's last block runs
+// along for the barriers and writes nothing).  MEAS: the adjoint (dL/dmeas)
+// is accumulated; ANT: the position term.  With the adjoint the K = 32
+// kernel fits in 128 VGPRs (64 of them the accumulators) at 4 waves per SIMD,
+// the cap costing a few spills in the epilogue only; without it (position
+// gradient only, as when the measurements take no gradient) there are no
+// accumulators and it runs at 7 waves per SIMD.  Measured at config 2 (3600
+// rows, 256 samples, 1.57 M voxels) on one box: 195 ms with both gradients,
+// 143 ms for the position gradient alone, against 220 / 175 ms when every
+// wave formed its own voxels' set-up.
 template <int K, bool MEAS, bool ANT>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(MEAS ? 4 : 7))) void k_bwd(
@@ -202,93 +161,140 @@ __attribute__((amdgpu_waves_per_eu(MEAS ? 4 : 7))) void k_bwd(
     const float4 *__restrict__ rec4, const float *__restrict__ recg, int nvp,
     int chunk, const float *__restrict__ ant, const float2 *__restrict__ coef,
     const float *__restrict__ rxo, double f0c, float dfc32, double dfc, int A, int R,
-    int F, int is_ti, int nfbmax, int nfb, int64_t nitems) {
+    int F, int is_ti, int nfbmax, int nfb, int nfbg) {
   static_assert(K % 8 == 0, "coefficients are read 8 at a time");
+  static_assert(MEAS || ANT, "nothing to compute");
+  __shared__ VoxShared sh[2][kStep];
   const int lane = threadIdx.x % kWave;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + wv;
-  const bool live = item < nitems;
-  const int64_t it = live ? item : nitems - 1;
-  const int a = (int)(it % A);
-  const int64_t rest = it / A;
-  const int fb = (int)(rest % nfb);
-  const int c = (int)(rest / nfb);
-  const int f_lo = fb * K;
+  const int64_t wg = blockIdx.x;
+  const int a = (int)(wg % A);
+  const int64_t rest = wg / A;
+  const int fbg = (int)(rest % nfbg);
+  const int c = (int)(rest / nfbg);
+  const int fb = fbg * kWavesPerBlock + wv;
+  const bool live = fb < nfb;
+  const int fbl = live ? fb : nfb - 1;
+  const int f_lo = fbl * K;
   const int nf = min(K, F - f_lo);
-  const float2 *__restrict__ cf = coef + ((size_t)a * nfb + fb) * K;
+  const float2 *__restrict__ cf = coef + ((size_t)a * nfb + fbl) * K;
 
   const float px = ant[3 * a + 0], py = ant[3 * a + 1], pz = ant[3 * a + 2];
   const int rx = a % R;
   const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
   const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
   const double fc = fma((double)f_lo, dfc, f0c);
+  const double d_off = is_ti ? 0.15 : 0.0;
 
-  static_assert(MEAS || ANT, "nothing to compute");
   float accr[MEAS ? K : 1], acci[MEAS ? K : 1];
 #pragma unroll
   for (int t = 0; t < (MEAS ? K : 1); ++t) accr[t] = acci[t] = 0.0f;
   float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
 
-  // chunk and the padded voxel count are multiples of kPad, so every voxel
-  // vb + 64 u below is a real or padding record
   const int v0 = c * chunk;
   const int v1 = v0 + min(chunk, nvp - v0);
-  for (int vb = v0 + lane; vb < v1; vb += kPad) {
-    VoxStart p[kVV];
-#pragma unroll
-    for (int u = 0; u < kVV; ++u) {
-      VoxRec r;
-      r.p = rec4[vb + kWave * u];
-      r.gi = recg[vb + kWave * u];
-      p[u] = vox_start(r, px, py, pz, ox, oy, oz, mono, is_ti, fc, dfc32);
-    }
-    float r0[kVV], i0[kVV], r1[kVV], i1[kVV], tt[kVV];
-#pragma unroll
-    for (int u = 0; u < kVV; ++u) {
-      r0[u] = p[u].z0r; i0[u] = p[u].z0i; r1[u] = p[u].z1r; i1[u] = p[u].z1i;
-      tt[u] = 0.0f;
-    }
-    float2 sp[8];
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-      if (ANT && t % 8 == 0) {
-        // 8 coefficients at a time through scalar loads (cache hits), so
-        // that K complex values never sit in scalar registers at once
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int q = 0; q < 8; ++q) sp[q] = cf[t + q];
+  int buf = 0;
+  for (int vb0 = v0; vb0 < v1; vb0 += kStep, buf ^= 1) {
+    {  // phase A: this thread's voxel
+      const int v = vb0 + (int)threadIdx.x;
+      const float4 p4 = rec4[v];
+      const float gim = recg[v];
+      const double dx = (double)p4.x - (double)px, dy = (double)p4.y - (double)py,
+                   dz = (double)p4.z - (double)pz;
+      const double s1 = fma(dz, dz, fma(dy, dy, dx * dx));
+      const float y1 = __builtin_amdgcn_rsqf((float)s1);
+      const double y1d = (double)y1;
+      const double e0 = s1 * y1d;
+      const double d1 = fma(0.5 * y1d, fma(-e0, e0, s1), e0);
+      const float fx = p4.x - px, fy = p4.y - py, fz = p4.z - pz;
+      double d;
+      VoxShared o;
+      if (mono) {
+        d = d1 + d1;
+        const float y2 = y1 + y1;
+        o.gx = fx * y2;
+        o.gy = fy * y2;
+        o.gz = fz * y2;
+      } else {
+        const double ex = dx - (double)ox, ey = dy - (double)oy, ez = dz - (double)oz;
+        d = d1 + sqrt_nr64(fma(ez, ez, fma(ey, ey, ex * ex)));
+        const float bx = (float)ex, by = (float)ey, bz = (float)ez;
+        const float n2 = rsqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
+        o.gx = fmaf(bx, n2, fx * y1);
+        o.gy = fmaf(by, n2, fy * y1);
+        o.gz = fmaf(bz, n2, fz * y1);
       }
-      if constexpr (MEAS) {
-        float sr = r0[0], si = i0[0];
-#pragma unroll
-        for (int u = 1; u < kVV; ++u) {
-          sr += r0[u];
-          si += i0[u];
-        }
-        accr[t] += sr;
-        acci[t] += si;
-      }
+      d += d_off;
+      const float ts = (float)d * dfc32;
+      const float fr = ts - rintf(ts);
+      o.rr = __builtin_amdgcn_cosf(fr);
+      o.ri = __builtin_amdgcn_sinf(fr);
+      o.gr = p4.w;
+      o.gi = gim;
+      o.d = d;
+      o.pad = 0.0f;
+      sh[buf][threadIdx.x] = o;
+    }
+    __syncthreads();
+    // phase B: this wave's block over the step's voxels, two per lane at a time
+#pragma unroll 1
+    for (int j = 0; j < kStep / kPass; ++j) {
+      float r0[kVV], i0[kVV], r1[kVV], i1[kVV], c2[kVV], tt[kVV], gx[kVV], gy[kVV], gz[kVV];
 #pragma unroll
       for (int u = 0; u < kVV; ++u) {
-        if constexpr (ANT) {
-          tt[u] = fmaf(sp[t % 8].x, r0[u], tt[u]);
-          tt[u] = fmaf(sp[t % 8].y, i0[u], tt[u]);
-        }
-        const float r2 = fmaf(p[u].c2, r1[u], -r0[u]);
-        const float i2 = fmaf(p[u].c2, i1[u], -i0[u]);
-        r0[u] = r1[u]; i0[u] = i1[u]; r1[u] = r2; i1[u] = i2;
+        const VoxShared q = sh[buf][j * kPass + u * kWave + lane];
+        float ar, ai;
+        phasor_hw(q.d * fc, ar, ai);
+        ai = -ai;
+        r0[u] = q.gr * ar - q.gi * ai;
+        i0[u] = q.gr * ai + q.gi * ar;
+        r1[u] = r0[u] * q.rr - i0[u] * q.ri;
+        i1[u] = r0[u] * q.ri + i0[u] * q.rr;
+        c2[u] = q.rr + q.rr;
+        gx[u] = q.gx;
+        gy[u] = q.gy;
+        gz[u] = q.gz;
+        tt[u] = 0.0f;
       }
-    }
-    if constexpr (ANT) {
+      float2 sp[8];
 #pragma unroll
-      for (int u = 0; u < kVV; ++u) {
-        gpx = fmaf(-tt[u], p[u].gx, gpx);
-        gpy = fmaf(-tt[u], p[u].gy, gpy);
-        gpz = fmaf(-tt[u], p[u].gz, gpz);
+      for (int t = 0; t < K; ++t) {
+        if (ANT && t % 8 == 0) {
+          asm volatile("" ::: "memory");
+#pragma unroll
+          for (int qq = 0; qq < 8; ++qq) sp[qq] = cf[t + qq];
+        }
+        if constexpr (MEAS) {
+          float sr = r0[0], si = i0[0];
+#pragma unroll
+          for (int u = 1; u < kVV; ++u) {
+            sr += r0[u];
+            si += i0[u];
+          }
+          accr[t] += sr;
+          acci[t] += si;
+        }
+#pragma unroll
+        for (int u = 0; u < kVV; ++u) {
+          if constexpr (ANT) {
+            tt[u] = fmaf(sp[t % 8].x, r0[u], tt[u]);
+            tt[u] = fmaf(sp[t % 8].y, i0[u], tt[u]);
+          }
+          const float r2 = fmaf(c2[u], r1[u], -r0[u]);
+          const float i2 = fmaf(c2[u], i1[u], -i0[u]);
+          r0[u] = r1[u]; i0[u] = i1[u]; r1[u] = r2; i1[u] = i2;
+        }
+      }
+      if constexpr (ANT) {
+#pragma unroll
+        for (int u = 0; u < kVV; ++u) {
+          gpx = fmaf(-tt[u], gx[u], gpx);
+          gpy = fmaf(-tt[u], gy[u], gpy);
+          gpz = fmaf(-tt[u], gz[u], gpz);
+        }
       }
     }
   }
-  // butterfly over the wave: every lane ends with the wave's totals
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) {
     if constexpr (MEAS) {
@@ -397,15 +403,15 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   // the position gradient alone (no adjoint accumulators, twice the waves)
   using KernT = void (*)(float2 *, float *, const float4 *, const float *, int, int,
                          const float *, const float2 *, const float *, double, float, double,
-                         int, int, int, int, int, int, int64_t);
+                         int, int, int, int, int, int, int);
   const bool gm_on = grad_meas != nullptr, ga_on = grad_ant != nullptr;
   KernT kern;
   if (K == 16)
     kern = gm_on && ga_on ? k_bwd<16, true, true> : gm_on ? k_bwd<16, true, false>
-                                                         : k_bwd<16, false, true>;
+                                                            : k_bwd<16, false, true>;
   else
     kern = gm_on && ga_on ? k_bwd<32, true, true> : gm_on ? k_bwd<32, true, false>
-                                                         : k_bwd<32, false, true>;
+                                                            : k_bwd<32, false, true>;
   int per_cu = 0, dev = 0, cus = 256;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
       per_cu < 1)
@@ -415,9 +421,10 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   (void)hipGetLastError();
   // voxel chunks: enough waves for 8 rounds of resident slots, at least 256
   // voxels per wave, at most the workspace's chunk count
-  const int64_t slots_waves = (int64_t)per_cu * cus * kWavesPerBlock;
-  const int64_t pairs = (int64_t)A * nfb;
-  int64_t ch = (8 * slots_waves + pairs - 1) / pairs;
+  const int nfbg = (nfb + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t slots_wg = (int64_t)per_cu * cus;
+  const int64_t pairs = (int64_t)A * nfbg;
+  int64_t ch = (8 * slots_wg + pairs - 1) / pairs;
   const int64_t by_vox = (nvox + 255) / 256;
   if (ch > by_vox) ch = by_vox;
   const int cmax = chunk_max(A, F, nvox);
@@ -428,7 +435,7 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   int64_t chunk = (npad + ch - 1) / ch;
   chunk = (chunk + kPad - 1) / kPad * kPad;
   ch = (npad + chunk - 1) / chunk;
-  const int64_t nitems = pairs * ch;
+  const int64_t nwgs = pairs * ch;
   hipLaunchKernelGGL(k_pack, dim3(mmn_blocks(npad, kBlock)), dim3(kBlock), 0, st, w.rec4,
                      w.recg, (const float2 *)g, x_locs, y_locs, z_locs, vox_idx, nvox, npad,
                      ny, nz);
@@ -438,10 +445,10 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   const int Fs = nfb * K;
   hipLaunchKernelGGL(k_coef, dim3(mmn_blocks((int64_t)A * Fs, kBlock)), dim3(kBlock), 0, st,
                      w.coef, (const float2 *)meas, k0, dk, F, Fs, (int64_t)A * Fs);
-  hipLaunchKernelGGL(kern, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
-                     dim3(kBlock), 0, st, w.gs, w.gp, w.rec4, w.recg, (int)npad, (int)chunk,
-                     ant, (const float2 *)w.coef, rx_off, f0 * inv_c, (float)(df * inv_c),
-                     df * inv_c, A, num_rx, F, is_ti_radar, nfbmax, nfb, nitems);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nwgs), dim3(kBlock), 0, st, w.gs, w.gp, w.rec4,
+                     w.recg, (int)npad, (int)chunk, ant, (const float2 *)w.coef, rx_off,
+                     f0 * inv_c, (float)(df * inv_c), df * inv_c, A, num_rx, F, is_ti_radar,
+                     nfbmax, nfb, nfbg);
   if (grad_meas)
     hipLaunchKernelGGL(k_reduce_meas, dim3(mmn_blocks((int64_t)A * F, kBlock)),
                        dim3(kBlock), 0, st, (float2 *)grad_meas, w.gs, (int)ch,
