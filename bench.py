@@ -342,6 +342,7 @@ def main(argv=None):
         if a.plain_steps > 0:
             sp = {"rec_over_plain_fast": f_s / k_avg, "rec_over_plain_sincosf": p_s / k_avg,
                   "target": 20.0, "target_met_vs_plain_fast": bool(f_s / k_avg >= 20.0),
+                  "target_met_vs_plain_sincosf": bool(p_s / k_avg >= 20.0),
                   "denominator": "plain_fast: the optimised direct FP32 kernel (hardware "
                                  "sin/cos of the phase fraction; the same accuracy as "
                                  "plain, whose library sincosf of the ~1000 rad phase pays "

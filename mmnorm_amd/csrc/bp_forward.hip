@@ -160,7 +160,8 @@ __global__ __launch_bounds__(kBlock) void k_bp_plain(
       if (is_ti) d[k] += 0.15f;
       cr[k] = ci[k] = 0.0f;
     }
-    for (int j = 0; j < F; ++j) {
+    // one sample for every voxel of the lane
+    auto sample = [&](int j) {
       const float2 m = row[j];
       const float kj = fmaf((float)j, dk, k0);
       const float uj = fmaf((float)j, du, u0);
@@ -181,7 +182,15 @@ __global__ __launch_bounds__(kBlock) void k_bp_plain(
         cr[k] = fmaf(m.x, c, fmaf(m.y, s, cr[k]));
         ci[k] = fmaf(m.y, c, fmaf(-m.x, s, ci[k]));
       }
+    };
+    // groups of 8 samples, so that their loads go out together (one 64-byte
+    // scalar load) instead of one load and one wait per sample
+    int j = 0;
+    for (; j + 8 <= F; j += 8) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sample(j + q);
     }
+    for (; j < F; ++j) sample(j);
     const float vax = px + ox / 2.0f, vay = py + oy / 2.0f, vaz = pz + oz / 2.0f;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
@@ -891,11 +900,20 @@ extern "C" int mmn_bp_forward(float *img, float *moment, const float *x_locs,
     const float dk = (float)(2.0 * kPi * df * inv_c);
     const float u0 = (float)(f0 * inv_c), du = (float)(df * inv_c);
     const bool fast = variant == MMN_BP_PLAIN_FAST;
-    // VPT 1: the transcendental-bound loop gains nothing from a second voxel
-    auto kern = fast ? k_bp_plain<1, true> : k_bp_plain<1, false>;
-    hipLaunchKernelGGL(kern, dim3(mmn_blocks(nvox, kBlock)), dim3(kBlock), 0, st, im, mo, x_locs,
-                       y_locs, z_locs, vox_idx, nvox, ant, ms, rx_off, k0, dk, u0, du, ny,
-                       nz, num_rows, num_rx, samples, is_ti_radar);
+    // Voxels per lane (MMN_PLAIN_VPT = 1, 2 or 4 overrides): measured at
+    // config 2 with the samples in groups of 8, the library-sincosf kernel
+    // 881 / 913 / 1152 ms at 1 / 2 / 4 and the hardware sin/cos kernel
+    // 453 / 430 / 432 ms (3430 and 457 ms with one load and wait per sample).
+    static const int pvpt_env = env_int("MMN_PLAIN_VPT", 0);
+    const int pvpt = pvpt_env == 1 || pvpt_env == 2 || pvpt_env == 4 ? pvpt_env
+                                                                      : (fast ? 2 : 1);
+    auto kern = fast ? (pvpt == 4 ? k_bp_plain<4, true> : pvpt == 2 ? k_bp_plain<2, true>
+                                                                    : k_bp_plain<1, true>)
+                     : (pvpt == 4 ? k_bp_plain<4, false> : pvpt == 2 ? k_bp_plain<2, false>
+                                                                     : k_bp_plain<1, false>);
+    hipLaunchKernelGGL(kern, dim3(mmn_blocks(nvox, kBlock * pvpt)), dim3(kBlock), 0, st, im, mo,
+                       x_locs, y_locs, z_locs, vox_idx, nvox, ant, ms, rx_off, k0, dk, u0, du,
+                       ny, nz, num_rows, num_rx, samples, is_ti_radar);
     return mmn_check_launch("mmn_bp_forward(plain)");
   }
   RecArgs ra{im, mo, x_locs, y_locs, z_locs, vox_idx, nvox, ant, ms, rx_off,
