@@ -203,6 +203,18 @@ int mmn_bp_interp(float *img, float *moment, const float *x_locs, const float *y
                   void *stream);
 
 /*
+ * mmn_sqrt_rn_check -- self-check of the square root mmn_bp_interp uses for
+ * the reference's FP32 distances (the hardware square root and a neighbour
+ * test, in place of sqrtf's longer sequence): over the float bit patterns
+ * [first, first + count) (first + count <= 2^32) adds to result[0] the number
+ * whose root differs from sqrtf's (NaNs compare equal) and lowers result[1]
+ * to the smallest such pattern.  result is device memory the caller sets to
+ * {0, ~0} first.  No reference counterpart: it pins that the distances, and
+ * so the range bins, stay those of the reference's correctly rounded sqrtf.
+ */
+int mmn_sqrt_rn_check(uint64_t first, uint64_t count, unsigned long long *result, void *stream);
+
+/*
  * mmn_sn_from_moment -- normals[n][3] = Re(conj(I) * moment[n][3]) / |I| with
  * I = img[vox_idx[n]] (img indexed by linear voxel index).  Completes the
  * full-frequency vote (the reference's weight, surface_normal_est.cu:138-139,

@@ -53,6 +53,7 @@ SIGNATURES = {
     "mmn_bp_profile_backward": (_c_int, [_p, _p, _p, _c_d, _p, _p, _p, _p, _c_i64, _p, _p,
                                          _p, _c_d, _c_d, _c_int, _c_int, _c_int, _c_int,
                                          _c_int, _c_int, _c_int, _p, _p, _sz, _p]),
+    "mmn_sqrt_rn_check": (_c_int, [ctypes.c_uint64, ctypes.c_uint64, _p, _p]),
     "mmn_bp_interp": (_c_int, [_p, _p, _p, _p, _p, _p, _c_i64, _p, _p, _p, _c_f, _c_f,
                                _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
     "mmn_sn_from_moment": (_c_int, [_p, _p, _p, _p, _c_i64, _p]),

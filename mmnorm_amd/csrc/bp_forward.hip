@@ -104,6 +104,18 @@ __device__ __forceinline__ void load_voxels(
   }
 }
 
+// Marks the voxel coordinates as ready before a row loop that gathers: the
+// wait for their loads then comes once, here.  Otherwise the first use inside
+// the loop carries a wait on the load counter, which in every later row also
+// waits for the gathers issued before it in that row, so that a lane's
+// gathers return one after another instead of together.
+template <int VPT>
+__device__ __forceinline__ void settle_voxels(float (&vx)[VPT], float (&vy)[VPT],
+                                              float (&vz)[VPT]) {
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) asm volatile("" : "+v"(vx[k]), "+v"(vy[k]), "+v"(vz[k]));
+}
+
 // Voxel t (< nvox) of the launch goes to img[t - off] (and mom[t - off]).
 template <int VPT>
 __device__ __forceinline__ void store_voxels(
@@ -217,16 +229,58 @@ __global__ __launch_bounds__(kBlock) void k_bp_plain(
 // M = sum_i (p_i + o_i/2 - v) c_i in the same pass, so that the vote of
 // surface_normal_est.cu:110-149 is Re(conj(I) M) / |I| without a second pass.
 // --------------------------------------------------------------------------
+// The correctly rounded FP32 square root (round to nearest even, the value
+// sqrtf returns) in fewer instructions than sqrtf: the hardware square root
+// s (within 1 ulp) and a test of its neighbours by the sign of the residual
+// x - s's, the correction the compiler's own sqrtf applies.  sqrtf adds a
+// scaling of arguments below 2^-96, where the hardware result is not within
+// 1 ulp, and a select for 0 and infinity; the neighbour test already returns
+// 0, -0, infinity and NaN unchanged, so only arguments below 2^-96 (squared
+// distances under 1.3e-29 m^2, and negative ones) go to sqrtf here.
+// mmn_sqrt_rn_check compares the two over every float.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (x < 0x1p-96f) return sqrtf(x);
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  float r = fmaf(-dn, s, x) <= 0.0f ? dn : s;
+  r = fmaf(-up, s, x) > 0.0f ? up : r;
+  return r;
+}
+
+// Bit patterns [first, first + count) as floats: counts where sqrt_rn and
+// sqrtf differ (NaN results compare equal whatever their payload) in res[0]
+// and keeps the smallest such pattern in res[1].
+__global__ __launch_bounds__(kBlock) void k_sqrt_rn_check(unsigned long long *__restrict__ res,
+                                                          uint64_t first, uint64_t count) {
+  unsigned long long bad = 0, lo = ~0ull;
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < count;
+       t += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t bits = (uint32_t)(first + t);
+    const float x = __uint_as_float(bits);
+    const float a = sqrt_rn(x), b = sqrtf(x);
+    const bool same = __float_as_uint(a) == __float_as_uint(b) || (isnan(a) && isnan(b));
+    if (!same) {
+      ++bad;
+      lo = min(lo, (unsigned long long)bits);
+    }
+  }
+  if (bad) {
+    atomicAdd(&res[0], bad);
+    atomicMin(&res[1], lo);
+  }
+}
+
 __device__ __forceinline__ float interp_dist(float vx, float vy, float vz, float px, float py,
                                              float pz, float ox, float oy, float oz, bool mono,
                                              int is_ti) {
 #pragma clang fp contract(off)
   const float ax = vx - px, ay = vy - py, az = vz - pz;
-  const float fwd = sqrtf(ax * ax + ay * ay + az * az);
+  const float fwd = sqrt_rn(ax * ax + ay * ay + az * az);
   float back = fwd;  // o = 0: the second leg is the first, bit for bit
   if (!mono) {
     const float bx = ax - ox, by = ay - oy, bz = az - oz;
-    back = sqrtf(bx * bx + by * by + bz * bz);
+    back = sqrt_rn(bx * bx + by * by + bz * bz);
   }
   float d = fwd + back;
   if (is_ti) d = (float)((double)d + 0.15);
@@ -241,12 +295,14 @@ __device__ __forceinline__ int interp_bin(float d, float fft_spacing) {
 // floor(RN(d / fs) / 2) without the correctly rounded division where it is
 // not needed: q = d * RN(1 / fs) is within 2 ulp of RN(d / fs), so when the
 // interval q * (1 -+ 4e-7) (3.3 ulp each side) holds no even integer the
-// floor is decided; otherwise the exact division runs.
+// floor is decided; otherwise the exact division runs.  The halving is
+// folded into the factors: RN(q * RN(1 -+ 4e-7)) / 2 = RN(q * (RN(1 -+ 4e-7) / 2))
+// exactly, a power-of-two scale commuting with rounding (no underflow here).
 __device__ __forceinline__ int interp_bin_fast(float d, float fft_spacing, float inv_fs) {
 #pragma clang fp contract(off)
   const float q = d * inv_fs;
-  const float a = floorf(q * (1.0f - 4e-7f) * 0.5f);
-  const float b = floorf(q * (1.0f + 4e-7f) * 0.5f);
+  const float a = floorf(q * ((1.0f - 4e-7f) * 0.5f));
+  const float b = floorf(q * ((1.0f + 4e-7f) * 0.5f));
   if (a == b) return (int)a;
   return interp_bin(d, fft_spacing);
 }
@@ -254,6 +310,31 @@ __device__ __forceinline__ int interp_bin_fast(float d, float fft_spacing, float
 __device__ __forceinline__ float half_off(float p, float o) {
 #pragma clang fp contract(off)
   return p + o / 2.0f;
+}
+
+// The lane's taps of one row: the reference's distances and the spectrum
+// value at each distance's bin (bin 0 when the distance is out of range; the
+// caller zeroes those).  The gathers are issued here and not waited for.
+template <int VPT>
+__device__ __forceinline__ void interp_taps(int i, const float *__restrict__ ant,
+                                            const float2 *__restrict__ meas,
+                                            const float *__restrict__ rxo, int R, int S,
+                                            float fft_spacing, float inv_fs, float dmax,
+                                            int is_ti, const float (&vx)[VPT],
+                                            const float (&vy)[VPT], const float (&vz)[VPT],
+                                            float (&d)[VPT], float2 (&m)[VPT]) {
+  const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
+  const int rx = i % R;
+  const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
+  const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
+  const float2 *__restrict__ row = meas + (size_t)i * S;
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    d[k] = interp_dist(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono, is_ti);
+    const bool in = !(d[k] < 0.0f || d[k] > dmax);
+    const int b = interp_bin_fast(d[k], fft_spacing, inv_fs);
+    m[k] = row[in ? b : 0];
+  }
 }
 
 template <int VPT>
@@ -267,6 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
   const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
   float vx[VPT], vy[VPT], vz[VPT];
   load_voxels<VPT>(xl, yl, zl, vidx, nvox, ny, nz, base, vx, vy, vz);
+  settle_voxels<VPT>(vx, vy, vz);
   float Ir[VPT], Ii[VPT], M[VPT][6];
 #pragma unroll
   for (int k = 0; k < VPT; ++k) {
@@ -275,22 +357,19 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
     for (int q = 0; q < 6; ++q) M[k][q] = 0.0f;
   }
   const float dmax = fft_spacing * (float)S;
-  for (int i = 0; i < rows; ++i) {
+  // row i's phasors and sums from its taps
+  auto consume = [&](int i, const float (&d)[VPT], const float2 (&mr)[VPT]) {
     const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
     const int rx = i % R;
     const float ox = rxo[3 * rx + 0], oy = rxo[3 * rx + 1], oz = rxo[3 * rx + 2];
-    const float2 *__restrict__ row = meas + (size_t)i * S;
     const float vax = half_off(px, ox), vay = half_off(py, oy), vaz = half_off(pz, oz);
-    const bool mono = (ox == 0.0f) & (oy == 0.0f) & (oz == 0.0f);
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const float d = interp_dist(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono, is_ti);
-      const bool in = !(d < 0.0f || d > dmax);
-      // out-of-range rows read bin 0 and contribute nothing
-      const float2 mr = row[in ? interp_bin_fast(d, fft_spacing, inv_fs) : 0];
-      const float2 m = in ? mr : make_float2(0.0f, 0.0f);
       float pr, pi;
-      phasor_hw((double)d * inv_lambda, pr, pi);
+      phasor_hw((double)d[k] * inv_lambda, pr, pi);
+      // out-of-range rows read bin 0 and contribute nothing
+      const bool in = !(d[k] < 0.0f || d[k] > dmax);
+      const float2 m = in ? mr[k] : make_float2(0.0f, 0.0f);
       const float cr = m.x * pr - m.y * pi;
       const float ci = m.x * pi + m.y * pr;
       Ir[k] += cr;
@@ -303,6 +382,16 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
       M[k][4] = fmaf(vecz, cr, M[k][4]);
       M[k][5] = fmaf(vecz, ci, M[k][5]);
     }
+  };
+  auto taps = [&](int i, float (&d)[VPT], float2 (&m)[VPT]) {
+    interp_taps<VPT>(i, ant, meas, rxo, R, S, fft_spacing, inv_fs, dmax, is_ti, vx, vy, vz, d,
+                     m);
+  };
+  for (int i = 0; i < rows; ++i) {
+    float d[VPT];
+    float2 mr[VPT];
+    taps(i, d, mr);
+    consume(i, d, mr);
   }
   store_voxels<VPT>(img, mom, nvox, base, Ir, Ii, M);
 }
@@ -827,6 +916,17 @@ extern "C" int mmn_bp_interp(float *img, float *moment, const float *x_locs,
                      1.0 / (double)wavelength, fft_spacing, 1.0f / fft_spacing, ny, nz,
                      num_rows, num_rx, samples, is_ti_radar);
   return mmn_check_launch("mmn_bp_interp");
+}
+
+extern "C" int mmn_sqrt_rn_check(uint64_t first, uint64_t count, unsigned long long *result,
+                                 void *stream) {
+  MMN_REQUIRE(result, MMN_ERR_NULL, "mmn_sqrt_rn_check: NULL result");
+  MMN_REQUIRE(count <= (1ull << 32) && first + count <= (1ull << 32), MMN_ERR_ARG,
+              "mmn_sqrt_rn_check: the range must lie within the 2^32 float bit patterns");
+  if (count == 0) return 0;
+  hipLaunchKernelGGL(k_sqrt_rn_check, dim3(8192), dim3(kBlock), 0, (hipStream_t)stream, result,
+                     first, count);
+  return mmn_check_launch("mmn_sqrt_rn_check");
 }
 
 extern "C" int mmn_cuda_image(float *img, const float *x_locs,

@@ -174,3 +174,22 @@ def test_row_split_with_the_queried_workspace_vs_fp64(oracle):
     assert rel(imgs[0], imgs[2]) <= 1e-6
     # too small a workspace and none both run unsplit: the same bits
     assert np.array_equal(imgs[1], imgs[2])
+
+
+def test_interp_sqrt_equals_sqrtf_on_every_float():
+    """The interpolated branch's distances use a shorter correctly rounded
+    square root (hardware root and a neighbour test); over all 2^32 float bit
+    patterns it returns what sqrtf returns, so the distances, and with them the
+    range bins, stay the reference's (imaging_gpu.cu:83-88)."""
+    from mmnorm_amd import _lib
+    res = torch.tensor([0, -1], dtype=torch.int64, device=dev())
+    _lib.call("mmn_sqrt_rn_check", 0, 1 << 32, res.data_ptr(), _lib.stream_handle(dev()))
+    bad, first = res.cpu().tolist()
+    assert bad == 0, f"{bad} patterns differ from sqrtf, the first 0x{first & 0xffffffff:08x}"
+
+
+def test_sqrt_check_rejects_a_range_past_the_patterns():
+    from mmnorm_amd import _lib
+    res = torch.tensor([0, -1], dtype=torch.int64, device=dev())
+    with pytest.raises(Exception, match="2\\^32"):
+        _lib.call("mmn_sqrt_rn_check", 1, 1 << 32, res.data_ptr(), _lib.stream_handle(dev()))
