@@ -25,6 +25,8 @@
 // Each wave writes its partial sums to the workspace; a second kernel
 // reduces the partials in a fixed order (FP64), so results are bitwise
 // reproducible.
+#include <algorithm>
+
 #include "mmn_internal.h"
 
 namespace {
@@ -46,9 +48,8 @@ constexpr int kMaxK = 32;
 
 int64_t pad_vox(int64_t nvox) { return (nvox + kPad - 1) / kPad * kPad; }
 
-// Voxel chunks (one per wave along the voxel axis) the workspace is sized
-// for: at least ~64 voxels per chunk, at most 1024 chunks and ~1 GiB of
-// partial sums.
+// Most voxel chunks a launch may use: at least ~64 voxels per chunk, at
+// most 1024 chunks and ~1 GiB of partial sums.
 int chunk_max(int A, int F, int64_t nvox) {
   int64_t m = (nvox + 63) / 64;
   const int64_t by_ws = ((int64_t)1 << 30) / ((int64_t)A * F * 8);
@@ -67,9 +68,9 @@ struct BwdWs {
   float2 *coef;   // [A][nfb*K]               position-term coefficients
 };
 
-size_t bwd_layout(int A, int F, int64_t nvox, BwdWs *w, char *base) {
+// Workspace of a launch with `sub` voxel chunks.
+size_t bwd_layout(int A, int F, int64_t nvox, int sub, BwdWs *w, char *base) {
   const int64_t nv = pad_vox(nvox);
-  const int sub = chunk_max(A, F, nvox);
   const int nfbmax = (F + kMinK - 1) / kMinK;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -90,6 +91,43 @@ size_t bwd_layout(int A, int F, int64_t nvox, BwdWs *w, char *base) {
     w->coef = (float2 *)cf;
   }
   return off;
+}
+
+// Voxel chunks of a launch.  A workgroup walks every voxel of its chunk, and
+// the workgroups of one chunk (all rows, all sample-block groups) run side
+// by side, so a chunk whose records (20 bytes a voxel) fit well inside one
+// XCD's 4 MB L2 is read from memory about once per XCD: chunks of at most
+// kChunkRecBytes of records.  Measured at config 2 (1.57 M voxels, 3600
+// rows, 256 samples), both gradients: 2 chunks (31 MB of records between
+// them) fetched 27-30 GB per launch, 8 chunks 2.2 GB, 16 chunks 0.53 GB, 32
+// chunks 0.69 GB, while the time went from 182.7 ms to 179.5-180.0 ms; the
+// partial sums grow by A * F * 8 bytes a chunk (7.4 MB there).  Also at
+// least 8 rounds of resident workgroups (occupancy x CUs), at least 256
+// voxels a chunk and at most chunk_max.  The chunk length is a multiple of
+// kPad; the count it implies never exceeds the planned one.
+constexpr int64_t kChunkRecBytes = (int64_t)2 << 20;
+constexpr int kRecBytes = 20;
+
+struct BwdPlan {
+  int ch = 1;         // chunks
+  int64_t chunk = 0;  // voxels per chunk (multiple of kPad)
+};
+
+BwdPlan plan_bwd(int A, int F, int64_t nvox, int K, int64_t slots, int cap) {
+  const int nfb = (F + K - 1) / K;
+  const int nfbg = (nfb + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t pairs = (int64_t)A * nfbg;
+  const int64_t npad = pad_vox(nvox);
+  int64_t ch = (8 * slots + pairs - 1) / pairs;
+  ch = std::max(ch, (npad * kRecBytes + kChunkRecBytes - 1) / kChunkRecBytes);
+  ch = std::min(ch, (nvox + 255) / 256);
+  ch = std::min(ch, (int64_t)std::min(cap, chunk_max(A, F, nvox)));
+  ch = std::max(ch, (int64_t)1);
+  BwdPlan p;
+  p.chunk = (npad + ch - 1) / ch;
+  p.chunk = (p.chunk + kPad - 1) / kPad * kPad;
+  p.ch = (int)((npad + p.chunk - 1) / p.chunk);
+  return p;
 }
 
 __global__ __launch_bounds__(kBlock) void k_pack(
@@ -295,28 +333,49 @@ __attribute__((amdgpu_waves_per_eu(MEAS ? 4 : 7))) void k_bwd(
       }
     }
   }
+  if constexpr (MEAS) {
+    // Wave sums of the 2K adjoint values v = (Re acc_0..K-1, Im acc_0..K-1),
+    // scattered: lane l ends with the total of v[l] (l < 2K).  Rounds over
+    // partner lanes l ^ 32, l ^ 16, ..., l ^ 1, each adding own + partner as
+    // a full butterfly does (so the totals are the butterfly's, bit for bit);
+    // from o = K down, a lane keeps only the half of its values whose index
+    // has bit o equal to its lane's and sends the other half, so the values a
+    // lane holds halve each round (2K adds in all, not 12K, and no spills).
+    constexpr int NV = 2 * K;
+    float v[NV];
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    if constexpr (MEAS) {
+    for (int t = 0; t < K; ++t) {
+      v[t] = accr[t];
+      v[K + t] = acci[t];
+    }
 #pragma unroll
-      for (int t = 0; t < K; ++t) {
-        accr[t] += __shfl_xor(accr[t], o, kWave);
-        acci[t] += __shfl_xor(acci[t], o, kWave);
+    for (int o = kWave / 2; o >= NV; o >>= 1) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) v[j] += __shfl_xor(v[j], o, kWave);
+    }
+#pragma unroll
+    for (int o = NV / 2; o > 0; o >>= 1) {
+      const bool up = (lane & o) != 0;
+#pragma unroll
+      for (int j = 0; j < o; ++j) {
+        const float send = up ? v[j] : v[j + o];
+        const float keep = up ? v[j + o] : v[j];
+        v[j] = keep + __shfl_xor(send, o, kWave);
       }
     }
-    if constexpr (ANT) {
+    if (live && lane < NV && lane % K < nf)
+      reinterpret_cast<float *>(gs + ((size_t)c * A + a) * F + f_lo)[2 * (lane % K) + lane / K] =
+          v[0];
+  }
+  if constexpr (ANT) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
       gpx += __shfl_xor(gpx, o, kWave);
       gpy += __shfl_xor(gpy, o, kWave);
       gpz += __shfl_xor(gpz, o, kWave);
     }
   }
   if (live) {
-    if constexpr (MEAS) {
-      float2 *__restrict__ out = gs + ((size_t)c * A + a) * F + f_lo;
-#pragma unroll
-      for (int t = 0; t < K; ++t)
-        if (lane == t && t < nf) out[t] = make_float2(accr[t], acci[t]);
-    }
     if (ANT && lane == 0) {
       float *__restrict__ o = gp + (((size_t)c * nfbmax + fb) * A + a) * 3;
       o[0] = gpx;
@@ -351,12 +410,43 @@ __global__ __launch_bounds__(kBlock) void k_reduce_ant(
   grad[t] = (float)(scale * s);
 }
 
+// Resident workgroups of a backward kernel on the current device
+// (occupancy x CUs); without a device (the workspace query may run on a host
+// with no GPU), MI355X's 256 CUs and the kernel's register cap (4 workgroups
+// per CU with the adjoint accumulators, 7 without).
+template <int K, bool MEAS, bool ANT>
+int64_t bwd_slots() {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd<K, MEAS, ANT>, kBlock, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = MEAS ? 4 : 7;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  (void)hipGetLastError();
+  return (int64_t)per_cu * cus;
+}
+
+template <int K>
+int bwd_chunks_any(int A, int F, int64_t nvox) {
+  const int cap = 1 << 30;
+  return std::max({plan_bwd(A, F, nvox, K, bwd_slots<K, true, true>(), cap).ch,
+                   plan_bwd(A, F, nvox, K, bwd_slots<K, true, false>(), cap).ch,
+                   plan_bwd(A, F, nvox, K, bwd_slots<K, false, true>(), cap).ch});
+}
+
 }  // namespace
 
+// Bytes of the largest workspace a launch with these sizes plans (over K = 16
+// and 32 and the three output selections).
 extern "C" size_t mmn_bp_backward_workspace_size(int num_rows, int samples,
                                                  int64_t n_vox) {
   if (num_rows <= 0 || samples <= 0 || n_vox <= 0) return 0;
-  return bwd_layout(num_rows, samples, n_vox, nullptr, nullptr);
+  const int sub = std::max(bwd_chunks_any<16>(num_rows, samples, n_vox),
+                           bwd_chunks_any<32>(num_rows, samples, n_vox));
+  return bwd_layout(num_rows, samples, n_vox, sub, nullptr, nullptr);
 }
 
 extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
@@ -391,11 +481,7 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
       rc = mmn_zero_async(grad_ant, sizeof(float) * 3 * (size_t)A, st, "mmn_bp_backward");
     return rc;
   }
-  const size_t need = bwd_layout(A, F, nvox, nullptr, nullptr);
-  MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
-              "mmn_bp_backward: workspace of %zu bytes needed", need);
-  BwdWs w;
-  bwd_layout(A, F, nvox, &w, (char *)workspace);
+  MMN_REQUIRE(workspace, MMN_ERR_WS, "mmn_bp_backward: NULL workspace");
   const int64_t npad = pad_vox(nvox);
   const int nfb = (F + K - 1) / K;
   const int nfbmax = (F + kMinK - 1) / kMinK;
@@ -419,23 +505,20 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   (void)hipGetLastError();
-  // voxel chunks: enough waves for 8 rounds of resident slots, at least 256
-  // voxels per wave, at most the workspace's chunk count
   const int nfbg = (nfb + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int64_t slots_wg = (int64_t)per_cu * cus;
-  const int64_t pairs = (int64_t)A * nfbg;
-  int64_t ch = (8 * slots_wg + pairs - 1) / pairs;
-  const int64_t by_vox = (nvox + 255) / 256;
-  if (ch > by_vox) ch = by_vox;
-  const int cmax = chunk_max(A, F, nvox);
-  if (ch > cmax) ch = cmax;
-  if (ch < 1) ch = 1;
-  // chunk length a multiple of kPad, then the chunk count it implies (never
-  // more than before, so the workspace still holds the partials)
-  int64_t chunk = (npad + ch - 1) / ch;
-  chunk = (chunk + kPad - 1) / kPad * kPad;
-  ch = (npad + chunk - 1) / chunk;
-  const int64_t nwgs = pairs * ch;
+  // the planned chunks, fewer if the workspace given holds fewer partials
+  const int64_t slots = (int64_t)per_cu * cus;
+  BwdPlan plan = plan_bwd(A, F, nvox, K, slots, 1 << 30);
+  while (plan.ch > 1 && bwd_layout(A, F, nvox, plan.ch, nullptr, nullptr) > workspace_bytes)
+    plan = plan_bwd(A, F, nvox, K, slots, plan.ch - 1);
+  const size_t need = bwd_layout(A, F, nvox, plan.ch, nullptr, nullptr);
+  MMN_REQUIRE(workspace_bytes >= need, MMN_ERR_WS,
+              "mmn_bp_backward: workspace of %zu bytes needed", need);
+  BwdWs w;
+  bwd_layout(A, F, nvox, plan.ch, &w, (char *)workspace);
+  const int ch = plan.ch;
+  const int64_t chunk = plan.chunk;
+  const int64_t nwgs = (int64_t)A * nfbg * ch;
   hipLaunchKernelGGL(k_pack, dim3(mmn_blocks(npad, kBlock)), dim3(kBlock), 0, st, w.rec4,
                      w.recg, (const float2 *)g, x_locs, y_locs, z_locs, vox_idx, nvox, npad,
                      ny, nz);
