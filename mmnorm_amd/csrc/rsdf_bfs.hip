@@ -641,20 +641,27 @@ __global__ __launch_bounds__(kSmallThreads) void k_bfs_small(
     // count
     unsigned won[PER];
     int cnt = 0;
+    // one request per enterable neighbour (a single CU's memory pipeline
+    // bounds a level), all issued before any is compared: the empty asm
+    // takes the six values, so the compiler cannot fold a compare into a
+    // load's branch and wait for that load before issuing the next
+    int32_t k[PER][6];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      // one request per enterable neighbour: a single CU's memory pipeline,
-      // not latency, bounds a level
-      int32_t k[6];
+    for (int q = 0; q < PER; ++q)
 #pragma unroll
       for (int d = 0; d < 6; ++d)
-        k[d] = (m[q] >> d & 1) ? __hip_atomic_load(&at32(claim, u[q] + dir_step(d, ny, nz)),
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : -1;
+        k[q][d] = (m[q] >> d & 1)
+                      ? __hip_atomic_load(&at32(claim, u[q] + dir_step(d, ny, nz)),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      : -1;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      asm volatile("" : "+v"(k[q][0]), "+v"(k[q][1]), "+v"(k[q][2]), "+v"(k[q][3]),
+                   "+v"(k[q][4]), "+v"(k[q][5]));
       unsigned wm = 0;
 #pragma unroll
       for (int d = 0; d < 6; ++d)
-        if ((m[q] >> d & 1) && k[d] == key_of(q0, p0 + q, d)) wm |= 1u << d;
+        if ((m[q] >> d & 1) && k[q][d] == key_of(q0, p0 + q, d)) wm |= 1u << d;
       won[q] = wm;
       cnt += __popc(wm);
     }
