@@ -291,6 +291,45 @@ def test_backward_of_no_voxels_is_zero(entry):
     assert int(torch.count_nonzero(gm).item()) == 0 and int(torch.count_nonzero(ga).item()) == 0
 
 
+def test_backward_workspace_smaller_than_the_plan():
+    """The launch plans enough voxel chunks to keep each chunk's records in
+    an XCD's L2 (here: 64 rows and 32,768 voxels, so many chunks of few
+    voxels); given a quarter of the workspace the size query asks for, it
+    uses fewer chunks and gives the same gradients (the chunk sums add in
+    another grouping: to FP32 rounding); a workspace too small for one chunk
+    is refused."""
+    from mmnorm_amd import _lib
+    sc = small_case(seed=3, F=64, grid=(32, 32, 32))
+    meas, a = T(sc.meas, np.complex64), T(sc.ant, np.float32)
+    xl, yl, zl = (T(v, np.float32) for v in (sc.x_locs, sc.y_locs, sc.z_locs))
+    A, F = meas.shape
+    V = xl.numel() * yl.numel() * zl.numel()
+    rng = np.random.default_rng(4)
+    g = T(rng.normal(size=V) + 1j * rng.normal(size=V), np.complex64)
+    rxo = torch.zeros((1, 3), dtype=torch.float32, device=dev())
+    full = int(_lib.load().mmn_bp_backward_workspace_size(A, F, V))
+    outs = []
+    for ws_bytes in (full, full // 4):
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev())
+        gm = torch.empty((A, F), dtype=torch.complex64, device=dev())
+        ga = torch.empty((A, 3), dtype=torch.float32, device=dev())
+        _lib.call("mmn_bp_backward", gm.data_ptr(), ga.data_ptr(), g.data_ptr(), 1.0 / A,
+                  xl.data_ptr(), yl.data_ptr(), zl.data_ptr(), None, 0, a.data_ptr(),
+                  meas.data_ptr(), rxo.data_ptr(), float(sc.f0), float(sc.df), xl.numel(),
+                  yl.numel(), zl.numel(), A, 1, F, 0, 32, ws.data_ptr(), ws_bytes,
+                  _lib.stream_handle(dev()))
+        outs.append((gm.cpu().numpy(), ga.cpu().numpy()))
+    (m0, a0), (m1, a1) = outs
+    assert np.linalg.norm(m1 - m0) <= 1e-6 * np.linalg.norm(m0)
+    assert np.linalg.norm(a1 - a0) <= 1e-5 * np.linalg.norm(a0)
+    ws = torch.empty(4096, dtype=torch.uint8, device=dev())
+    with pytest.raises(ValueError, match="workspace"):
+        _lib.call("mmn_bp_backward", None, ga.data_ptr(), g.data_ptr(), 1.0, xl.data_ptr(),
+                  yl.data_ptr(), zl.data_ptr(), None, 0, a.data_ptr(), meas.data_ptr(),
+                  rxo.data_ptr(), float(sc.f0), float(sc.df), xl.numel(), yl.numel(),
+                  zl.numel(), A, 1, F, 0, 32, ws.data_ptr(), 4096, _lib.stream_handle(dev()))
+
+
 @pytest.mark.parametrize("variant", ["rec", "profile"])
 def test_position_gradient_config4_geometry_vs_fd(oracle, variant):
     """Config 4 itself (40x40 aperture at 2 mm, 128 samples 77-81 GHz, 96^3
