@@ -797,7 +797,7 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   // thread spreads a level's scattered requests over all 16 waves: on fields
   // of 57,483 / 392,019 reached voxels (64 / 307 levels) PER 1 takes 0.66 /
   // 3.3-3.4 ms, PER 2 0.71 / 3.45-3.6 ms.
-  static const int per = [] {
+  const int per = [] {
     const char *e = getenv("MMN_BFS_PER");
     const int v = e ? atoi(e) : 1;
     return v == 0 || v == 2 || v == 4 || v == 8 ? v : 1;
@@ -807,7 +807,7 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   // (about 1.5 ns per scattered request, some 14 requests per entry), the
   // multi-block kernels' by two launches (about 10 us a level): on a
   // 1.5M-voxel field caps of 512 and 1024 give the same time.
-  static const int cap_env = [] {
+  const int cap_env = [] {
     const char *e = getenv("MMN_BFS_CAP");
     return e ? atoi(e) : 1024;
   }();
@@ -817,7 +817,7 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   // 6-10 empty levels (an empty k_level returns at once, ~3 us): with k_level
   // on the 1.5M-voxel field 32 / 64 / 128 levels per batch take 1.02 / 1.00 /
   // 1.20 ms, on the 8.4M-voxel one 2.89 / 2.92 / 2.85 ms (256: 2.67).
-  static const int batch = [] {
+  const int batch = [] {
     const char *e = getenv("MMN_BFS_BATCH");
     const int v = e ? atoi(e) : 64;
     return v < 1 ? 1 : v > 256 ? 256 : v;
@@ -826,20 +826,20 @@ extern "C" int mmn_rsdf_bfs(double *rsdf, int32_t *hops, const double *normals,
   // 1024; default 256).  A level's frontier has a few thousand entries; with
   // 256 or 1024 blocks a level's kernels take the same time (4.7 + 5.7 us on
   // the 1.5M-voxel field), and 256 reads a quarter of the counts.
-  static const int lgrid = [] {
+  const int lgrid = [] {
     const char *e = getenv("MMN_BFS_GRID");
     const int v = e ? atoi(e) : 256;
     return v == 512 || v == kGrid ? v : 256;
   }();
   // One launch per multi-block level (k_level, MMN_BFS_FUSED=1, the default)
   // or two (k_count, k_emit_claim; MMN_BFS_FUSED=0, kept for measurement).
-  static const bool fused = [] {
+  const bool fused = [] {
     const char *e = getenv("MMN_BFS_FUSED");
     return !(e && atoi(e) == 0);
   }();
   // Blocks of k_level (MMN_BFS_FGRID, 16-1024, default 64): the look-back
   // covers 64 blocks per round trip.
-  static const int fgrid = [] {
+  const int fgrid = [] {
     const char *e = getenv("MMN_BFS_FGRID");
     const int v = e ? atoi(e) : 64;
     return v < 16 ? 16 : v > kGrid ? kGrid : v;

@@ -108,6 +108,27 @@ def test_every_row_split_agrees(scene, oracle, rs):
     assert rel(img.cpu().numpy(), want) <= 1e-5
 
 
+@pytest.mark.parametrize("vpt,split,K", [(1, 1, 64), (1, 2, 16), (4, 1, 64), (2, 2, 32),
+                                         (1, 1, 256), (4, 1, 128)])
+def test_lane_shapes_agree(scene, oracle, vpt, split, K):
+    """The recurrence's other lane shapes (voxels per lane, even/odd Horner
+    chains; MMN_REC_VPT / MMN_REC_SPLIT) on the ragged bistatic TI inputs,
+    with and without the moment, against the FP64 oracle."""
+    sc, rows, meas, rxo = scene
+    os.environ["MMN_REC_VPT"], os.environ["MMN_REC_SPLIT"] = str(vpt), str(split)
+    try:
+        img, mom = ops.backproject(T(meas, np.complex64), T(rows, np.float32),
+                                   T(sc.x_locs, np.float32), T(sc.y_locs, np.float32),
+                                   T(sc.z_locs, np.float32), T(rxo, np.float32), f0=sc.f0,
+                                   df=sc.df, K=K, moment=True)
+    finally:
+        del os.environ["MMN_REC_VPT"], os.environ["MMN_REC_SPLIT"]
+    want, wmom = oracle.bp_f64(sc.x_locs, sc.y_locs, sc.z_locs, rows, meas, rxo, sc.f0, sc.df,
+                               moment=True)
+    assert rel(img.cpu().numpy(), want) <= 1e-5
+    assert rel(mom.cpu().numpy().reshape(-1, 3), wmom) <= 1e-5
+
+
 def test_interp_bins_at_bin_boundaries(oracle):
     """Voxels placed where d / fft_spacing / 2 lands on or next to an integer,
     one antenna, a spectrum whose bins all differ: the MI355X interpolated

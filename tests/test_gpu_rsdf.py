@@ -66,6 +66,41 @@ def test_bfs_bit_exact_vs_fifo(shape, density, seed, blobs, oracle):
     np.testing.assert_array_equal(hops, whops)
 
 
+@pytest.mark.parametrize("env", [
+    {"MMN_BFS_FUSED": "0"},
+    {"MMN_BFS_FUSED": "0", "MMN_BFS_GRID": "1024"},
+    {"MMN_BFS_PER": "2"},
+    {"MMN_BFS_PER": "4", "MMN_BFS_CAP": "4096"},
+    {"MMN_BFS_PER": "8", "MMN_BFS_CAP": "8192"},
+    {"MMN_BFS_PER": "0"},
+    {"MMN_BFS_CAP": "16", "MMN_BFS_BATCH": "1"},
+    {"MMN_BFS_FGRID": "16"},
+    {"MMN_BFS_FGRID": "1024", "MMN_BFS_BATCH": "256"},
+], ids=lambda e: ",".join(f"{k[8:]}={v}" for k, v in e.items()))
+def test_bfs_launch_shapes_bit_exact(env, oracle):
+    """The BFS's other launch shapes (the two-kernel levels, positions per
+    thread of the single-workgroup loop, its frontier cap, levels per batch,
+    grid sizes) on the single-seed field whose frontier peaks at 9,225
+    entries, so that both level forms run and hand over both ways: bit-exact
+    against the C FIFO restatement."""
+    import os
+    shape = (96, 96, 64)
+    n, labeled, nlab = random_field(shape, 0.9, 5, False)
+    coords, xl = coords_of(shape, 1.5e-3)
+    sizes = ndimage.sum(np.ones(shape), labeled, np.arange(2, nlab + 2))
+    valid = np.arange(2, nlab + 2)[sizes >= 2]
+    seeds = oracle.rsdf_seeds(n, coords, labeled, valid)
+    want, whops = oracle.bfs_fifo(n, labeled, seeds, xl[1] - xl[0])
+    os.environ.update(env)
+    try:
+        got, hops = msdf.rsdf_components(n, labeled, seeds, xl[1] - xl[0], return_hops=True)
+    finally:
+        for k in env:
+            del os.environ[k]
+    assert got.view(np.uint64).tobytes() == want.view(np.uint64).tobytes()
+    np.testing.assert_array_equal(hops, whops)
+
+
 def test_bfs_matches_python_fifo_small(oracle):
     n, labeled, nlab = random_field((7, 6, 5), 0.55, 9, False)
     coords, xl = coords_of((7, 6, 5), 1e-3)
