@@ -188,7 +188,8 @@ int mmn_bp_profile_backward(float *grad_meas, float *grad_ant, const float *g, d
  * in MI355X form, plus the vote moment.  Per (voxel, row): the reference's
  * FP32 distance and bin floor(d / fft_spacing / 2) with the same operations
  * (so the same bins; rows with d > fft_spacing * samples are skipped), the
- * phasor exp(-j 2 pi d / wavelength) from an FP64-reduced phase and the
+ * phasor exp(-j 2 pi d / wavelength) from the phase fraction formed with
+ * 1/wavelength split into two floats (exact products by FMA) and the
  * hardware sine/cosine (the reference: FP64 cos/sin), and, if moment !=
  * NULL, moment[v][k] = sum_i (p_i + o_i/2 - v)_k * c_iv so that the vote of
  * surface_normal_est.cu:110-149 is mmn_sn_from_moment(img, moment).  meas

@@ -312,6 +312,21 @@ __device__ __forceinline__ float half_off(float p, float o) {
   return p + o / 2.0f;
 }
 
+// exp(-j 2 pi d / lambda) for an FP32 distance d, with 1/lambda split into
+// two floats hi + lo (hi = RN(1/lambda), lo = RN(1/lambda - hi)): d * hi =
+// p + e exactly (e from an FMA), so the phase's fraction
+// (p - rint(p)) + (e + d * lo) is formed in FP32 to about an ulp of the
+// fraction (p - rint(p) is exact), as the FP64 product and reduction gives
+// it, and goes to the hardware sine and cosine (which take revolutions).
+__device__ __forceinline__ void phasor_split(float d, float hi, float lo, float &re, float &im) {
+#pragma clang fp contract(off)
+  const float p = d * hi;
+  const float e = fmaf(d, hi, -p);
+  const float fr = (p - rintf(p)) + fmaf(d, lo, e);
+  re = __builtin_amdgcn_cosf(fr);
+  im = -__builtin_amdgcn_sinf(fr);
+}
+
 // The lane's taps of one row: the reference's distances and the spectrum
 // value at each distance's bin (bin 0 when the distance is out of range; the
 // caller zeroes those).  The gathers are issued here and not waited for.
@@ -333,7 +348,9 @@ __device__ __forceinline__ void interp_taps(int i, const float *__restrict__ ant
     d[k] = interp_dist(vx[k], vy[k], vz[k], px, py, pz, ox, oy, oz, mono, is_ti);
     const bool in = !(d[k] < 0.0f || d[k] > dmax);
     const int b = interp_bin_fast(d[k], fft_spacing, inv_fs);
-    m[k] = row[in ? b : 0];
+    // a 32-bit byte offset from the row (bins < S <= 2^28): the load takes
+    // the row's address from scalar registers and one VGPR of offset
+    m[k] = *(const float2 *)((const char *)row + (uint32_t)(in ? b : 0) * 8u);
   }
 }
 
@@ -343,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
     const float *__restrict__ xl, const float *__restrict__ yl,
     const float *__restrict__ zl, const int32_t *__restrict__ vidx,
     int64_t nvox, const float *__restrict__ ant,
-    const float2 *__restrict__ meas, const float *__restrict__ rxo, double inv_lambda,
+    const float2 *__restrict__ meas, const float *__restrict__ rxo, float il_hi, float il_lo,
     float fft_spacing, float inv_fs, int ny, int nz, int rows, int R, int S, int is_ti) {
   const int64_t base = (int64_t)blockIdx.x * (kBlock * VPT) + threadIdx.x;
   float vx[VPT], vy[VPT], vz[VPT];
@@ -357,7 +374,10 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
     for (int q = 0; q < 6; ++q) M[k][q] = 0.0f;
   }
   const float dmax = fft_spacing * (float)S;
-  // row i's phasors and sums from its taps
+  // row i's phasors and sums from its taps.  The moment is accumulated as
+  // sum_i a_i c_i with the row's vote point a_i = p_i + o_i/2 (wave-uniform,
+  // a scalar operand of the FMAs); v * sum_i c_i = v * I is taken off once
+  // per voxel at the end, which gives sum_i (a_i - v) c_i.
   auto consume = [&](int i, const float (&d)[VPT], const float2 (&mr)[VPT]) {
     const float px = ant[3 * i + 0], py = ant[3 * i + 1], pz = ant[3 * i + 2];
     const int rx = i % R;
@@ -366,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       float pr, pi;
-      phasor_hw((double)d[k] * inv_lambda, pr, pi);
+      phasor_split(d[k], il_hi, il_lo, pr, pi);
       // out-of-range rows read bin 0 and contribute nothing
       const bool in = !(d[k] < 0.0f || d[k] > dmax);
       const float2 m = in ? mr[k] : make_float2(0.0f, 0.0f);
@@ -374,13 +394,12 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
       const float ci = m.x * pi + m.y * pr;
       Ir[k] += cr;
       Ii[k] += ci;
-      const float vecx = vax - vx[k], vecy = vay - vy[k], vecz = vaz - vz[k];
-      M[k][0] = fmaf(vecx, cr, M[k][0]);
-      M[k][1] = fmaf(vecx, ci, M[k][1]);
-      M[k][2] = fmaf(vecy, cr, M[k][2]);
-      M[k][3] = fmaf(vecy, ci, M[k][3]);
-      M[k][4] = fmaf(vecz, cr, M[k][4]);
-      M[k][5] = fmaf(vecz, ci, M[k][5]);
+      M[k][0] = fmaf(vax, cr, M[k][0]);
+      M[k][1] = fmaf(vax, ci, M[k][1]);
+      M[k][2] = fmaf(vay, cr, M[k][2]);
+      M[k][3] = fmaf(vay, ci, M[k][3]);
+      M[k][4] = fmaf(vaz, cr, M[k][4]);
+      M[k][5] = fmaf(vaz, ci, M[k][5]);
     }
   };
   auto taps = [&](int i, float (&d)[VPT], float2 (&m)[VPT]) {
@@ -392,6 +411,15 @@ __global__ __launch_bounds__(kBlock) void k_bp_interp(
     float2 mr[VPT];
     taps(i, d, mr);
     consume(i, d, mr);
+  }
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    M[k][0] = fmaf(-vx[k], Ir[k], M[k][0]);
+    M[k][1] = fmaf(-vx[k], Ii[k], M[k][1]);
+    M[k][2] = fmaf(-vy[k], Ir[k], M[k][2]);
+    M[k][3] = fmaf(-vy[k], Ii[k], M[k][3]);
+    M[k][4] = fmaf(-vz[k], Ir[k], M[k][4]);
+    M[k][5] = fmaf(-vz[k], Ii[k], M[k][5]);
   }
   store_voxels<VPT>(img, mom, nvox, base, Ir, Ii, M);
 }
@@ -910,11 +938,14 @@ extern "C" int mmn_bp_interp(float *img, float *moment, const float *x_locs,
     nvox = n_vox;
   }
   if (nvox == 0) return 0;
+  MMN_REQUIRE(samples <= (1 << 28), MMN_ERR_ARG, "mmn_bp_interp: more than 2^28 samples a row");
+  const double il = 1.0 / (double)wavelength;
+  const float il_hi = (float)il, il_lo = (float)(il - (double)il_hi);
   hipLaunchKernelGGL(k_bp_interp<2>, dim3(mmn_blocks(nvox, kBlock * 2)), dim3(kBlock), 0,
                      (hipStream_t)stream, (float2 *)img, (float2 *)moment, x_locs, y_locs,
-                     z_locs, vox_idx, nvox, ant, (const float2 *)meas, rx_off,
-                     1.0 / (double)wavelength, fft_spacing, 1.0f / fft_spacing, ny, nz,
-                     num_rows, num_rx, samples, is_ti_radar);
+                     z_locs, vox_idx, nvox, ant, (const float2 *)meas, rx_off, il_hi, il_lo,
+                     fft_spacing, 1.0f / fft_spacing, ny, nz, num_rows, num_rx, samples,
+                     is_ti_radar);
   return mmn_check_launch("mmn_bp_interp");
 }
 
