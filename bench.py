@@ -13,7 +13,7 @@ ONE JSON line:
              launch's mean duration from HIP events on the launch stream,
              against the MI355X FP32 vector peak (157.3 TFLOP/s: one wave64
              v_fma_f32 per SIMD every 2 cycles at 2.4 GHz), and against the
-             126.7 TFLOP/s an FMA-only microbenchmark reached on MI355X (a
+             139 TFLOP/s a Horner-shaped FMA stream reached on MI355X (a
              constant, not measured by this run); traffic from the committed
              rocprofv3 --pmc summary when present (its source is named)
 * accuracy   the timed output against the FP64 oracle on a seeded voxel
@@ -61,11 +61,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32 = 157.3e12
-# FMA-only microbenchmark on MI355X: 16 independent non-packed v_fma_f32
-# chains per lane, every wave slot filled; a constant here, not re-measured.
-FMA_CEILING_MEASURED = 126.7e12
-FMA_CEILING_SOURCE = ("FMA-only microbenchmark on MI355X (16 independent v_fma_f32 chains "
-                      "per lane), measured once, not by this run")
+# FMA-only microbenchmark on MI355X with the recurrence's own operand pattern
+# (complex Horner steps of two voxels per lane, half the FMAs taking a sample
+# from an SGPR, non-packed v_fma_f32, 8 waves per SIMD): 134-139.5 TFLOP/s at
+# a 2.35-2.38 GHz shader clock; a constant here, not re-measured.
+FMA_CEILING_MEASURED = 139.0e12
+FMA_CEILING_SOURCE = ("Horner-shaped FMA microbenchmark on MI355X (two complex chains per "
+                      "lane, half the FMAs with an SGPR operand): 134-139.5 TFLOP/s at "
+                      "2.35-2.38 GHz, measured once, not by this run")
 
 
 def flops_fwd(V, A, F):
