@@ -13,8 +13,9 @@ ONE JSON line:
              launch's mean duration from HIP events on the launch stream,
              against the MI355X FP32 vector peak (157.3 TFLOP/s: one wave64
              v_fma_f32 per SIMD every 2 cycles at 2.4 GHz), and against the
-             139 TFLOP/s a Horner-shaped FMA stream reached on MI355X (a
-             constant, not measured by this run); traffic from the committed
+             119 TFLOP/s a Horner-shaped FMA stream sustains on MI355X with
+             bounded data (power-limited to a 2.0 GHz clock; a constant, not
+             measured by this run); traffic from the committed
              rocprofv3 --pmc summary when present (its source is named)
 * accuracy   the timed output against the FP64 oracle on a seeded voxel
              sample that includes the strongest voxels (outside the timed
@@ -63,12 +64,14 @@ sys.path.insert(0, ROOT)
 PEAK_FP32 = 157.3e12
 # FMA-only microbenchmark on MI355X with the recurrence's own operand pattern
 # (complex Horner steps of two voxels per lane, half the FMAs taking a sample
-# from an SGPR, non-packed v_fma_f32, 8 waves per SIMD): 134-139.5 TFLOP/s at
-# a 2.35-2.38 GHz shader clock; a constant here, not re-measured.
-FMA_CEILING_MEASURED = 139.0e12
+# from an SGPR, non-packed v_fma_f32, 8 waves per SIMD) and bounded data (the
+# step on the unit circle): 117.9-119.2 TFLOP/s sustained over 120 ms, the
+# shader clock power-limited to 1.99-2.00 GHz; a constant, not re-measured.
+FMA_CEILING_MEASURED = 119.0e12
 FMA_CEILING_SOURCE = ("Horner-shaped FMA microbenchmark on MI355X (two complex chains per "
-                      "lane, half the FMAs with an SGPR operand): 134-139.5 TFLOP/s at "
-                      "2.35-2.38 GHz, measured once, not by this run")
+                      "lane, half the FMAs with an SGPR operand, bounded data): 117.9-119.2 "
+                      "TFLOP/s sustained at a power-limited 2.0 GHz, measured once, not by "
+                      "this run")
 
 
 def flops_fwd(V, A, F):
