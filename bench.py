@@ -374,7 +374,7 @@ def main(argv=None):
         # the position gradient alone (what autograd asks for when the
         # measurements take no gradient, as in the pipeline's steps)
         _, pk2, _ = timed(lambda: ops.backproject_backward(
-            g, meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, scale=1.0 / A, K=32,
+            g, meas, ant, xl, yl, zl, f0=sc.f0, df=sc.df, scale=1.0 / A,
             need_meas=False), 2, 1)
         rec["backward"]["ant_only_ms"] = float(np.mean(pk2))
         log(f"backward: {b_s * 1e3:.2f} ms (position gradient alone "

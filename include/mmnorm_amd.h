@@ -235,8 +235,11 @@ int mmn_sn_from_moment(float *normals, const float *img, const float *moment,
  * with Q_iv = sum_j (2 pi f_j / c) meas[i][j] exp(-j 2 pi f_j d_iv / c).
  * Either output may be NULL (not computed).  No reference counterpart:
  * the reference computes no gradients.  Deterministic: partial sums go to
- * the workspace and are reduced in a fixed order.  K (16 or 32; 0 = 32)
- * is the sample block of one wave and its re-anchor interval.
+ * the workspace and are reduced in a fixed order.  K (16 or 32, or 64 with
+ * grad_meas NULL; 0 = 32) is the sample block of one wave and its re-anchor
+ * interval: longer blocks cost less set-up and accumulate more rounding
+ * (dL/dant within ~3e-6 of FP64 at K = 32, ~2e-5 at K = 64, on bistatic test
+ * cases).
  */
 size_t mmn_bp_backward_workspace_size(int num_rows, int samples, int64_t n_vox);
 int mmn_bp_backward(float *grad_meas, float *grad_ant, const float *g,

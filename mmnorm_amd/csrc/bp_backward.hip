@@ -39,7 +39,7 @@ constexpr int kStep = kBlock;          // voxels per workgroup step (one per thr
 constexpr int kPass = kWave * kVV;     // voxels of one wave pass over a step
 constexpr int kPad = kStep;            // voxel records are padded to this
 constexpr int kMinK = 16;              // smallest sample block (workspace sizing)
-constexpr int kMaxK = 32;
+constexpr int kMaxK = 64;              // the position gradient alone (no accumulators)
 
 // Per-voxel record: position and Re(g) in one 16-byte array element (rec4),
 // Im(g) in a second array (recg), 20 bytes per voxel in all.  Records past
@@ -437,15 +437,21 @@ int bwd_chunks_any(int A, int F, int64_t nvox) {
                    plan_bwd(A, F, nvox, K, bwd_slots<K, false, true>(), cap).ch});
 }
 
+int bwd_chunks_ant64(int A, int F, int64_t nvox) {
+  return plan_bwd(A, F, nvox, 64, bwd_slots<64, false, true>(), 1 << 30).ch;
+}
+
 }  // namespace
 
 // Bytes of the largest workspace a launch with these sizes plans (over K = 16
-// and 32 and the three output selections).
+// and 32 and the three output selections, and K = 64 for the position
+// gradient alone).
 extern "C" size_t mmn_bp_backward_workspace_size(int num_rows, int samples,
                                                  int64_t n_vox) {
   if (num_rows <= 0 || samples <= 0 || n_vox <= 0) return 0;
-  const int sub = std::max(bwd_chunks_any<16>(num_rows, samples, n_vox),
-                           bwd_chunks_any<32>(num_rows, samples, n_vox));
+  const int sub = std::max({bwd_chunks_any<16>(num_rows, samples, n_vox),
+                            bwd_chunks_any<32>(num_rows, samples, n_vox),
+                            bwd_chunks_ant64(num_rows, samples, n_vox)});
   return bwd_layout(num_rows, samples, n_vox, sub, nullptr, nullptr);
 }
 
@@ -465,8 +471,9 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
                   samples > 0,
               MMN_ERR_ARG, "mmn_bp_backward: sizes must be positive");
   if (K == 0) K = 32;
-  MMN_REQUIRE(K == 16 || K == 32, MMN_ERR_ARG,
-              "mmn_bp_backward: K must be 16 or 32 (got %d)", K);
+  MMN_REQUIRE(K == 16 || K == 32 || (K == 64 && !grad_meas), MMN_ERR_ARG,
+              "mmn_bp_backward: K must be 16 or 32, or 64 for the position gradient alone "
+              "(got %d)", K);
   const int64_t nvox = vox_idx ? n_vox : (int64_t)nx * ny * nz;
   MMN_REQUIRE(nvox >= 0, MMN_ERR_ARG, "mmn_bp_backward: n_vox < 0");
   MMN_REQUIRE(pad_vox(nvox) <= INT32_MAX, MMN_ERR_ARG,
@@ -492,7 +499,9 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
                          int, int, int, int, int, int, int);
   const bool gm_on = grad_meas != nullptr, ga_on = grad_ant != nullptr;
   KernT kern;
-  if (K == 16)
+  if (K == 64)
+    kern = k_bwd<64, false, true>;
+  else if (K == 16)
     kern = gm_on && ga_on ? k_bwd<16, true, true> : gm_on ? k_bwd<16, true, false>
                                                             : k_bwd<16, false, true>;
   else

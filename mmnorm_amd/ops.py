@@ -226,9 +226,13 @@ def sn_from_moment(img, moment, vox_idx):
 
 
 def backproject_backward(g, meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f0, df,
-                         scale=1.0, K=32, vox_idx=None, is_ti=False, need_meas=True,
+                         scale=1.0, K=None, vox_idx=None, is_ti=False, need_meas=True,
                          need_ant=True, variant="rec", bbox=None):
     """Adjoint of ``scale * backproject``: (dL/dmeas complex64 (rows,F), dL/dant float32 (rows,3)).
+
+    ``K`` is the recurrence backward's sample block: 16 or 32, or 64 for the
+    position gradient alone; by default 64 when only dL/dant is asked for
+    (no adjoint accumulators, so the longer block pays) and 32 otherwise.
 
     ``variant="rec"`` recomputes every (row, sample, voxel) phase
     (mmn_bp_backward); ``variant="profile"`` evaluates the same gradients by
@@ -245,6 +249,8 @@ def backproject_backward(g, meas, ant, x_locs, y_locs, z_locs, rx_off=None, *, f
         raise ValueError(f"grad has {g.numel()} entries, expected {n}")
     dev = meas.device
     A, F = meas.shape
+    if K is None:
+        K = 64 if (need_ant and not need_meas) else 32
     gm = torch.empty((A, F), dtype=torch.complex64, device=dev) if need_meas else None
     ga = torch.empty((A, 3), dtype=torch.float32, device=dev) if need_ant else None
     lib = _lib.load()
@@ -291,7 +297,9 @@ class Backproject(torch.autograd.Function):
         img = backproject(meas, ant, x_locs, y_locs, z_locs, rx_off, f0=f0, df=df,
                           variant=variant, K=K, vox_idx=vox_idx, is_ti=is_ti, bbox=bbox)
         ctx.save_for_backward(meas, ant, x_locs, y_locs, z_locs, rx_off, vox_idx)
-        ctx.args = (f0, df, scale, 32 if K >= 32 else 16, is_ti, variant, bbox)
+        # the backward's sample block: its default for the outputs asked for
+        # (64 for the position gradient alone, else 32), 16 after a short forward block
+        ctx.args = (f0, df, scale, None if K >= 32 else 16, is_ti, variant, bbox)
         return img * scale
 
     @staticmethod

@@ -291,6 +291,48 @@ def test_backward_of_no_voxels_is_zero(entry):
     assert int(torch.count_nonzero(gm).item()) == 0 and int(torch.count_nonzero(ga).item()) == 0
 
 
+@pytest.mark.parametrize("n_list", [77, None])
+def test_position_gradient_with_64_sample_blocks(n_list):
+    """The position-gradient kernel with 64-sample blocks (the default when
+    only dL/dant is asked for; F = 150 leaves a short last block), and with 32
+    and 16, against the FP64 NumPy backward on the bistatic TI case; K = 64
+    with the adjoint is refused.  The blocks' real-coefficient phasor
+    recurrence accumulates rounding over a block, so its error grows with K."""
+    sc = small_case(seed=8, n_ap=6, F=150, grid=(9, 7, 5))
+    rng = np.random.default_rng(13)
+    rxo = np.array([[0.0, 0.0, 0.0], [3e-3, -2e-3, 0.0]])
+    meas = np.repeat(sc.meas, 2, axis=0)[: 2 * len(sc.ant)]
+    ant = np.repeat(sc.ant, 2, axis=0).astype(np.float32).astype(np.float64)
+    X, Y, Z = np.meshgrid(sc.x_locs.astype(np.float32).astype(np.float64),
+                          sc.y_locs.astype(np.float32).astype(np.float64),
+                          sc.z_locs.astype(np.float32).astype(np.float64), indexing="ij")
+    vall = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    if n_list is None:
+        idx, v = None, vall
+    else:
+        sel = np.sort(rng.choice(len(vall), size=n_list, replace=False)).astype(np.int32)
+        idx, v = T(sel, np.int32), vall[sel]
+    g = rng.normal(size=len(v)) + 1j * rng.normal(size=len(v))
+    scale = 1.0 / len(ant)
+    xl, yl, zl = (T(q, np.float32) for q in (sc.x_locs, sc.y_locs, sc.z_locs))
+    args = (T(g, np.complex64), T(meas, np.complex64), T(ant, np.float32), xl, yl, zl,
+            T(rxo, np.float32))
+    kw = dict(f0=sc.f0, df=sc.df, scale=scale, vox_idx=idx, is_ti=True)
+    _, want = _backward64(g, meas.astype(np.complex64).astype(np.complex128), ant,
+                          rxo.astype(np.float32).astype(np.float64), v, sc.f0, sc.df, scale,
+                          True)
+    err = {}
+    for K in (16, 32, None):
+        none_m, ga = ops.backproject_backward(*args, K=K, need_meas=False, **kw)
+        assert none_m is None
+        got = ga.cpu().numpy().astype(np.float64)
+        err[K or 64] = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    print("position gradient rel. L2 error vs FP64 by block length:", err)
+    assert all(e <= 1e-4 for e in err.values()), err
+    with pytest.raises(ValueError, match="64"):
+        ops.backproject_backward(*args, K=64, **kw)
+
+
 def test_backward_workspace_smaller_than_the_plan():
     """The launch plans enough voxel chunks to keep each chunk's records in
     an XCD's L2 (here: 64 rows and 32,768 voxels, so many chunks of few
