@@ -82,7 +82,8 @@ def host_bbox(x_locs, y_locs, z_locs):
 def _run_image(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offset, slope,
                wavelength, bandwidth, num_samples, use_4_rx, is_ti_radar, interpolated,
                variant, K):
-    """py_image_gpu.py:237-305 on one device: returns (image / N, N, rows' TX positions)."""
+    """py_image_gpu.py:237-300 on one device: returns (the kernel's image on the
+    device, N, rows' TX positions); _reference_scaled applies :301-302."""
     norm_factor = np.asarray(antenna_locs).shape[0]
     samples_per_meas = np.asarray(radar_data).shape[1]
     rx, meas, ant_rows = _prep_inputs(radar_data, rx_offset, antenna_locs, use_4_rx)
@@ -108,11 +109,42 @@ def _run_image(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offset, slop
         bbox = host_bbox(x_locs, y_locs, z_locs) if variant == "profile" else None
         raw = ops.backproject(meas_t, ant_t, xl, yl, zl, rx_t, f0=f0, df=df,
                               variant=variant, K=K, is_ti=is_ti_radar, bbox=bbox)
-    raw = raw.cpu().numpy()
-    p_xyz = raw.real + 1j * raw.imag  # complex64, as p_xyz_r + 1j*p_xyz_i
+    return raw, norm_factor, ant_rows
+
+
+_FLT_MIN = float(np.finfo(np.float32).tiny)
+
+
+def _reference_scaled(raw, norm_factor, divisions):
+    """The reference's host arithmetic on the kernel's output, bit for bit:
+    ``p_xyz = p_xyz_r + 1j*p_xyz_i`` (complex64), then ``p_xyz /= norm_factor``
+    ``divisions`` times (py_image_gpu.py:301-302, and :330 for the full
+    branch).  Returns a complex64 NumPy array.
+
+    NumPy divides a complex64 value by a real n as (re + im*rat)*scl and
+    (im - re*rat)*scl with rat = 0/n and scl = float32(1)/float32(n); the
+    ``+ 1j*`` step adds signed zeros.  When every component is finite and
+    nonzero and stays a normal float through the divisions, both steps reduce
+    to one float32 multiply by scl per component, which the device does
+    before the copy to the host (one min/max reduction decides).  Any other
+    image (zeros, NaN, infinities, values near the subnormal range) takes
+    NumPy's own complex64 steps on the host."""
+    n = int(norm_factor)
+    scl = np.float32(1.0) / np.float32(n) if n >= 1 else None
+    f = torch.view_as_real(raw)
+    if scl is not None and f.numel():
+        lo, hi = torch.aminmax(f.abs())
+        lo, hi = float(lo), float(hi)
+        if math.isfinite(hi) and lo * float(scl) ** divisions >= 2.0 * _FLT_MIN:
+            for _ in range(divisions):
+                f.mul_(float(scl))
+            return raw.cpu().numpy()
+    host = raw.cpu().numpy()
+    p_xyz = host.real + 1j * host.imag
     p_xyz = p_xyz.astype(np.complex64, copy=False)
-    p_xyz /= norm_factor
-    return p_xyz, norm_factor, ant_rows
+    for _ in range(divisions):
+        p_xyz /= norm_factor
+    return p_xyz
 
 
 def _phase_centers(locs_rep, rx_offsets):
@@ -130,10 +162,10 @@ def image_cuda(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offsets, slo
     if variant not in ("rec", "profile", "plain", "ref"):
         raise ValueError("variant must be 'rec', 'profile', 'plain' or 'ref'")
     if not use_interpolated_processing:
-        sum_image, norm_factor, locs_rep = _run_image(
+        raw, norm_factor, locs_rep = _run_image(
             x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offsets, slope,
             wavelength, bandwidth, num_samples, use_4_rx, is_ti_radar, False, variant, K)
-        sum_image /= norm_factor
+        sum_image = _reference_scaled(raw, norm_factor, 2)
         center_locs = _phase_centers(locs_rep, rx_offsets)
     else:
         num_rx = len(rx_offsets)
@@ -143,10 +175,11 @@ def image_cuda(x_locs, y_locs, z_locs, antenna_locs, radar_data, rx_offsets, slo
         sum_image, centers = None, []
         for i in range(groups):
             lo, hi = i * MAX_GROUP_ROWS // num_rx, (i + 1) * MAX_GROUP_ROWS // num_rx
-            img, _, locs_rep = _run_image(
+            raw, norm_factor, locs_rep = _run_image(
                 x_locs, y_locs, z_locs, antenna_locs[lo:hi], radar_data[lo:hi], rx_offsets,
                 slope, wavelength, bandwidth, num_samples, use_4_rx, is_ti_radar, True,
                 variant, K)
+            img = _reference_scaled(raw, norm_factor, 1)
             if return_phase_centers:
                 centers.append(_phase_centers(locs_rep, rx_offsets))
             sum_image = img if sum_image is None else sum_image + img
