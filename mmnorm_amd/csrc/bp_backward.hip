@@ -528,6 +528,9 @@ extern "C" int mmn_bp_backward(float *grad_meas, float *grad_ant,
   const int ch = plan.ch;
   const int64_t chunk = plan.chunk;
   const int64_t nwgs = (int64_t)A * nfbg * ch;
+  MMN_REQUIRE(nwgs * kBlock <= (int64_t)UINT32_MAX, MMN_ERR_ARG,
+              "mmn_bp_backward: %lld workgroups of %d lanes exceed one launch (2^32 lanes); "
+              "split the voxels or rows over calls", (long long)nwgs, kBlock);
   hipLaunchKernelGGL(k_pack, dim3(mmn_blocks(npad, kBlock)), dim3(kBlock), 0, st, w.rec4,
                      w.recg, (const float2 *)g, x_locs, y_locs, z_locs, vox_idx, nvox, npad,
                      ny, nz);
