@@ -377,6 +377,32 @@ int mmn_simulate_channel(float *channel, const float *grid, float tau, int width
                          void *workspace, size_t workspace_bytes, void *stream);
 
 /*
+ * mmn_simulate_channels -- n_sim channels of mmn_simulate_channel in one
+ * launch: the search loop of isosurface_optimization.py:475-521 simulates
+ * every (offset, antenna, component) of the grid search with one call each
+ * (:111-209); here a batch shares one launch per kernel, each simulation
+ * with its own antenna, direction, offset and component, and each channel
+ * equals mmn_simulate_channel's for the same inputs bit for bit (the same
+ * per-ray arithmetic, chunks and reduction order).  grid: n_comp components
+ * of NX*NY*NZ floats each, one after another (device); comp[s] selects
+ * simulation s's component and must lie in [0, n_comp).  Host arrays:
+ * comp[n_sim], taus[n_sim], eyes[3 n_sim], rx_dirs[3 n_sim],
+ * rx_locs[3 n_sim] (double), bounding_box[6], grid_res[3]; they are copied
+ * before the call returns.  channels: complex [n_sim][samples] (device).
+ * n_sim <= 65535.  Workspace from mmn_simulate_channels_workspace_size.
+ */
+size_t mmn_simulate_channels_workspace_size(int width, int height, int samples,
+                                            int n_sim);
+int mmn_simulate_channels(float *channels, const float *grid, int n_comp,
+                          const int32_t *comp, const float *taus, const float *eyes,
+                          const float *rx_dirs, const double *rx_locs, int n_sim,
+                          int width, int height, float width_fov, float height_fov,
+                          const float *bounding_box, const int *grid_res,
+                          float voxel_size, double f0, double df, int samples,
+                          int is_ti_radar, void *workspace, size_t workspace_bytes,
+                          void *stream);
+
+/*
  * mmn_channel_loss -- loss_fn of isosurface_optimization.py:211-246 for
  * n_pairs (simulated, measured) channel pairs (complex [n_pairs][samples]
  * each): the DFT of each zero-padded to n_fft, bins [bin_lo, bin_hi) (the

@@ -21,6 +21,8 @@
 //   * the channel sum uses the FP64-anchored phase recurrence of the
 //     backward (anchor every K = 32 samples) instead of one FP64 exp per
 //     (ray, sample), and FP32 lane sums reduced in FP64 in a fixed order.
+#include <vector>
+
 #include "mmn_internal.h"
 
 namespace {
@@ -32,6 +34,7 @@ constexpr int kMaxSteps = 1000;        // ray_tracing_kernel.cu:342
 constexpr float kUndef = -50.0f;       // UNDEF_VAL
 constexpr float kMinStep = 0.005f;     // MIN_STEP
 constexpr int kK = 32;                 // samples per wave of the synthesis
+constexpr int kMaxSims = 65535;        // simulations of one batched launch (grid y)
 
 struct RayGeom {
   float fov_w, fov_h;       // degrees
@@ -41,6 +44,19 @@ struct RayGeom {
   int width, height;
   float tau;                // isosurface offset subtracted from the grid
 };
+
+// One simulation of a batch (mmn_simulate_channels): the antenna, its look
+// direction, the isosurface offset, the grid component and the receiver
+// position of the FP64 round trip.  64 bytes, read with scalar loads.
+struct SimParam {
+  float eye[3];
+  float rxd[3];
+  float tau;
+  int32_t comp;
+  double rx[3];
+  double pad;
+};
+static_assert(sizeof(SimParam) == 64, "SimParam is one 64-byte record");
 
 __device__ __forceinline__ float len3(float x, float y, float z) {
 #pragma clang fp contract(off)
@@ -224,13 +240,33 @@ __device__ __forceinline__ int64_t torch_index(float v, int res) {
 // Per ray: march, refine (isosurface_optimization.py:85-109,190-196) and
 // the round-trip distance in FP64 (isosurface_optimization.py:198-201);
 // NaN marks a ray without a hit.
+// BATCH: simulation s = blockIdx.y takes its antenna, direction, offset,
+// component (grid + comp * comp_stride) and receiver from sp[s] and writes
+// dist[s * rays + p]; the arithmetic of a ray is the same either way.
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
                                                      const float *__restrict__ grid, RayGeom g,
                                                      float vsz, double rx0, double rx1,
-                                                     double rx2, int is_ti) {
+                                                     double rx2, int is_ti,
+                                                     const SimParam *__restrict__ sp,
+                                                     int64_t comp_stride) {
 #pragma clang fp contract(off)
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= g.width * g.height) return;
+  if constexpr (BATCH) {
+    const SimParam &q = sp[blockIdx.y];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      g.eye[a] = q.eye[a];
+      g.rxd[a] = q.rxd[a];
+    }
+    g.tau = q.tau;
+    rx0 = q.rx[0];
+    rx1 = q.rx[1];
+    rx2 = q.rx[2];
+    grid += (int64_t)q.comp * comp_stride;
+    dist += (int64_t)blockIdx.y * g.width * g.height;
+  }
   float d[3], hit[3], vox[3];
   ray_dir(g, p, d[0], d[1], d[2]);
   intersect(grid, g, d[0], d[1], d[2], hit, vox);
@@ -284,7 +320,10 @@ __global__ __launch_bounds__(kBlock) void k_ray_synth(float2 *__restrict__ part,
                                                       const double *__restrict__ dist,
                                                       int nrays, int chunk, double f0c,
                                                       double dfc, int F, int nfb,
-                                                      int64_t nitems) {
+                                                      int64_t nitems, int nchunks) {
+  // simulation blockIdx.y of a batch: its rays and its partials
+  dist += (int64_t)blockIdx.y * nrays;
+  part += (int64_t)blockIdx.y * nchunks * F;
   const int lane = threadIdx.x % kWave;
   const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   const bool live = item < nitems;
@@ -342,6 +381,8 @@ __global__ __launch_bounds__(kBlock) void k_synth_reduce(float2 *__restrict__ ou
                                                          int nchunks, int F) {
   const int f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= F) return;
+  out += (int64_t)blockIdx.y * F;
+  part += (int64_t)blockIdx.y * nchunks * F;
   double sr = 0.0, si = 0.0;
   for (int c = 0; c < nchunks; ++c) {
     const float2 v = part[(size_t)c * F + f];
@@ -361,6 +402,11 @@ __global__ __launch_bounds__(kBlock) void k_channel_loss(float *__restrict__ los
                                                          const float2 *__restrict__ tgt,
                                                          int F, int n_fft, int lo, int hi,
                                                          int hi_target) {
+  // No contraction: the simulated and the target channel go through the same
+  // rounded operations, so a channel against itself gives a loss of exactly
+  // 0, as the reference's difference of two normalised tensors does (a fused
+  // multiply-add can contract the two sides differently).
+#pragma clang fp contract(off)
   __shared__ double sh[3][kBlock];
   __shared__ float mag[2][1024];
   const float2 *a = sim + (size_t)blockIdx.x * F;
@@ -487,8 +533,9 @@ extern "C" int mmn_simulate_channel(float *channel, const float *grid, float tau
   double *dist = (double *)workspace;
   float2 *part = (float2 *)((char *)workspace + align256(sizeof(double) * (size_t)n));
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_ray_dist, dim3(mmn_blocks(n, kBlock)), dim3(kBlock), 0, st, dist, grid,
-                     g, voxel_size, rx_loc[0], rx_loc[1], rx_loc[2], is_ti_radar);
+  hipLaunchKernelGGL(k_ray_dist<false>, dim3(mmn_blocks(n, kBlock)), dim3(kBlock), 0, st, dist,
+                     grid, g, voxel_size, rx_loc[0], rx_loc[1], rx_loc[2], is_ti_radar,
+                     (const SimParam *)nullptr, (int64_t)0);
   const int nch = synth_chunks(n);
   const int chunk = (n + nch - 1) / nch;
   const int nfb = (samples + kK - 1) / kK;
@@ -496,10 +543,100 @@ extern "C" int mmn_simulate_channel(float *channel, const float *grid, float tau
   const double inv_c = 1.0 / MMN_C0;
   hipLaunchKernelGGL(k_ray_synth, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
                      dim3(kBlock), 0, st, part, dist, n, chunk, f0 * inv_c, df * inv_c,
-                     samples, nfb, nitems);
+                     samples, nfb, nitems, nch);
   hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock)), dim3(kBlock), 0, st,
                      (float2 *)channel, part, nch, samples);
   return mmn_check_launch("mmn_simulate_channel");
+}
+
+namespace {
+
+size_t batch_ws(int n, int samples, int nsim) {
+  return align256(sizeof(SimParam) * (size_t)nsim) +
+         align256(sizeof(double) * (size_t)n * nsim) +
+         align256(sizeof(float2) * (size_t)synth_chunks(n) * samples * nsim);
+}
+
+}  // namespace
+
+extern "C" size_t mmn_simulate_channels_workspace_size(int width, int height, int samples,
+                                                       int n_sim) {
+  if (width <= 0 || height <= 0 || samples <= 0 || n_sim <= 0 ||
+      (int64_t)width * height > INT32_MAX)
+    return 0;
+  return batch_ws(width * height, samples, n_sim);
+}
+
+extern "C" int mmn_simulate_channels(float *channels, const float *grid, int n_comp,
+                                     const int32_t *comp, const float *taus,
+                                     const float *eyes, const float *rx_dirs,
+                                     const double *rx_locs, int n_sim, int width, int height,
+                                     float width_fov, float height_fov,
+                                     const float *bounding_box, const int *grid_res,
+                                     float voxel_size, double f0, double df, int samples,
+                                     int is_ti_radar, void *workspace, size_t workspace_bytes,
+                                     void *stream) {
+  MMN_REQUIRE(n_sim >= 0 && n_sim <= kMaxSims, MMN_ERR_ARG,
+              "mmn_simulate_channels: n_sim must be in [0, %d] (got %d)", kMaxSims, n_sim);
+  if (n_sim == 0) return 0;
+  MMN_REQUIRE(channels && grid && comp && taus && eyes && rx_dirs && rx_locs && bounding_box &&
+                  grid_res,
+              MMN_ERR_NULL, "mmn_simulate_channels: NULL pointer argument");
+  const float zero3[3] = {0.0f, 0.0f, 0.0f};
+  RayGeom g;
+  MMN_REQUIRE(geom_from(g, width, height, width_fov, height_fov, bounding_box, grid_res, zero3,
+                        zero3, 0.0f) &&
+                  samples > 0 && n_comp > 0,
+              MMN_ERR_ARG, "mmn_simulate_channels: bad sizes");
+  for (int s = 0; s < n_sim; ++s)
+    MMN_REQUIRE(comp[s] >= 0 && comp[s] < n_comp, MMN_ERR_ARG,
+                "mmn_simulate_channels: comp[%d] = %d outside [0, %d)", s, comp[s], n_comp);
+  const int n = width * height;
+  const size_t need = batch_ws(n, samples, n_sim);
+  MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
+              "mmn_simulate_channels: workspace of %zu bytes needed", need);
+  // the simulations' parameters, packed on the host and copied to the
+  // workspace's head (the copy has left the host buffer when it returns)
+  std::vector<SimParam> prm((size_t)n_sim);
+  for (int s = 0; s < n_sim; ++s) {
+    SimParam &q = prm[(size_t)s];
+    for (int a = 0; a < 3; ++a) {
+      q.eye[a] = eyes[3 * s + a];
+      q.rxd[a] = rx_dirs[3 * s + a];
+      q.rx[a] = rx_locs[3 * s + a];
+    }
+    q.tau = taus[s];
+    q.comp = comp[s];
+    q.pad = 0.0;
+  }
+  char *w = (char *)workspace;
+  SimParam *sp = (SimParam *)w;
+  double *dist = (double *)(w + align256(sizeof(SimParam) * (size_t)n_sim));
+  float2 *part = (float2 *)((char *)dist + align256(sizeof(double) * (size_t)n * n_sim));
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(sp, prm.data(), sizeof(SimParam) * (size_t)n_sim,
+                                hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
+    mmn_set_error("mmn_simulate_channels: parameter copy: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  const int64_t vox = (int64_t)grid_res[0] * grid_res[1] * grid_res[2];
+  hipLaunchKernelGGL(k_ray_dist<true>, dim3(mmn_blocks(n, kBlock), (unsigned)n_sim),
+                     dim3(kBlock), 0, st, dist, grid, g, voxel_size, 0.0, 0.0, 0.0, is_ti_radar,
+                     (const SimParam *)sp, vox);
+  const int nch = synth_chunks(n);
+  const int chunk = (n + nch - 1) / nch;
+  const int nfb = (samples + kK - 1) / kK;
+  const int64_t nitems = (int64_t)nfb * nch;
+  const double inv_c = 1.0 / MMN_C0;
+  hipLaunchKernelGGL(k_ray_synth,
+                     dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock),
+                          (unsigned)n_sim),
+                     dim3(kBlock), 0, st, part, dist, n, chunk, f0 * inv_c, df * inv_c,
+                     samples, nfb, nitems, nch);
+  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock), (unsigned)n_sim),
+                     dim3(kBlock), 0, st, (float2 *)channels, part, nch, samples);
+  return mmn_check_launch("mmn_simulate_channels");
 }
 
 extern "C" int mmn_channel_loss(float *loss, const float *sim, const float *target,

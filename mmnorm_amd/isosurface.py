@@ -124,6 +124,43 @@ def _simulate_channels(bounding_box, bounding_box_lower, voxel_size, grid_res_x,
     return combined
 
 
+def _simulate_channel_batch(bounding_box, voxel_size, width, height, width_fov, height_fov,
+                            grid_cm, rx_locs, rx_dirs, wavelengths, taus, tau_bounds,
+                            is_real_data):
+    """_simulate_channels for several antennas at once: row r is the combined
+    channel of antenna (rx_locs[r], rx_dirs[r]) over every component of
+    ``grid_cm`` (n_comp, NX, NY, NZ) with the clipped offsets ``taus``, bit for
+    bit what _simulate_channels returns for it (the same per-simulation values,
+    the components added to zeros in order), from one batched launch
+    (ops.simulate_channels)."""
+    n_comp = grid_cm.shape[0]
+    n_wl = int(wavelengths.shape[0])
+    f0, df = _freqs(wavelengths, n_wl)
+    rx = np.asarray(rx_locs.detach().cpu().numpy() if torch.is_tensor(rx_locs) else rx_locs,
+                    dtype=np.float64).reshape(-1, 3)
+    rd = np.asarray(rx_dirs.detach().cpu().numpy() if torch.is_tensor(rx_dirs) else rx_dirs,
+                    dtype=np.float64).reshape(-1, 3)
+    n_ant = len(rx)
+    tv = np.empty(n_comp, dtype=np.float32)
+    for k in range(n_comp):
+        lo, hi = tau_bounds[k]
+        tv[k] = float(np.float32(_scalar(torch.clip(
+            torch.as_tensor(taus[k], dtype=torch.float32),
+            torch.as_tensor(lo, dtype=torch.float32), torch.as_tensor(hi, dtype=torch.float32)))))
+    comp = np.tile(np.arange(n_comp, dtype=np.int32), n_ant)
+    rep = lambda a: np.repeat(a, n_comp, axis=0)
+    vsz = float(np.float32(voxel_size.item() if torch.is_tensor(voxel_size) else voxel_size))
+    sims = ops.simulate_channels(grid_cm, comp, np.tile(tv, n_ant), rep(rx.astype(np.float32)),
+                                 rep(rd.astype(np.float32)), rep(rx), int(width), int(height),
+                                 float(width_fov), float(height_fov),
+                                 [float(v) for v in bounding_box], vsz, f0, df, n_wl,
+                                 is_ti=is_real_data).view(n_ant, n_comp, n_wl)
+    combined = torch.zeros((n_ant, n_wl), dtype=torch.complex64, device=grid_cm.device)
+    for k in range(n_comp):
+        combined += sims[:, k]
+    return combined
+
+
 def loss_fn(output_ch, target_chs, is_real_data):
     """Range-profile loss of one simulated / measured channel pair
     (isosurface_optimization.py:211-246); needs set_range_window first."""
@@ -245,14 +282,19 @@ def search_taus(sdf, rx_locs, rx_dirs, channels, bounding_box, wavelengths, voxe
     order = torch.randperm(len(rx_locs), generator=generator)[:num_rx]
     dev = sdf.device
     tgt = channels[order].to(dev)
+    # every antenna and component of one offset in one batched simulation
+    grid_cm = sdf.permute(3, 0, 1, 2).float().contiguous()
+    sel = order.tolist()
+    locs = (rx_locs.detach().cpu().numpy() if torch.is_tensor(rx_locs)
+            else np.asarray(rx_locs))[sel]
+    dirs = (rx_dirs.detach().cpu().numpy() if torch.is_tensor(rx_dirs)
+            else np.asarray(rx_dirs))[sel]
     for flat_i in range(int(np.prod(shape))):
         idx = np.unravel_index(flat_i, shape)
         taus = tau_grid[idx]
-        sims = torch.stack([
-            _simulate_channels(bounding_box, None, voxel_size, *sdf.shape[:3], width, height,
-                               width_fov, height_fov, sdf, rx_locs[r], rx_dirs[r], wl, taus,
-                               tau_bounds, is_real_data)
-            for r in order.tolist()])
+        sims = _simulate_channel_batch(bounding_box, voxel_size, width, height, width_fov,
+                                       height_fov, grid_cm, locs, dirs, wl, taus, tau_bounds,
+                                       is_real_data)
         losses[idx] = batch_loss(sims, tgt, is_real_data).sum().cpu()
     taus = _select_tau(tau_grid.numpy(), losses.numpy())
     return taus, tau_grid, losses

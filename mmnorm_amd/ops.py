@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -479,6 +480,49 @@ def simulate_channel(grid, tau, width, height, width_fov, height_fov, bounding_b
               _f32s(eye, 3), _f32s(rx_dir, 3), rx, float(voxel_size), float(f0), float(df),
               int(samples), int(bool(is_ti)), ws.data_ptr(), ws_bytes, _lib.stream_handle(dev))
     return ch
+
+
+MAX_BATCH_RAYS = 1 << 27   # rays of one mmn_simulate_channels launch (1 GiB of distances)
+
+
+def simulate_channels(grid, comp, taus, eyes, rx_dirs, rx_locs, width, height, width_fov,
+                      height_fov, bounding_box, voxel_size, f0, df, samples, is_ti=False):
+    """Channels of many simulations at once (complex64, (n_sim, samples)):
+    simulation s is simulate_channel on component ``grid[comp[s]]`` with
+    offset ``taus[s]``, antenna ``eyes[s]``, direction ``rx_dirs[s]`` and
+    receiver ``rx_locs[s]``, bit for bit.  ``grid``: float32 (n_comp, NX, NY,
+    NZ) on the device; the per-simulation values are host arrays.  Launches
+    hold at most 65535 simulations and MAX_BATCH_RAYS rays."""
+    _lib.require_gpu()
+    _check(grid, "grid", torch.float32, 4)
+    dev = grid.device
+    comp = np.ascontiguousarray(comp, dtype=np.int32).reshape(-1)
+    n = len(comp)
+    taus = np.ascontiguousarray(taus, dtype=np.float32).reshape(n)
+    eyes = np.ascontiguousarray(eyes, dtype=np.float32).reshape(n, 3)
+    rx_dirs = np.ascontiguousarray(rx_dirs, dtype=np.float32).reshape(n, 3)
+    rx_locs = np.ascontiguousarray(rx_locs, dtype=np.float64).reshape(n, 3)
+    if n and (comp.min() < 0 or comp.max() >= grid.shape[0]):
+        raise ValueError(f"comp must lie in [0, {grid.shape[0]})")
+    out = torch.empty((n, int(samples)), dtype=torch.complex64, device=dev)
+    lib = _lib.load()
+    res = (ctypes.c_int * 3)(*grid.shape[1:])
+    bb = _f32s(bounding_box, 6)
+    rays = int(width) * int(height)
+    per = max(1, min(65535, MAX_BATCH_RAYS // max(rays, 1)))
+    for lo in range(0, n, per):
+        hi = min(n, lo + per)
+        ws_bytes = int(lib.mmn_simulate_channels_workspace_size(int(width), int(height),
+                                                                int(samples), hi - lo))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.call("mmn_simulate_channels", out[lo:hi].data_ptr(), grid.data_ptr(),
+                  int(grid.shape[0]), comp[lo:hi].ctypes.data, taus[lo:hi].ctypes.data,
+                  eyes[lo:hi].ctypes.data, rx_dirs[lo:hi].ctypes.data,
+                  rx_locs[lo:hi].ctypes.data, hi - lo, int(width), int(height),
+                  float(width_fov), float(height_fov), bb, res, float(voxel_size), float(f0),
+                  float(df), int(samples), int(bool(is_ti)), ws.data_ptr(), ws_bytes,
+                  _lib.stream_handle(dev))
+    return out
 
 
 def channel_loss(sim, target, n_fft, bin_lo, bin_hi, bin_hi_target=None):

@@ -99,6 +99,11 @@ def test_channel_loss_matches_numpy_fft():
         t /= np.linalg.norm(t)
         want = np.sum(np.abs(np.abs(a) - np.abs(t)))
         assert abs(got[b] - want) <= 1e-5 * want
+    # a channel against itself: exactly zero, as the reference's difference of
+    # two normalised tensors
+    same = ops.channel_loss(torch.from_numpy(sim).to(dev()), torch.from_numpy(sim).to(dev()),
+                            N, lo, hi, hi).cpu().numpy()
+    assert np.all(same == 0.0)
 
 
 def test_ray_matching_dropin_shapes_and_values():
@@ -129,3 +134,118 @@ def test_simulate_channels_dropin_sums_components_and_clips_tau():
                                 [0.001, 0.0, 0.3], [0, 0, 1], rx.tolist(), VS, f[0], f[1] - f[0],
                                 F) for t in (0.001, 0.004)]
     assert torch.allclose(both, one[0] + one[1], atol=1e-3)
+
+
+def two_component_grid():
+    """Two RSDF-like components of different radii, component-major."""
+    g1, bbox = sphere_scene()
+    g2, _ = sphere_scene(radius=0.012)
+    return np.stack([g1, g2]), bbox
+
+
+def test_simulate_channels_batch_equals_single_calls(monkeypatch):
+    """Each simulation of a batch -- its own antenna, direction, offset,
+    component, TI flag shared -- equals mmn_simulate_channel bit for bit, also
+    when the wrapper splits the batch over several launches."""
+    gcm, bbox = two_component_grid()
+    gt = torch.from_numpy(gcm).to(dev())
+    rng = np.random.default_rng(5)
+    n = 7
+    comp = rng.integers(0, 2, n)
+    taus = rng.uniform(-0.003, 0.003, n).astype(np.float32)
+    eyes = np.stack([rng.uniform(-0.01, 0.01, n), rng.uniform(-0.01, 0.01, n),
+                     rng.uniform(0.25, 0.3, n)], 1)
+    dirs = np.stack([rng.uniform(-0.1, 0.1, n), rng.uniform(-0.1, 0.1, n), np.ones(n)], 1)
+    W, H, F = 40, 36, 96
+    f0, df = 77e9, 4e9 / (F - 1)
+    for is_ti in (False, True):
+        one = torch.stack([ops.simulate_channel(
+            gt[int(comp[s])].contiguous(), float(taus[s]), W, H, 8.0, 7.0, bbox,
+            np.float32(eyes[s]).tolist(), np.float32(dirs[s]).tolist(), eyes[s].tolist(), VS,
+            f0, df, F, is_ti=is_ti) for s in range(n)])
+        assert torch.count_nonzero(one) > 0
+        got = ops.simulate_channels(gt, comp, taus, eyes, dirs, eyes, W, H, 8.0, 7.0, bbox, VS,
+                                    f0, df, F, is_ti=is_ti)
+        assert torch.equal(got, one)
+        monkeypatch.setattr(ops, "MAX_BATCH_RAYS", 3 * W * H)   # three launches of 3, 3, 1
+        split = ops.simulate_channels(gt, comp, taus, eyes, dirs, eyes, W, H, 8.0, 7.0, bbox,
+                                      VS, f0, df, F, is_ti=is_ti)
+        monkeypatch.undo()
+        assert torch.equal(split, one)
+
+
+def test_simulate_channels_rejects_bad_components():
+    gcm, bbox = two_component_grid()
+    gt = torch.from_numpy(gcm).to(dev())
+    args = ([0.0], [[0, 0, 0.3]], [[0, 0, 1]], [[0, 0, 0.3]], 8, 8, 8.0, 8.0, bbox, VS, 77e9,
+            1e8, 16)
+    with pytest.raises(ValueError):
+        ops.simulate_channels(gt, [2], *args)
+    with pytest.raises(ValueError):
+        ops.simulate_channels(gt, [-1], *args)
+
+
+def test_channel_batch_equals_per_antenna_dropin():
+    """isosurface's batch of one offset over several antennas and both
+    components (one offset clipped) equals _simulate_channels per antenna."""
+    gcm, bbox = two_component_grid()
+    g_last = torch.from_numpy(np.ascontiguousarray(np.moveaxis(gcm, 0, -1))).to(dev())
+    F = 64
+    wl = torch.from_numpy(synth.C0 / (77e9 + 4e9 / (F - 1) * np.arange(F)))
+    locs = torch.tensor([[0.001, 0.0, 0.3], [-0.004, 0.003, 0.28], [0.006, -0.002, 0.31]],
+                        dtype=torch.float64)
+    dirs = torch.tensor([[0.0, 0.0, 1.0], [0.05, 0.0, 1.0], [0.0, -0.04, 1.0]])
+    bounds = np.array([[-0.004, 0.004], [-0.004, 0.004]])
+    taus = torch.tensor([0.0013, 0.5])
+    want = torch.stack([isosurface._simulate_channels(
+        bbox, None, VS, *gcm.shape[1:], 24, 20, 8.0, 8.0, g_last, locs[r], dirs[r], wl, taus,
+        bounds, True) for r in range(3)])
+    got = isosurface._simulate_channel_batch(bbox, VS, 24, 20, 8.0, 8.0,
+                                             torch.from_numpy(gcm).to(dev()), locs, dirs, wl,
+                                             taus, bounds, True)
+    assert torch.count_nonzero(want) > 0
+    assert torch.equal(got, want)
+
+
+def test_search_taus_matches_per_call_loop_and_finds_generating_offset():
+    """search_taus (isosurface_optimization.py:454-521) against the loop it
+    replaces -- _simulate_channels per (offset, antenna) and batch_loss --
+    bit for bit, on targets simulated at one offset of the grid: the loss is
+    zero there and larger everywhere else."""
+    grid, bbox = sphere_scene()
+    sdf = torch.from_numpy(grid[..., None]).to(dev())
+    F = 64
+    wl = torch.from_numpy(synth.C0 / (77e9 + 4e9 / (F - 1) * np.arange(F)))
+    rng = np.random.default_rng(11)
+    n_ant = 24
+    locs = torch.from_numpy(np.stack([rng.uniform(-0.012, 0.012, n_ant),
+                                      rng.uniform(-0.012, 0.012, n_ant),
+                                      np.full(n_ant, 0.3)], 1))
+    dirs = torch.zeros(n_ant, 3)
+    dirs[:, 2] = 1.0
+    bounds = np.array([[-0.006, 0.006]])
+    step = float(wl[0]) / 2
+    tau_axis = torch.arange(-0.006, 0.006, step, dtype=torch.float32)
+    t_star = tau_axis[2:3]
+    chans = torch.stack([isosurface._simulate_channels(
+        bbox, None, VS, *grid.shape, 32, 32, 10.0, 10.0, sdf, locs[r], dirs[r], wl, t_star,
+        bounds, False) for r in range(n_ant)])
+    kw = dict(is_real_data=False, width=32, height=32, width_fov=10.0, height_fov=10.0,
+              num_rx=10, skip=0, min_dist=0.2, max_dist=0.4)
+    taus, tau_grid, losses = isosurface.search_taus(
+        sdf, locs, dirs, chans, bbox, wl, VS, bounds, F,
+        generator=torch.Generator().manual_seed(3), **kw)
+    # the loop of the reference, on the same antennas
+    ch_f, loc_f, dir_f = isosurface._filter_channels(chans, locs, dirs, False)
+    assert len(ch_f) >= 10
+    order = torch.randperm(len(loc_f), generator=torch.Generator().manual_seed(3))[:10]
+    want = torch.zeros(len(tau_grid), dtype=torch.float32)
+    for i in range(len(tau_grid)):
+        sims = torch.stack([isosurface._simulate_channels(
+            bbox, None, VS, *grid.shape, 32, 32, 10.0, 10.0, sdf, loc_f[r], dir_f[r], wl,
+            tau_grid[i], bounds, False) for r in order.tolist()])
+        want[i] = isosurface.batch_loss(sims, ch_f[order].to(dev()), False).sum().cpu()
+    assert torch.equal(losses, want)
+    assert torch.equal(tau_grid[:, 0], tau_axis)
+    assert float(losses[2]) == 0.0
+    assert float(torch.min(torch.cat([losses[:2], losses[3:]]))) > 0.0
