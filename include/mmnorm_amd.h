@@ -403,6 +403,42 @@ int mmn_simulate_channels(float *channels, const float *grid, int n_comp,
                           void *stream);
 
 /*
+ * mmn_ray_prefix / mmn_simulate_channels_prefixed -- the isosurface offset
+ * search (isosurface_optimization.py:475-521) simulates the same antennas
+ * and components for every offset tau.  A ray's march does not depend on
+ * tau while the ray is outside the box or in a cell with an undefined corner
+ * (grid - tau < UNDEF_VAL + 10 only turns true as tau grows, so a cell
+ * undefined at the search's smallest offset tau_lo is undefined at every
+ * tau >= tau_lo, and its step is MIN_STEP).  mmn_ray_prefix marches each ray
+ * of n_sim simulations (comp, eyes, rx_dirs as for mmn_simulate_channels) up
+ * to its first step in a cell fully defined at tau_lo, records that step, and
+ * lists the rays that reach one or hit before it ("live" rays); it writes
+ * each simulation's live-ray count to live_counts[n_sim] (host) and
+ * synchronises the stream.  mmn_simulate_channels_prefixed then gives the
+ * channels of mmn_simulate_channels for offsets taus[s] >= tau_lo, bit for
+ * bit, marching and summing the live rays only.  The prefix buffer
+ * (mmn_ray_prefix_size bytes, device) belongs to one set of simulations:
+ * pass the same comp, eyes, rx_dirs, grid and geometry to both calls.
+ */
+size_t mmn_ray_prefix_size(int width, int height, int n_sim);
+int mmn_ray_prefix(void *prefix, size_t prefix_bytes, int *live_counts, const float *grid,
+                   int n_comp, const int32_t *comp, const float *eyes, const float *rx_dirs,
+                   int n_sim, int width, int height, float width_fov, float height_fov,
+                   const float *bounding_box, const int *grid_res, float tau_lo,
+                   void *stream);
+size_t mmn_simulate_channels_prefixed_workspace_size(int width, int height, int samples,
+                                                     int n_sim, int live_max);
+int mmn_simulate_channels_prefixed(float *channels, const void *prefix, size_t prefix_bytes,
+                                   const int *live_counts, const float *grid, int n_comp,
+                                   const int32_t *comp, const float *taus, const float *eyes,
+                                   const float *rx_dirs, const double *rx_locs, int n_sim,
+                                   int width, int height, float width_fov,
+                                   float height_fov, const float *bounding_box,
+                                   const int *grid_res, float voxel_size, double f0,
+                                   double df, int samples, int is_ti_radar, float tau_lo,
+                                   void *workspace, size_t workspace_bytes, void *stream);
+
+/*
  * mmn_channel_loss -- loss_fn of isosurface_optimization.py:211-246 for
  * n_pairs (simulated, measured) channel pairs (complex [n_pairs][samples]
  * each): the DFT of each zero-padded to n_fft, bins [bin_lo, bin_hi) (the

@@ -83,6 +83,15 @@ SIGNATURES = {
     "mmn_simulate_channels": (_c_int, [_p, _p, _c_int, _p, _p, _p, _p, _p, _c_int, _c_int,
                                        _c_int, _c_f, _c_f, _p, _p, _c_f, _c_d, _c_d, _c_int,
                                        _c_int, _p, _sz, _p]),
+    "mmn_ray_prefix_size": (_sz, [_c_int, _c_int, _c_int]),
+    "mmn_ray_prefix": (_c_int, [_p, _sz, _p, _p, _c_int, _p, _p, _p, _c_int, _c_int, _c_int,
+                                _c_f, _c_f, _p, _p, _c_f, _p]),
+    "mmn_simulate_channels_prefixed_workspace_size": (_sz, [_c_int, _c_int, _c_int, _c_int,
+                                                            _c_int]),
+    "mmn_simulate_channels_prefixed": (_c_int, [_p, _p, _sz, _p, _p, _c_int, _p, _p, _p, _p,
+                                                _p, _c_int, _c_int, _c_int, _c_f, _c_f, _p,
+                                                _p, _c_f, _c_d, _c_d, _c_int, _c_int, _c_f,
+                                                _p, _sz, _p]),
     "mmn_channel_loss": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_int, _c_int, _c_int,
                                   _c_int, _p]),
 }

@@ -21,6 +21,9 @@
 //   * the channel sum uses the FP64-anchored phase recurrence of the
 //     backward (anchor every K = 32 samples) instead of one FP64 exp per
 //     (ray, sample), and FP32 lane sums reduced in FP64 in a fixed order.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "mmn_internal.h"
@@ -43,6 +46,7 @@ struct RayGeom {
   float eye[3], rxd[3];
   int width, height;
   float tau;                // isosurface offset subtracted from the grid
+  int small;                // NX NY NZ <= 2^24: corner indices in integers (value_at)
 };
 
 // One simulation of a batch (mmn_simulate_channels): the antenna, its look
@@ -132,10 +136,48 @@ __device__ __forceinline__ int64_t flat_idx(const RayGeom &g, float x, float y, 
   return i < 0 ? 0 : (i >= V ? V - 1 : i);
 }
 
-__device__ __forceinline__ float grid_at(const float *grid, const RayGeom &g, float x, float y,
-                                         float z) {
+// The cell of ValueAt (ray_tracing_kernel.cu:255-270) for a point inside the
+// box: its minimum corner as the reference's float indices, and the grid at
+// its eight corners (before tau is subtracted), in the order
+// 000, 010, 001, 011, 100, 110, 101, 111 (x, y, z offsets).
+__device__ __forceinline__ void cell_at(const float *grid, const RayGeom &g, float px, float py,
+                                        float pz, float &ix, float &iy, float &iz,
+                                        float (&c)[8]) {
 #pragma clang fp contract(off)
-  return grid[flat_idx(g, x, y, z)] - g.tau;
+  const float vs = (g.bmax[1] - g.bmin[1]) / (float)(g.res[1] - 1);
+  ix = floorf((px - g.bmin[0]) / vs);
+  iy = floorf((py - g.bmin[1]) / vs);
+  iz = floorf((pz - g.bmin[2]) / vs);
+  if (ix == (g.bmax[0] - g.bmin[0]) / vs) ix = (g.bmax[0] - g.bmin[0]) / vs - 1;
+  if (iy == (g.bmax[1] - g.bmin[1]) / vs) iy = (g.bmax[1] - g.bmin[1]) / vs - 1;
+  if (iz == (g.bmax[2] - g.bmin[2]) / vs) iz = (g.bmax[2] - g.bmin[2]) / vs - 1;
+  if (g.small) {
+    // flat_idx sums non-negative integer terms in FP32: below 2^24 every
+    // partial sum is exact, and a true index at or above 2^24 (>= V here)
+    // stays there and is clamped to V - 1 either way.  So the corners are the
+    // base index plus their offsets in integers, clamped the same.
+    const int64_t s2 = g.res[2], s1 = (int64_t)g.res[1] * g.res[2], V = g.res[0] * s1;
+    const int64_t b = (int64_t)(int)iz + (int64_t)(int)iy * s2 + (int64_t)(int)ix * s1;
+    auto at = [&](int64_t o) { return grid[min(b + o, V - 1)]; };
+    c[0] = at(0), c[1] = at(s2), c[2] = at(1), c[3] = at(s2 + 1);
+    c[4] = at(s1), c[5] = at(s1 + s2), c[6] = at(s1 + 1), c[7] = at(s1 + s2 + 1);
+  } else {
+    c[0] = grid[flat_idx(g, ix, iy, iz)], c[1] = grid[flat_idx(g, ix, iy + 1, iz)];
+    c[2] = grid[flat_idx(g, ix, iy, iz + 1)], c[3] = grid[flat_idx(g, ix, iy + 1, iz + 1)];
+    c[4] = grid[flat_idx(g, ix + 1, iy, iz)], c[5] = grid[flat_idx(g, ix + 1, iy + 1, iz)];
+    c[6] = grid[flat_idx(g, ix + 1, iy, iz + 1)];
+    c[7] = grid[flat_idx(g, ix + 1, iy + 1, iz + 1)];
+  }
+}
+
+// A corner is undefined when grid - tau < UNDEF_VAL + 10 (ray_tracing_kernel.cu:274-285).
+__device__ __forceinline__ bool any_undefined(const float (&c)[8], float tau) {
+#pragma clang fp contract(off)
+  const float lim = kUndef + 10;
+  bool u = false;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) u |= c[k] - tau < lim;
+  return u;
 }
 
 // ValueAt (ray_tracing_kernel.cu:206-301) on grid - tau.
@@ -147,21 +189,12 @@ __device__ float value_at(const float *grid, const RayGeom &g, float px, float p
     return -1;
   }
   const float vs = (g.bmax[1] - g.bmin[1]) / (float)(g.res[1] - 1);
-  float ix = floorf((px - g.bmin[0]) / vs);
-  float iy = floorf((py - g.bmin[1]) / vs);
-  float iz = floorf((pz - g.bmin[2]) / vs);
-  if (ix == (g.bmax[0] - g.bmin[0]) / vs) ix = (g.bmax[0] - g.bmin[0]) / vs - 1;
-  if (iy == (g.bmax[1] - g.bmin[1]) / vs) iy = (g.bmax[1] - g.bmin[1]) / vs - 1;
-  if (iz == (g.bmax[2] - g.bmin[2]) / vs) iz = (g.bmax[2] - g.bmin[2]) / vs - 1;
-  const float lim = kUndef + 10;
-  const float g000 = grid_at(grid, g, ix, iy, iz), g010 = grid_at(grid, g, ix, iy + 1, iz);
-  const float g001 = grid_at(grid, g, ix, iy, iz + 1), g011 = grid_at(grid, g, ix, iy + 1, iz + 1);
-  const float g100 = grid_at(grid, g, ix + 1, iy, iz), g110 = grid_at(grid, g, ix + 1, iy + 1, iz);
-  const float g101 = grid_at(grid, g, ix + 1, iy, iz + 1);
-  const float g111 = grid_at(grid, g, ix + 1, iy + 1, iz + 1);
-  if (g000 < lim || g010 < lim || g001 < lim || g011 < lim || g100 < lim || g110 < lim ||
-      g101 < lim || g111 < lim)
-    return kMinStep;
+  float ix, iy, iz, c[8];
+  cell_at(grid, g, px, py, pz, ix, iy, iz, c);
+  if (any_undefined(c, g.tau)) return kMinStep;
+  const float g000 = c[0] - g.tau, g010 = c[1] - g.tau, g001 = c[2] - g.tau;
+  const float g011 = c[3] - g.tau, g100 = c[4] - g.tau, g110 = c[5] - g.tau;
+  const float g101 = c[6] - g.tau, g111 = c[7] - g.tau;
   const float tx = (px - (g.bmin[0] + ix * vs)) / vs;
   const float c01 = (1.f - tx) * g000 + tx * g100;
   const float c23 = (1.f - tx) * g010 + tx * g110;
@@ -174,36 +207,47 @@ __device__ float value_at(const float *grid, const RayGeom &g, float px, float p
   return (1.f - tz) * c0 + tz * c1;
 }
 
-// Intersect (ray_tracing_kernel.cu:305-421) for one ray: the rough hit and
-// the voxel index of its minimum corner, or all -1.
-__device__ void intersect(const float *grid, const RayGeom &g, float dx, float dy, float dz,
-                          float (&hit)[3], float (&vox)[3]) {
+// The rough hit at the march point r and the voxel index of its minimum
+// corner (ray_tracing_kernel.cu:383-400).
+__device__ __forceinline__ void hit_at(const RayGeom &g, const float (&r)[3], float (&hit)[3],
+                                       float (&vox)[3]) {
 #pragma clang fp contract(off)
   const float vs = (g.bmax[1] - g.bmin[1]) / (float)(g.res[1] - 1);
-  float depth = 0;
-  bool first = true;
-  for (int s = 0; s < kMaxSteps; ++s) {
-    const float rx = g.eye[0] + depth * dx;
-    const float ry = g.eye[1] + depth * dy;
-    const float rz = g.eye[2] + depth * dz;
-    const float dist = value_at(grid, g, rx, ry, rz, dx, dy, dz, first);
-    first = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float v = floorf((r[a] - g.bmin[a]) / vs);
+    if (v == (float)(g.res[a] - 1)) v = v - 1;
+    vox[a] = v;
+    hit[a] = r[a];
+  }
+}
+
+// Intersect (ray_tracing_kernel.cu:305-421) for one ray from step s0 at
+// depth `depth` (0 and 0.0 for the whole march; the first step is the one
+// that may enter the box): the rough hit and the voxel index of its minimum
+// corner, or all -1.
+__device__ void intersect_from(const float *grid, const RayGeom &g, float dx, float dy, float dz,
+                               float depth, int s0, float (&hit)[3], float (&vox)[3]) {
+#pragma clang fp contract(off)
+  const float vs = (g.bmax[1] - g.bmin[1]) / (float)(g.res[1] - 1);
+  for (int s = s0; s < kMaxSteps; ++s) {
+    const float r[3] = {g.eye[0] + depth * dx, g.eye[1] + depth * dy, g.eye[2] + depth * dz};
+    const float dist = value_at(grid, g, r[0], r[1], r[2], dx, dy, dz, s == 0);
     if (dist == -1) break;
     if (dist < vs && dist != 0.0f) {
-      const float r[3] = {rx, ry, rz};
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        float v = floorf((r[a] - g.bmin[a]) / vs);
-        if (v == (float)(g.res[a] - 1)) v = v - 1;
-        vox[a] = v;
-        hit[a] = r[a];
-      }
+      hit_at(g, r, hit, vox);
       return;
     }
     depth += dist;
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) hit[a] = vox[a] = -1.0f;
+}
+
+__device__ __forceinline__ void intersect(const float *grid, const RayGeom &g, float dx,
+                                          float dy, float dz, float (&hit)[3],
+                                          float (&vox)[3]) {
+  intersect_from(grid, g, dx, dy, dz, 0.0f, 0, hit, vox);
 }
 
 // ray_matching drop-in: rays, march, outputs as the reference's three
@@ -237,43 +281,12 @@ __device__ __forceinline__ int64_t torch_index(float v, int res) {
   return i < 0 ? 0 : (i >= res ? res - 1 : i);
 }
 
-// Per ray: march, refine (isosurface_optimization.py:85-109,190-196) and
-// the round-trip distance in FP64 (isosurface_optimization.py:198-201);
-// NaN marks a ray without a hit.
-// BATCH: simulation s = blockIdx.y takes its antenna, direction, offset,
-// component (grid + comp * comp_stride) and receiver from sp[s] and writes
-// dist[s * rays + p]; the arithmetic of a ray is the same either way.
-template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
-                                                     const float *__restrict__ grid, RayGeom g,
-                                                     float vsz, double rx0, double rx1,
-                                                     double rx2, int is_ti,
-                                                     const SimParam *__restrict__ sp,
-                                                     int64_t comp_stride) {
+// The refine step (isosurface_optimization.py:85-109,190-196) of a ray that
+// hit and its round trip in FP64 (isosurface_optimization.py:198-201).
+__device__ double refine_dist(const float *grid, const RayGeom &g, float vsz,
+                              const double (&rx)[3], int is_ti, const float (&d)[3],
+                              const float (&hit)[3], const float (&vox)[3]) {
 #pragma clang fp contract(off)
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= g.width * g.height) return;
-  if constexpr (BATCH) {
-    const SimParam &q = sp[blockIdx.y];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      g.eye[a] = q.eye[a];
-      g.rxd[a] = q.rxd[a];
-    }
-    g.tau = q.tau;
-    rx0 = q.rx[0];
-    rx1 = q.rx[1];
-    rx2 = q.rx[2];
-    grid += (int64_t)q.comp * comp_stride;
-    dist += (int64_t)blockIdx.y * g.width * g.height;
-  }
-  float d[3], hit[3], vox[3];
-  ray_dir(g, p, d[0], d[1], d[2]);
-  intersect(grid, g, d[0], d[1], d[2], hit, vox);
-  if (vox[0] == -1.0f) {  // mask = voxel index x != -1
-    dist[p] = NAN;
-    return;
-  }
   int64_t ii[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) ii[a] = torch_index(vox[a], g.res[a]);
@@ -298,7 +311,6 @@ __global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
   const float c0 = (1 - t[1]) * c01 + t[1] * c23;
   const float c1 = (1 - t[1]) * c45 + t[1] * c67;
   const float sv = (1 - t[2]) * c0 + t[2] * c1;
-  const double rx[3] = {rx0, rx1, rx2};
   double s = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -308,7 +320,208 @@ __global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
   }
   double dd = sqrt(s) * 2;
   if (is_ti) dd += 0.15;
-  dist[p] = dd;
+  return dd;
+}
+
+// Simulation blockIdx.y of a batch: its antenna, direction, offset and
+// receiver; returns the offset of its component in the grid.
+__device__ __forceinline__ int64_t apply_sim(const SimParam &q, RayGeom &g, double (&rx)[3],
+                                             int64_t comp_stride) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    g.eye[a] = q.eye[a];
+    g.rxd[a] = q.rxd[a];
+    rx[a] = q.rx[a];
+  }
+  g.tau = q.tau;
+  return (int64_t)q.comp * comp_stride;
+}
+
+// Per ray: march, refine and the FP64 round trip; NaN marks a ray without a
+// hit.  BATCH: simulation s = blockIdx.y takes its values from sp[s]
+// (apply_sim) and writes dist[s * rays + p]; a ray's arithmetic is the same
+// either way.
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
+                                                     const float *__restrict__ grid, RayGeom g,
+                                                     float vsz, double rx0, double rx1,
+                                                     double rx2, int is_ti,
+                                                     const SimParam *__restrict__ sp,
+                                                     int64_t comp_stride) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= g.width * g.height) return;
+  double rx[3] = {rx0, rx1, rx2};
+  if constexpr (BATCH) {
+    grid += apply_sim(sp[blockIdx.y], g, rx, comp_stride);
+    dist += (int64_t)blockIdx.y * g.width * g.height;
+  }
+  float d[3], hit[3], vox[3];
+  ray_dir(g, p, d[0], d[1], d[2]);
+  intersect(grid, g, d[0], d[1], d[2], hit, vox);
+  // mask = voxel index x != -1
+  dist[p] = vox[0] == -1.0f ? (double)NAN : refine_dist(grid, g, vsz, rx, is_ti, d, hit, vox);
+}
+
+// ---------------------------------------------------------------------------
+// The offset-independent part of the march.  While a ray is outside the box,
+// or in a cell with an undefined corner, its step does not depend on the
+// isosurface offset tau: whether grid - tau < UNDEF_VAL + 10 at some corner
+// only turns from false to true as tau grows (the rounded difference falls
+// monotonically), so a cell undefined at the smallest offset tau_lo of a
+// search is undefined at every offset >= tau_lo, and its step is MIN_STEP.
+// k_ray_prefix marches each ray until its first step in a cell that is fully
+// defined at tau_lo and records that step and depth (RESUME), a hit reached
+// before it (HIT: the first step's box-distance quirk, or MIN_STEP below a
+// voxel), or the end of the march without a hit (NONE).  Resuming the march
+// from the recorded step for any tau >= tau_lo repeats exactly what the whole
+// march computes, and a NONE ray has no hit at any such tau.
+// ---------------------------------------------------------------------------
+constexpr int kStNone = 0, kStResume = 1, kStHit = 2;
+
+struct RayState {
+  float depth;
+  int code;  // status | step << 2
+};
+
+__device__ int march_prefix(const float *grid, const RayGeom &g, float dx, float dy, float dz,
+                            float tau_lo, float &depth) {
+#pragma clang fp contract(off)
+  const float vs = (g.bmax[1] - g.bmin[1]) / (float)(g.res[1] - 1);
+  depth = 0.0f;
+  for (int s = 0; s < kMaxSteps; ++s) {
+    const float r[3] = {g.eye[0] + depth * dx, g.eye[1] + depth * dy, g.eye[2] + depth * dz};
+    float dist;
+    if (!inside(g, r[0], r[1], r[2])) {
+      dist = s == 0 ? box_distance(g, r[0], r[1], r[2], dx, dy, dz) + 0.00001f : -1.0f;
+    } else {
+      float ix, iy, iz, c[8];
+      cell_at(grid, g, r[0], r[1], r[2], ix, iy, iz, c);
+      if (!any_undefined(c, tau_lo)) return kStResume | (s << 2);
+      dist = kMinStep;
+    }
+    if (dist == -1) return kStNone;
+    if (dist < vs && dist != 0.0f) return kStHit | (s << 2);
+    depth += dist;
+  }
+  return kStNone;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ray_prefix(RayState *__restrict__ st,
+                                                       const float *__restrict__ grid,
+                                                       RayGeom g, const SimParam *__restrict__ sp,
+                                                       int64_t comp_stride, float tau_lo) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const int n = g.width * g.height;
+  if (p >= n) return;
+  double rx[3];
+  grid += apply_sim(sp[blockIdx.y], g, rx, comp_stride);
+  float d[3], depth;
+  ray_dir(g, p, d[0], d[1], d[2]);
+  const int code = march_prefix(grid, g, d[0], d[1], d[2], tau_lo, depth);
+  st[(int64_t)blockIdx.y * n + p] = RayState{depth, code};
+}
+
+// The live rays (status RESUME or HIT) of each synthesis class q = c * 64 + l
+// (chunk c, lane l: rays r0 + l, r0 + l + 64, ... of the chunk, the order in
+// which k_ray_synth's lane l adds them): counted, scanned into offsets
+// off[s][q] (off[s][Q] = the simulation's live rays), listed in that order.
+__device__ __forceinline__ void class_range(int q, int n, int chunk, int &r, int &r1) {
+  const int c = q / kWave;
+  r = c * chunk + q % kWave;
+  r1 = min(n, c * chunk + chunk);
+}
+
+__global__ __launch_bounds__(kBlock) void k_live_count(int *__restrict__ off,
+                                                       const RayState *__restrict__ st, int n,
+                                                       int chunk, int Q) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= Q) return;
+  const RayState *sts = st + (int64_t)blockIdx.y * n;
+  int r, r1, k = 0;
+  class_range(q, n, chunk, r, r1);
+  for (; r < r1; r += kWave) k += (sts[r].code & 3) != kStNone;
+  off[(int64_t)blockIdx.y * (Q + 1) + q] = k;
+}
+
+// Exclusive scan of one simulation's Q counts (one workgroup of 1024 threads
+// per simulation, each thread a contiguous run); the total goes to off[Q]
+// and to tot[s].
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_live_scan(int *__restrict__ off,
+                                                            int *__restrict__ tot, int Q) {
+  __shared__ int part[kScanThreads];
+  int *o = off + (int64_t)blockIdx.x * (Q + 1);
+  const int per = (Q + kScanThreads - 1) / kScanThreads;
+  const int lo = min(Q, (int)threadIdx.x * per), hi = min(Q, lo + per);
+  int sum = 0;
+  for (int q = lo; q < hi; ++q) sum += o[q];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < kScanThreads; d <<= 1) {
+    const int v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - sum;  // exclusive prefix of this run
+  for (int q = lo; q < hi; ++q) {
+    const int v = o[q];
+    o[q] = run;
+    run += v;
+  }
+  if (threadIdx.x == kScanThreads - 1) {
+    o[Q] = part[kScanThreads - 1];
+    tot[blockIdx.x] = part[kScanThreads - 1];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_live_fill(int *__restrict__ list,
+                                                      const int *__restrict__ off,
+                                                      const RayState *__restrict__ st, int n,
+                                                      int chunk, int Q) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= Q) return;
+  const RayState *sts = st + (int64_t)blockIdx.y * n;
+  int *ls = list + (int64_t)blockIdx.y * n;
+  int r, r1, k = off[(int64_t)blockIdx.y * (Q + 1) + q];
+  class_range(q, n, chunk, r, r1);
+  for (; r < r1; r += kWave)
+    if ((sts[r].code & 3) != kStNone) ls[k++] = r;
+}
+
+// One offset of a search from the recorded prefix: each live ray t (of
+// off[s][Q]) resumes its march (or takes its recorded hit) with the
+// simulation's tau, refines and writes its round trip (NaN: no hit) to
+// dl[s * live_max + t].
+__global__ __launch_bounds__(kBlock) void k_ray_resume(double *__restrict__ dl,
+                                                       const float *__restrict__ grid,
+                                                       RayGeom g, float vsz, int is_ti,
+                                                       const SimParam *__restrict__ sp,
+                                                       int64_t comp_stride,
+                                                       const RayState *__restrict__ st,
+                                                       const int *__restrict__ list,
+                                                       const int *__restrict__ off, int Q,
+                                                       int live_max) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int n = g.width * g.height;
+  const int live = min(off[(int64_t)blockIdx.y * (Q + 1) + Q], live_max);
+  if (t >= live) return;
+  double rx[3];
+  grid += apply_sim(sp[blockIdx.y], g, rx, comp_stride);
+  const int r = min(max(list[(int64_t)blockIdx.y * n + t], 0), n - 1);
+  const RayState q = st[(int64_t)blockIdx.y * n + r];
+  float d[3], hit[3], vox[3];
+  ray_dir(g, r, d[0], d[1], d[2]);
+  if ((q.code & 3) == kStHit) {
+    const float p3[3] = {g.eye[0] + q.depth * d[0], g.eye[1] + q.depth * d[1],
+                         g.eye[2] + q.depth * d[2]};
+    hit_at(g, p3, hit, vox);
+  } else {
+    intersect_from(grid, g, d[0], d[1], d[2], q.depth, q.code >> 2, hit, vox);
+  }
+  dl[(int64_t)blockIdx.y * live_max + t] =
+      vox[0] == -1.0f ? (double)NAN : refine_dist(grid, g, vsz, rx, is_ti, d, hit, vox);
 }
 
 // Channel synthesis: out[f] = sum over hit rays of exp(+j 2 pi (f0 + f df) d / c).
@@ -316,6 +529,50 @@ __global__ __launch_bounds__(kBlock) void k_ray_dist(double *__restrict__ dist,
 // and keep kK complex sums, from one FP64-reduced anchor and the two-term
 // recurrence z_{t+1} = 2 cos(w) z_t - z_{t-1} per ray; a butterfly reduces
 // the wave and the partials go to the workspace.
+//
+// One ray's kK terms added to the lane's sums (w = 1 for a hit; a miss, w =
+// 0, adds signed zeros, which leave every sum as it is: the sums start at +0
+// and +0 + -0 = +0).
+__device__ __forceinline__ void add_ray(double d, bool ok, double fc, double dfc,
+                                        float (&accr)[kK], float (&acci)[kK]) {
+#pragma clang fp contract(off)
+  const double dd = ok ? d : 0.0;
+  float ar, ai, rr, ri;
+  phasor_hw(dd * fc, ar, ai);
+  phasor_hw(dd * dfc, rr, ri);
+  const float w = ok ? 1.0f : 0.0f;
+  float zr = w * ar, zi = -w * ai;          // exp(+j ...)
+  const float sr = rr, si = -ri;
+  float pr = zr * sr - zi * si, pi_ = zr * si + zi * sr;
+  const float c2 = sr + sr;
+#pragma unroll
+  for (int t = 0; t < kK; ++t) {
+    accr[t] += zr;
+    acci[t] += zi;
+    const float nr = fmaf(c2, pr, -zr), ni = fmaf(c2, pi_, -zi);
+    zr = pr;
+    zi = pi_;
+    pr = nr;
+    pi_ = ni;
+  }
+}
+
+// The wave's sums (butterfly), sample t to lane t.
+__device__ __forceinline__ void store_wave(float2 *__restrict__ out, int lane, int nf,
+                                           float (&accr)[kK], float (&acci)[kK]) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+#pragma unroll
+    for (int t = 0; t < kK; ++t) {
+      accr[t] += __shfl_xor(accr[t], o, kWave);
+      acci[t] += __shfl_xor(acci[t], o, kWave);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kK; ++t)
+    if (lane == t && t < nf) out[t] = make_float2(accr[t], acci[t]);
+}
+
 __global__ __launch_bounds__(kBlock) void k_ray_synth(float2 *__restrict__ part,
                                                       const double *__restrict__ dist,
                                                       int nrays, int chunk, double f0c,
@@ -340,40 +597,53 @@ __global__ __launch_bounds__(kBlock) void k_ray_synth(float2 *__restrict__ part,
   const int r1 = r0 + min(chunk, nrays - r0);
   for (int r = r0 + lane; r < r1; r += kWave) {
     const double d = dist[r];
-    const bool ok = d == d;  // hit
-    const double dd = ok ? d : 0.0;
-    float ar, ai, rr, ri;
-    phasor_hw(dd * fc, ar, ai);
-    phasor_hw(dd * dfc, rr, ri);
-    const float w = ok ? 1.0f : 0.0f;
-    float zr = w * ar, zi = -w * ai;          // exp(+j ...)
-    const float sr = rr, si = -ri;
-    float pr = zr * sr - zi * si, pi_ = zr * si + zi * sr;
-    const float c2 = sr + sr;
+    add_ray(d, d == d, fc, dfc, accr, acci);
+  }
+  store_wave(part + (size_t)c * F + f_lo, lane, live ? nf : 0, accr, acci);
+}
+
+// The same sums over the live rays only, from the prefix's lists: lane l of
+// chunk c adds the hits of its class (c, l) in the order k_ray_synth's lane
+// would, and the misses it skips add signed zeros there, so the partials are
+// k_ray_synth's bit for bit; a wave without a hit writes its +0 partials
+// directly.
+__global__ __launch_bounds__(kBlock) void k_ray_synth_live(float2 *__restrict__ part,
+                                                           const double *__restrict__ dl,
+                                                           const int *__restrict__ off, int Q,
+                                                           int live_max, double f0c,
+                                                           double dfc, int F, int nfb,
+                                                           int64_t nitems, int nchunks) {
+  dl += (int64_t)blockIdx.y * live_max;
+  off += (int64_t)blockIdx.y * (Q + 1);
+  part += (int64_t)blockIdx.y * nchunks * F;
+  const int lane = threadIdx.x % kWave;
+  const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const bool live = item < nitems;
+  const int64_t it = live ? item : nitems - 1;
+  const int fb = (int)(it % nfb);
+  const int c = (int)(it / nfb);
+  const int f_lo = fb * kK;
+  const int nf = min(kK, F - f_lo);
+  const double fc = fma((double)f_lo, dfc, f0c);
+  float accr[kK], acci[kK];
 #pragma unroll
-    for (int t = 0; t < kK; ++t) {
-      accr[t] += zr;
-      acci[t] += zi;
-      const float nr = fmaf(c2, pr, -zr), ni = fmaf(c2, pi_, -zi);
-      zr = pr;
-      zi = pi_;
-      pr = nr;
-      pi_ = ni;
+  for (int t = 0; t < kK; ++t) accr[t] = acci[t] = 0.0f;
+  const int q = c * kWave + lane;
+  const int t1 = min(off[q + 1], live_max);
+  bool any = false;
+  for (int t = min(off[q], t1); t < t1; ++t) {
+    const double d = dl[t];
+    if (d == d) {
+      add_ray(d, true, fc, dfc, accr, acci);
+      any = true;
     }
   }
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-#pragma unroll
-    for (int t = 0; t < kK; ++t) {
-      accr[t] += __shfl_xor(accr[t], o, kWave);
-      acci[t] += __shfl_xor(acci[t], o, kWave);
-    }
+  float2 *out = part + (size_t)c * F + f_lo;
+  if (__ballot(any) == 0) {
+    if (live && lane < nf) out[lane] = make_float2(0.0f, 0.0f);
+    return;
   }
-  if (live) {
-#pragma unroll
-    for (int t = 0; t < kK; ++t)
-      if (lane == t && t < nf) part[(size_t)c * F + f_lo + t] = make_float2(accr[t], acci[t]);
-  }
+  store_wave(out, lane, live ? nf : 0, accr, acci);
 }
 
 __global__ __launch_bounds__(kBlock) void k_synth_reduce(float2 *__restrict__ out,
@@ -481,6 +751,12 @@ bool geom_from(RayGeom &g, int width, int height, float fov_w, float fov_h,
   g.width = width;
   g.height = height;
   g.tau = tau;
+  const int64_t v = (int64_t)res[0] * res[1] * res[2];
+  g.small = v <= ((int64_t)1 << 24);
+  // MMN_RAY_FP32_INDEX=1 keeps the FP32 index arithmetic for every grid (the
+  // two forms give the same indices; read at each call)
+  const char *e = getenv("MMN_RAY_FP32_INDEX");
+  if (e && e[0] == '1') g.small = 0;
   return true;
 }
 
@@ -637,6 +913,195 @@ extern "C" int mmn_simulate_channels(float *channels, const float *grid, int n_c
   hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock), (unsigned)n_sim),
                      dim3(kBlock), 0, st, (float2 *)channels, part, nch, samples);
   return mmn_check_launch("mmn_simulate_channels");
+}
+
+namespace {
+
+struct PrefixLayout {
+  SimParam *sp;
+  RayState *st;
+  int *off, *list, *tot;
+  size_t bytes;
+};
+
+PrefixLayout prefix_layout(int n, int n_sim, char *base) {
+  const int64_t Q = (int64_t)synth_chunks(n) * kWave;
+  PrefixLayout L;
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    char *q = base ? base + o : nullptr;
+    o += align256(b);
+    return q;
+  };
+  L.sp = (SimParam *)take(sizeof(SimParam) * (size_t)n_sim);
+  L.st = (RayState *)take(sizeof(RayState) * (size_t)n * n_sim);
+  L.off = (int *)take(sizeof(int) * (size_t)(Q + 1) * n_sim);
+  L.list = (int *)take(sizeof(int) * (size_t)n * n_sim);
+  L.tot = (int *)take(sizeof(int) * (size_t)n_sim);
+  L.bytes = o;
+  return L;
+}
+
+// Host checks shared by the batched entry points.
+int check_batch(const char *what, int n_sim, int n_comp, const int32_t *comp,
+                const float *bounding_box, const int *grid_res, int width, int height,
+                float width_fov, float height_fov, RayGeom &g) {
+  MMN_REQUIRE(n_sim > 0 && n_sim <= kMaxSims, MMN_ERR_ARG, "%s: n_sim must be in [1, %d]",
+              what, kMaxSims);
+  const float zero3[3] = {0.0f, 0.0f, 0.0f};
+  MMN_REQUIRE(n_comp > 0 && geom_from(g, width, height, width_fov, height_fov, bounding_box,
+                                      grid_res, zero3, zero3, 0.0f),
+              MMN_ERR_ARG, "%s: bad sizes", what);
+  for (int s = 0; s < n_sim; ++s)
+    MMN_REQUIRE(comp[s] >= 0 && comp[s] < n_comp, MMN_ERR_ARG,
+                "%s: comp[%d] = %d outside [0, %d)", what, s, comp[s], n_comp);
+  return 0;
+}
+
+int copy_params(const char *what, SimParam *dst, int n_sim, const int32_t *comp,
+                const float *taus, const float *eyes, const float *rx_dirs,
+                const double *rx_locs, hipStream_t st) {
+  std::vector<SimParam> prm((size_t)n_sim);
+  for (int s = 0; s < n_sim; ++s) {
+    SimParam &q = prm[(size_t)s];
+    for (int a = 0; a < 3; ++a) {
+      q.eye[a] = eyes[3 * s + a];
+      q.rxd[a] = rx_dirs[3 * s + a];
+      q.rx[a] = rx_locs ? rx_locs[3 * s + a] : 0.0;
+    }
+    q.tau = taus ? taus[s] : 0.0f;
+    q.comp = comp[s];
+    q.pad = 0.0;
+  }
+  const hipError_t e = hipMemcpyAsync(dst, prm.data(), sizeof(SimParam) * (size_t)n_sim,
+                                      hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
+    mmn_set_error("%s: parameter copy: %s", what, hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" size_t mmn_ray_prefix_size(int width, int height, int n_sim) {
+  if (width <= 0 || height <= 0 || n_sim <= 0 || (int64_t)width * height > INT32_MAX)
+    return 0;
+  return prefix_layout(width * height, n_sim, nullptr).bytes;
+}
+
+extern "C" int mmn_ray_prefix(void *prefix, size_t prefix_bytes, int *live_counts,
+                              const float *grid, int n_comp, const int32_t *comp,
+                              const float *eyes, const float *rx_dirs, int n_sim, int width,
+                              int height, float width_fov, float height_fov,
+                              const float *bounding_box, const int *grid_res, float tau_lo,
+                              void *stream) {
+  MMN_REQUIRE(prefix && live_counts && grid && comp && eyes && rx_dirs && bounding_box &&
+                  grid_res,
+              MMN_ERR_NULL, "mmn_ray_prefix: NULL pointer argument");
+  RayGeom g;
+  int rc = check_batch("mmn_ray_prefix", n_sim, n_comp, comp, bounding_box, grid_res, width,
+                       height, width_fov, height_fov, g);
+  if (rc) return rc;
+  MMN_REQUIRE(std::isfinite(tau_lo), MMN_ERR_ARG, "mmn_ray_prefix: tau_lo must be finite");
+  const int n = width * height;
+  PrefixLayout L = prefix_layout(n, n_sim, (char *)prefix);
+  MMN_REQUIRE(prefix_bytes >= L.bytes, MMN_ERR_WS,
+              "mmn_ray_prefix: prefix buffer of %zu bytes needed", L.bytes);
+  hipStream_t st = (hipStream_t)stream;
+  rc = copy_params("mmn_ray_prefix", L.sp, n_sim, comp, nullptr, eyes, rx_dirs, nullptr, st);
+  if (rc) return rc;
+  const int64_t vox = (int64_t)grid_res[0] * grid_res[1] * grid_res[2];
+  const int nch = synth_chunks(n), chunk = (n + nch - 1) / nch, Q = nch * kWave;
+  hipLaunchKernelGGL(k_ray_prefix, dim3(mmn_blocks(n, kBlock), (unsigned)n_sim), dim3(kBlock),
+                     0, st, L.st, grid, g, (const SimParam *)L.sp, vox, tau_lo);
+  hipLaunchKernelGGL(k_live_count, dim3(mmn_blocks(Q, kBlock), (unsigned)n_sim), dim3(kBlock),
+                     0, st, L.off, (const RayState *)L.st, n, chunk, Q);
+  hipLaunchKernelGGL(k_live_scan, dim3((unsigned)n_sim), dim3(kScanThreads), 0, st, L.off,
+                     L.tot, Q);
+  hipLaunchKernelGGL(k_live_fill, dim3(mmn_blocks(Q, kBlock), (unsigned)n_sim), dim3(kBlock),
+                     0, st, L.list, (const int *)L.off, (const RayState *)L.st, n, chunk, Q);
+  rc = mmn_check_launch("mmn_ray_prefix");
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(live_counts, L.tot, sizeof(int) * (size_t)n_sim,
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    mmn_set_error("mmn_ray_prefix: live counts: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
+}
+
+extern "C" size_t mmn_simulate_channels_prefixed_workspace_size(int width, int height,
+                                                                int samples, int n_sim,
+                                                                int live_max) {
+  if (width <= 0 || height <= 0 || samples <= 0 || n_sim <= 0 || live_max < 0 ||
+      (int64_t)width * height > INT32_MAX)
+    return 0;
+  const int n = width * height;
+  return align256(sizeof(SimParam) * (size_t)n_sim) +
+         align256(sizeof(double) * (size_t)std::max(live_max, 1) * n_sim) +
+         align256(sizeof(float2) * (size_t)synth_chunks(n) * samples * n_sim);
+}
+
+extern "C" int mmn_simulate_channels_prefixed(
+    float *channels, const void *prefix, size_t prefix_bytes, const int *live_counts,
+    const float *grid, int n_comp, const int32_t *comp, const float *taus, const float *eyes,
+    const float *rx_dirs, const double *rx_locs, int n_sim, int width, int height,
+    float width_fov, float height_fov, const float *bounding_box, const int *grid_res,
+    float voxel_size, double f0, double df, int samples, int is_ti_radar, float tau_lo,
+    void *workspace, size_t workspace_bytes, void *stream) {
+  MMN_REQUIRE(channels && prefix && live_counts && grid && comp && taus && eyes && rx_dirs &&
+                  rx_locs && bounding_box && grid_res,
+              MMN_ERR_NULL, "mmn_simulate_channels_prefixed: NULL pointer argument");
+  RayGeom g;
+  int rc = check_batch("mmn_simulate_channels_prefixed", n_sim, n_comp, comp, bounding_box,
+                       grid_res, width, height, width_fov, height_fov, g);
+  if (rc) return rc;
+  MMN_REQUIRE(samples > 0, MMN_ERR_ARG, "mmn_simulate_channels_prefixed: samples <= 0");
+  const int n = width * height;
+  int live_max = 0;
+  for (int s = 0; s < n_sim; ++s) {
+    MMN_REQUIRE(taus[s] >= tau_lo, MMN_ERR_ARG,
+                "mmn_simulate_channels_prefixed: taus[%d] below the prefix's tau_lo", s);
+    MMN_REQUIRE(live_counts[s] >= 0 && live_counts[s] <= n, MMN_ERR_ARG,
+                "mmn_simulate_channels_prefixed: live_counts[%d] outside [0, rays]", s);
+    live_max = std::max(live_max, live_counts[s]);
+  }
+  const PrefixLayout L = prefix_layout(n, n_sim, (char *)prefix);
+  MMN_REQUIRE(prefix_bytes >= L.bytes, MMN_ERR_WS,
+              "mmn_simulate_channels_prefixed: prefix of %zu bytes expected", L.bytes);
+  const size_t need = mmn_simulate_channels_prefixed_workspace_size(width, height, samples,
+                                                                    n_sim, live_max);
+  MMN_REQUIRE(workspace && workspace_bytes >= need, MMN_ERR_WS,
+              "mmn_simulate_channels_prefixed: workspace of %zu bytes needed", need);
+  char *w = (char *)workspace;
+  SimParam *sp = (SimParam *)w;
+  double *dl = (double *)(w + align256(sizeof(SimParam) * (size_t)n_sim));
+  float2 *part = (float2 *)((char *)dl +
+                            align256(sizeof(double) * (size_t)std::max(live_max, 1) * n_sim));
+  hipStream_t st = (hipStream_t)stream;
+  rc = copy_params("mmn_simulate_channels_prefixed", sp, n_sim, comp, taus, eyes, rx_dirs,
+                   rx_locs, st);
+  if (rc) return rc;
+  const int64_t vox = (int64_t)grid_res[0] * grid_res[1] * grid_res[2];
+  const int nch = synth_chunks(n), Q = nch * kWave;
+  const int lm = std::max(live_max, 1);
+  hipLaunchKernelGGL(k_ray_resume, dim3(mmn_blocks(lm, kBlock), (unsigned)n_sim), dim3(kBlock),
+                     0, st, dl, grid, g, voxel_size, is_ti_radar, (const SimParam *)sp, vox,
+                     (const RayState *)L.st, (const int *)L.list, (const int *)L.off, Q, lm);
+  const int nfb = (samples + kK - 1) / kK;
+  const int64_t nitems = (int64_t)nfb * nch;
+  const double inv_c = 1.0 / MMN_C0;
+  hipLaunchKernelGGL(k_ray_synth_live,
+                     dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock),
+                          (unsigned)n_sim),
+                     dim3(kBlock), 0, st, part, (const double *)dl, (const int *)L.off, Q, lm,
+                     f0 * inv_c, df * inv_c, samples, nfb, nitems, nch);
+  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock), (unsigned)n_sim),
+                     dim3(kBlock), 0, st, (float2 *)channels, part, nch, samples);
+  return mmn_check_launch("mmn_simulate_channels_prefixed");
 }
 
 extern "C" int mmn_channel_loss(float *loss, const float *sim, const float *target,

@@ -124,22 +124,42 @@ def _simulate_channels(bounding_box, bounding_box_lower, voxel_size, grid_res_x,
     return combined
 
 
+def _host_rows(a):
+    return np.asarray(a.detach().cpu().numpy() if torch.is_tensor(a) else a,
+                      dtype=np.float64).reshape(-1, 3)
+
+
+def _channel_prefix(bounding_box, width, height, width_fov, height_fov, grid_cm, rx_locs,
+                    rx_dirs, tau_bounds):
+    """ops.ray_prefix for the simulations of _simulate_channel_batch (antenna
+    r, component k in that order) and every offset the clip to ``tau_bounds``
+    can give (>= the smallest lower bound)."""
+    n_comp = grid_cm.shape[0]
+    rx, rd = _host_rows(rx_locs), _host_rows(rx_dirs)
+    tau_lo = min(float(np.float32(_scalar(torch.as_tensor(tau_bounds[k][0],
+                                                          dtype=torch.float32))))
+                 for k in range(n_comp))
+    return ops.ray_prefix(grid_cm, np.tile(np.arange(n_comp, dtype=np.int32), len(rx)),
+                          np.repeat(rx.astype(np.float32), n_comp, axis=0),
+                          np.repeat(rd.astype(np.float32), n_comp, axis=0), int(width),
+                          int(height), float(width_fov), float(height_fov),
+                          [float(v) for v in bounding_box], tau_lo)
+
+
 def _simulate_channel_batch(bounding_box, voxel_size, width, height, width_fov, height_fov,
                             grid_cm, rx_locs, rx_dirs, wavelengths, taus, tau_bounds,
-                            is_real_data):
+                            is_real_data, prefix=None):
     """_simulate_channels for several antennas at once: row r is the combined
     channel of antenna (rx_locs[r], rx_dirs[r]) over every component of
     ``grid_cm`` (n_comp, NX, NY, NZ) with the clipped offsets ``taus``, bit for
     bit what _simulate_channels returns for it (the same per-simulation values,
     the components added to zeros in order), from one batched launch
-    (ops.simulate_channels)."""
+    (ops.simulate_channels), or, given the ``prefix`` of _channel_prefix for
+    the same antennas, from its live rays (ops.simulate_channels_prefixed)."""
     n_comp = grid_cm.shape[0]
     n_wl = int(wavelengths.shape[0])
     f0, df = _freqs(wavelengths, n_wl)
-    rx = np.asarray(rx_locs.detach().cpu().numpy() if torch.is_tensor(rx_locs) else rx_locs,
-                    dtype=np.float64).reshape(-1, 3)
-    rd = np.asarray(rx_dirs.detach().cpu().numpy() if torch.is_tensor(rx_dirs) else rx_dirs,
-                    dtype=np.float64).reshape(-1, 3)
+    rx, rd = _host_rows(rx_locs), _host_rows(rx_dirs)
     n_ant = len(rx)
     tv = np.empty(n_comp, dtype=np.float32)
     for k in range(n_comp):
@@ -150,11 +170,16 @@ def _simulate_channel_batch(bounding_box, voxel_size, width, height, width_fov, 
     comp = np.tile(np.arange(n_comp, dtype=np.int32), n_ant)
     rep = lambda a: np.repeat(a, n_comp, axis=0)
     vsz = float(np.float32(voxel_size.item() if torch.is_tensor(voxel_size) else voxel_size))
-    sims = ops.simulate_channels(grid_cm, comp, np.tile(tv, n_ant), rep(rx.astype(np.float32)),
-                                 rep(rd.astype(np.float32)), rep(rx), int(width), int(height),
-                                 float(width_fov), float(height_fov),
-                                 [float(v) for v in bounding_box], vsz, f0, df, n_wl,
-                                 is_ti=is_real_data).view(n_ant, n_comp, n_wl)
+    if prefix is not None:
+        sims = ops.simulate_channels_prefixed(prefix, np.tile(tv, n_ant), rep(rx), vsz, f0, df,
+                                              n_wl, is_ti=is_real_data)
+    else:
+        sims = ops.simulate_channels(grid_cm, comp, np.tile(tv, n_ant),
+                                     rep(rx.astype(np.float32)), rep(rd.astype(np.float32)),
+                                     rep(rx), int(width), int(height), float(width_fov),
+                                     float(height_fov), [float(v) for v in bounding_box], vsz,
+                                     f0, df, n_wl, is_ti=is_real_data)
+    sims = sims.view(n_ant, n_comp, n_wl)
     combined = torch.zeros((n_ant, n_wl), dtype=torch.complex64, device=grid_cm.device)
     for k in range(n_comp):
         combined += sims[:, k]
@@ -289,12 +314,15 @@ def search_taus(sdf, rx_locs, rx_dirs, channels, bounding_box, wavelengths, voxe
             else np.asarray(rx_locs))[sel]
     dirs = (rx_dirs.detach().cpu().numpy() if torch.is_tensor(rx_dirs)
             else np.asarray(rx_dirs))[sel]
+    # the march up to the first offset-dependent step, once for the search
+    prefix = _channel_prefix(bounding_box, width, height, width_fov, height_fov, grid_cm,
+                             locs, dirs, tau_bounds)
     for flat_i in range(int(np.prod(shape))):
         idx = np.unravel_index(flat_i, shape)
         taus = tau_grid[idx]
         sims = _simulate_channel_batch(bounding_box, voxel_size, width, height, width_fov,
                                        height_fov, grid_cm, locs, dirs, wl, taus, tau_bounds,
-                                       is_real_data)
+                                       is_real_data, prefix=prefix)
         losses[idx] = batch_loss(sims, tgt, is_real_data).sum().cpu()
     taus = _select_tau(tau_grid.numpy(), losses.numpy())
     return taus, tau_grid, losses

@@ -508,10 +508,7 @@ def simulate_channels(grid, comp, taus, eyes, rx_dirs, rx_locs, width, height, w
     lib = _lib.load()
     res = (ctypes.c_int * 3)(*grid.shape[1:])
     bb = _f32s(bounding_box, 6)
-    rays = int(width) * int(height)
-    per = max(1, min(65535, MAX_BATCH_RAYS // max(rays, 1)))
-    for lo in range(0, n, per):
-        hi = min(n, lo + per)
+    for lo, hi in _batch_groups(n, int(width) * int(height)):
         ws_bytes = int(lib.mmn_simulate_channels_workspace_size(int(width), int(height),
                                                                 int(samples), hi - lo))
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
@@ -521,6 +518,98 @@ def simulate_channels(grid, comp, taus, eyes, rx_dirs, rx_locs, width, height, w
                   rx_locs[lo:hi].ctypes.data, hi - lo, int(width), int(height),
                   float(width_fov), float(height_fov), bb, res, float(voxel_size), float(f0),
                   float(df), int(samples), int(bool(is_ti)), ws.data_ptr(), ws_bytes,
+                  _lib.stream_handle(dev))
+    return out
+
+
+class RayPrefix:
+    """The offset-independent march of a set of simulations (mmn_ray_prefix):
+    per launch group of simulations, the device buffer and the host live-ray
+    counts, with the simulations' components, antennas and directions and the
+    geometry they were marched with."""
+
+    def __init__(self, grid, comp, eyes, rx_dirs, width, height, width_fov, height_fov,
+                 bounding_box, tau_lo, groups):
+        self.grid, self.comp, self.eyes, self.rx_dirs = grid, comp, eyes, rx_dirs
+        self.width, self.height = int(width), int(height)
+        self.width_fov, self.height_fov = float(width_fov), float(height_fov)
+        self.bounding_box = [float(v) for v in bounding_box]
+        self.tau_lo = float(np.float32(tau_lo))
+        self.groups = groups  # [(lo, hi, buffer, live_counts)]
+
+    @property
+    def live_rays(self):
+        return int(sum(int(g[3].sum()) for g in self.groups))
+
+
+def _batch_groups(n, rays):
+    per = max(1, min(65535, MAX_BATCH_RAYS // max(rays, 1)))
+    return [(lo, min(n, lo + per)) for lo in range(0, n, per)]
+
+
+def ray_prefix(grid, comp, eyes, rx_dirs, width, height, width_fov, height_fov, bounding_box,
+               tau_lo):
+    """March the rays of simulations (comp[s], eyes[s], rx_dirs[s]) as far as
+    no offset >= ``tau_lo`` changes them (mmn_ray_prefix); the result feeds
+    simulate_channels_prefixed for any such offsets."""
+    _lib.require_gpu()
+    _check(grid, "grid", torch.float32, 4)
+    dev = grid.device
+    comp = np.ascontiguousarray(comp, dtype=np.int32).reshape(-1)
+    n = len(comp)
+    eyes = np.ascontiguousarray(eyes, dtype=np.float32).reshape(n, 3)
+    rx_dirs = np.ascontiguousarray(rx_dirs, dtype=np.float32).reshape(n, 3)
+    if n and (comp.min() < 0 or comp.max() >= grid.shape[0]):
+        raise ValueError(f"comp must lie in [0, {grid.shape[0]})")
+    if not np.isfinite(tau_lo):
+        raise ValueError("tau_lo must be finite")
+    lib = _lib.load()
+    res = (ctypes.c_int * 3)(*grid.shape[1:])
+    bb = _f32s(bounding_box, 6)
+    groups = []
+    for lo, hi in _batch_groups(n, int(width) * int(height)):
+        nb = int(lib.mmn_ray_prefix_size(int(width), int(height), hi - lo))
+        buf = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
+        live = np.zeros(hi - lo, dtype=np.int32)
+        _lib.call("mmn_ray_prefix", buf.data_ptr(), nb, live.ctypes.data, grid.data_ptr(),
+                  int(grid.shape[0]), comp[lo:hi].ctypes.data, eyes[lo:hi].ctypes.data,
+                  rx_dirs[lo:hi].ctypes.data, hi - lo, int(width), int(height),
+                  float(width_fov), float(height_fov), bb, res, float(np.float32(tau_lo)),
+                  _lib.stream_handle(dev))
+        groups.append((lo, hi, buf, live))
+    return RayPrefix(grid, comp, eyes, rx_dirs, width, height, width_fov, height_fov,
+                     bounding_box, tau_lo, groups)
+
+
+def simulate_channels_prefixed(prefix, taus, rx_locs, voxel_size, f0, df, samples,
+                               is_ti=False):
+    """simulate_channels for the simulations of ``prefix`` with offsets
+    ``taus[s]`` (each >= prefix.tau_lo) and receivers ``rx_locs[s]``, bit for
+    bit, marching and summing only the rays the prefix left live."""
+    _lib.require_gpu()
+    n = len(prefix.comp)
+    taus = np.ascontiguousarray(taus, dtype=np.float32).reshape(n)
+    rx_locs = np.ascontiguousarray(rx_locs, dtype=np.float64).reshape(n, 3)
+    if n and taus.min() < np.float32(prefix.tau_lo):
+        raise ValueError(f"taus must be >= the prefix's tau_lo ({prefix.tau_lo})")
+    grid = prefix.grid
+    dev = grid.device
+    out = torch.empty((n, int(samples)), dtype=torch.complex64, device=dev)
+    lib = _lib.load()
+    res = (ctypes.c_int * 3)(*grid.shape[1:])
+    bb = _f32s(prefix.bounding_box, 6)
+    W, H = prefix.width, prefix.height
+    for lo, hi, buf, live in prefix.groups:
+        ws_bytes = int(lib.mmn_simulate_channels_prefixed_workspace_size(
+            W, H, int(samples), hi - lo, int(live.max()) if len(live) else 0))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.call("mmn_simulate_channels_prefixed", out[lo:hi].data_ptr(), buf.data_ptr(),
+                  buf.numel(), live.ctypes.data, grid.data_ptr(), int(grid.shape[0]),
+                  prefix.comp[lo:hi].ctypes.data, taus[lo:hi].ctypes.data,
+                  prefix.eyes[lo:hi].ctypes.data, prefix.rx_dirs[lo:hi].ctypes.data,
+                  rx_locs[lo:hi].ctypes.data, hi - lo, W, H, prefix.width_fov,
+                  prefix.height_fov, bb, res, float(voxel_size), float(f0), float(df),
+                  int(samples), int(bool(is_ti)), prefix.tau_lo, ws.data_ptr(), ws_bytes,
                   _lib.stream_handle(dev))
     return out
 

@@ -249,3 +249,70 @@ def test_search_taus_matches_per_call_loop_and_finds_generating_offset():
     assert torch.equal(tau_grid[:, 0], tau_axis)
     assert float(losses[2]) == 0.0
     assert float(torch.min(torch.cat([losses[:2], losses[3:]]))) > 0.0
+
+
+@pytest.mark.parametrize("res", [(40, 40, 40), (30, 50, 20)])
+def test_integer_corner_indices_equal_fp32_indices(monkeypatch, res):
+    """The sphere trace's corner indices in integers (grids up to 2^24
+    voxels) against the reference's FP32 index arithmetic
+    (MMN_RAY_FP32_INDEX=1): the same hits, voxels and channels, bit for bit,
+    also on a grid whose x extent exceeds what the y-derived voxel size
+    spans (indices past the axis wrap or clamp the same)."""
+    grid, bbox = sphere_scene(res=res)
+    if res[0] != res[1]:
+        bbox = list(bbox)
+        bbox[3] += 0.01          # x extent beyond (res_x - 1) voxels of the y size
+    gt = torch.from_numpy(grid).to(dev())
+    eye, rxd = [0.003, -0.002, 0.3], (0.02, -0.01, 1.0)
+    W, H, F = 96, 80, 64
+
+    def run():
+        rm = ops.ray_matching(gt, W, H, 12.0, 10.0, bbox, eye, rxd)
+        ch = ops.simulate_channel(gt, 0.0013, W, H, 12.0, 10.0, bbox, eye, rxd, eye, VS, 77e9,
+                                  4e9 / (F - 1), F, is_ti=True)
+        return [t.cpu() for t in rm] + [ch.cpu()]
+
+    fast = run()
+    monkeypatch.setenv("MMN_RAY_FP32_INDEX", "1")
+    ref = run()
+    assert (fast[1][..., 0] != -1).sum() > 300        # rays hit the sphere
+    for a, b in zip(fast, ref):
+        assert torch.equal(a, b)
+
+
+def test_prefixed_channels_equal_full_march(monkeypatch):
+    """The offset-independent prefix (mmn_ray_prefix) and the resumed march of
+    the live rays (mmn_simulate_channels_prefixed) give mmn_simulate_channels'
+    channels bit for bit for offsets >= tau_lo: antennas above, inside and
+    within a voxel of the box (the first step's box-distance hit), a band of
+    corners just below the undefined threshold (undefined for some offsets of
+    the range and not for others), several launch groups."""
+    gcm, bbox = two_component_grid()
+    gcm = gcm.copy()
+    band = (gcm[0] > 0.004) & (gcm[0] < 0.0055)
+    gcm[0][band] = -40.0 + 0.0015          # undefined iff tau > 0.0015
+    gt = torch.from_numpy(gcm).to(dev())
+    eyes = np.array([[0.002, -0.001, 0.3], [0.01, 0.004, 0.25], [-0.006, 0.003, 0.07],
+                     [0.0, 0.002, bbox[5] + 0.0005], [0.004, -0.008, 0.28]])
+    dirs = np.array([[0.0, 0.0, 1.0], [0.05, -0.02, 1.0], [0.0, 0.0, 1.0], [0.0, 0.0, 1.0],
+                     [-0.03, 0.04, 1.0]])
+    n = len(eyes)
+    comp = np.array([0, 1, 0, 1, 0])
+    W, H, F = 48, 40, 96
+    f0, df = 77e9, 4e9 / (F - 1)
+    monkeypatch.setattr(ops, "MAX_BATCH_RAYS", 2 * W * H)     # groups of 2, 2, 1
+    tau_lo = -0.003
+    pre = ops.ray_prefix(gt, comp, eyes, dirs, W, H, 12.0, 10.0, bbox, tau_lo)
+    assert len(pre.groups) == 3
+    assert 0 < pre.live_rays < n * W * H
+    for taus in (np.full(n, tau_lo), np.array([0.001, 0.0, 0.002, -0.001, 0.0016]),
+                 np.full(n, 0.003)):
+        taus = taus.astype(np.float32)
+        want = ops.simulate_channels(gt, comp, taus, eyes, dirs, eyes, W, H, 12.0, 10.0, bbox,
+                                     VS, f0, df, F, is_ti=True)
+        got = ops.simulate_channels_prefixed(pre, taus, eyes, VS, f0, df, F, is_ti=True)
+        assert torch.count_nonzero(want) > 0
+        assert torch.equal(got, want)
+    with pytest.raises(ValueError):
+        ops.simulate_channels_prefixed(pre, np.full(n, -0.004, np.float32), eyes, VS, f0, df,
+                                       F)
