@@ -602,60 +602,71 @@ __global__ __launch_bounds__(kBlock) void k_ray_synth(float2 *__restrict__ part,
   store_wave(part + (size_t)c * F + f_lo, lane, live ? nf : 0, accr, acci);
 }
 
-// The same sums over the live rays only, from the prefix's lists: lane l of
-// chunk c adds the hits of its class (c, l) in the order k_ray_synth's lane
-// would, and the misses it skips add signed zeros there, so the partials are
-// k_ray_synth's bit for bit; a wave without a hit writes its +0 partials
-// directly.
+// The same sums over the live rays only, from the prefix's lists: one wave
+// per chunk c and simulation; lane l adds the hits of its class (c, l) in
+// the order k_ray_synth's lane would, once per block of kK samples, and the
+// misses it skips add signed zeros there, so the partials are k_ray_synth's
+// bit for bit.  A chunk without a hit writes its +0 partials directly.
 __global__ __launch_bounds__(kBlock) void k_ray_synth_live(float2 *__restrict__ part,
                                                            const double *__restrict__ dl,
                                                            const int *__restrict__ off, int Q,
                                                            int live_max, double f0c,
                                                            double dfc, int F, int nfb,
-                                                           int64_t nitems, int nchunks) {
+                                                           int nchunks) {
   dl += (int64_t)blockIdx.y * live_max;
   off += (int64_t)blockIdx.y * (Q + 1);
   part += (int64_t)blockIdx.y * nchunks * F;
   const int lane = threadIdx.x % kWave;
-  const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  const bool live = item < nitems;
-  const int64_t it = live ? item : nitems - 1;
-  const int fb = (int)(it % nfb);
-  const int c = (int)(it / nfb);
-  const int f_lo = fb * kK;
-  const int nf = min(kK, F - f_lo);
-  const double fc = fma((double)f_lo, dfc, f0c);
-  float accr[kK], acci[kK];
-#pragma unroll
-  for (int t = 0; t < kK; ++t) accr[t] = acci[t] = 0.0f;
+  const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (c >= nchunks) return;
   const int q = c * kWave + lane;
-  const int t1 = min(off[q + 1], live_max);
+  const int t1 = min(off[q + 1], live_max), t0 = min(off[q], t1);
   bool any = false;
-  for (int t = min(off[q], t1); t < t1; ++t) {
-    const double d = dl[t];
-    if (d == d) {
-      add_ray(d, true, fc, dfc, accr, acci);
-      any = true;
-    }
-  }
-  float2 *out = part + (size_t)c * F + f_lo;
+  for (int t = t0; t < t1; ++t) any |= dl[t] == dl[t];
+  float2 *out = part + (size_t)c * F;
   if (__ballot(any) == 0) {
-    if (live && lane < nf) out[lane] = make_float2(0.0f, 0.0f);
+    for (int f = lane; f < F; f += kWave) out[f] = make_float2(0.0f, 0.0f);
     return;
   }
-  store_wave(out, lane, live ? nf : 0, accr, acci);
+  for (int fb = 0; fb < nfb; ++fb) {
+    const int f_lo = fb * kK;
+    const double fc = fma((double)f_lo, dfc, f0c);
+    float accr[kK], acci[kK];
+#pragma unroll
+    for (int t = 0; t < kK; ++t) accr[t] = acci[t] = 0.0f;
+    for (int t = t0; t < t1; ++t) {
+      const double d = dl[t];
+      if (d == d) add_ray(d, true, fc, dfc, accr, acci);
+    }
+    store_wave(out + f_lo, lane, min(kK, F - f_lo), accr, acci);
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_synth_reduce(float2 *__restrict__ out,
-                                                         const float2 *__restrict__ part,
-                                                         int nchunks, int F) {
-  const int f = blockIdx.x * kBlock + threadIdx.x;
+// The chunks' partials of each sample added in chunk order in FP64; 16 loads
+// are issued before their additions (the additions keep their order), and a
+// workgroup is one wave, so a few hundred samples spread over as many CUs.
+constexpr int kReduceBlock = 64;
+__global__ __launch_bounds__(kReduceBlock) void k_synth_reduce(float2 *__restrict__ out,
+                                                               const float2 *__restrict__ part,
+                                                               int nchunks, int F) {
+  const int f = blockIdx.x * kReduceBlock + threadIdx.x;
   if (f >= F) return;
   out += (int64_t)blockIdx.y * F;
-  part += (int64_t)blockIdx.y * nchunks * F;
+  part += (int64_t)blockIdx.y * nchunks * F + f;
   double sr = 0.0, si = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
-    const float2 v = part[(size_t)c * F + f];
+  int c = 0;
+  for (; c + 16 <= nchunks; c += 16) {
+    float2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = part[(size_t)(c + k) * F];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      sr += v[k].x;
+      si += v[k].y;
+    }
+  }
+  for (; c < nchunks; ++c) {
+    const float2 v = part[(size_t)c * F];
     sr += v.x;
     si += v.y;
   }
@@ -820,8 +831,8 @@ extern "C" int mmn_simulate_channel(float *channel, const float *grid, float tau
   hipLaunchKernelGGL(k_ray_synth, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
                      dim3(kBlock), 0, st, part, dist, n, chunk, f0 * inv_c, df * inv_c,
                      samples, nfb, nitems, nch);
-  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock)), dim3(kBlock), 0, st,
-                     (float2 *)channel, part, nch, samples);
+  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kReduceBlock)),
+                     dim3(kReduceBlock), 0, st, (float2 *)channel, part, nch, samples);
   return mmn_check_launch("mmn_simulate_channel");
 }
 
@@ -910,8 +921,8 @@ extern "C" int mmn_simulate_channels(float *channels, const float *grid, int n_c
                           (unsigned)n_sim),
                      dim3(kBlock), 0, st, part, dist, n, chunk, f0 * inv_c, df * inv_c,
                      samples, nfb, nitems, nch);
-  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock), (unsigned)n_sim),
-                     dim3(kBlock), 0, st, (float2 *)channels, part, nch, samples);
+  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kReduceBlock), (unsigned)n_sim),
+                     dim3(kReduceBlock), 0, st, (float2 *)channels, part, nch, samples);
   return mmn_check_launch("mmn_simulate_channels");
 }
 
@@ -1092,15 +1103,14 @@ extern "C" int mmn_simulate_channels_prefixed(
                      0, st, dl, grid, g, voxel_size, is_ti_radar, (const SimParam *)sp, vox,
                      (const RayState *)L.st, (const int *)L.list, (const int *)L.off, Q, lm);
   const int nfb = (samples + kK - 1) / kK;
-  const int64_t nitems = (int64_t)nfb * nch;
   const double inv_c = 1.0 / MMN_C0;
   hipLaunchKernelGGL(k_ray_synth_live,
-                     dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock),
+                     dim3((unsigned)((nch + kWavesPerBlock - 1) / kWavesPerBlock),
                           (unsigned)n_sim),
                      dim3(kBlock), 0, st, part, (const double *)dl, (const int *)L.off, Q, lm,
-                     f0 * inv_c, df * inv_c, samples, nfb, nitems, nch);
-  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kBlock), (unsigned)n_sim),
-                     dim3(kBlock), 0, st, (float2 *)channels, part, nch, samples);
+                     f0 * inv_c, df * inv_c, samples, nfb, nch);
+  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kReduceBlock), (unsigned)n_sim),
+                     dim3(kReduceBlock), 0, st, (float2 *)channels, part, nch, samples);
   return mmn_check_launch("mmn_simulate_channels_prefixed");
 }
 
