@@ -1,7 +1,7 @@
 """Drop-ins for the RSDF consumer: ``ray_tracing.ray_matching``
 (ray_tracing.cpp:34-76) and the isosurface search of
-``isosurface_optimization.py`` (channel simulation :137-245, loss :248-280,
-channel filter :282-330, tau selection :332-398, point cloud :400-416, the
+``isosurface_optimization.py`` (channel simulation :111-209, loss :211-246,
+channel filter :248-290, tau selection :292-358, point cloud :360-377, the
 search loop :475-521).  Same names, arguments and results.
 
 The per-ray work -- ray generation, sphere tracing, the refine step, the
