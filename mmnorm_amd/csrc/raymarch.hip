@@ -421,25 +421,25 @@ __global__ __launch_bounds__(kBlock) void k_ray_prefix(RayState *__restrict__ st
   st[(int64_t)blockIdx.y * n + p] = RayState{depth, code};
 }
 
-// The live rays (status RESUME or HIT) of each synthesis class q = c * 64 + l
-// (chunk c, lane l: rays r0 + l, r0 + l + 64, ... of the chunk, the order in
-// which k_ray_synth's lane l adds them): counted, scanned into offsets
-// off[s][q] (off[s][Q] = the simulation's live rays), listed in that order.
-__device__ __forceinline__ void class_range(int q, int n, int chunk, int &r, int &r1) {
-  const int c = q / kWave;
-  r = c * chunk + q % kWave;
-  r1 = min(n, c * chunk + chunk);
+// The live rays (status RESUME or HIT) of each class q < Q, the contiguous
+// rays [q * per, (q + 1) * per) with per = ceil(n / Q): counted, scanned into
+// offsets off[s][q] (off[s][Q] = the simulation's live rays) and listed, so
+// that each simulation's list holds its live rays in ray order.
+__device__ __forceinline__ void class_range(int q, int n, int Q, int &r, int &r1) {
+  const int64_t per = ((int64_t)n + Q - 1) / Q;
+  r = (int)min((int64_t)n, q * per);
+  r1 = (int)min((int64_t)n, r + per);
 }
 
 __global__ __launch_bounds__(kBlock) void k_live_count(int *__restrict__ off,
                                                        const RayState *__restrict__ st, int n,
-                                                       int chunk, int Q) {
+                                                       int Q) {
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q >= Q) return;
   const RayState *sts = st + (int64_t)blockIdx.y * n;
   int r, r1, k = 0;
-  class_range(q, n, chunk, r, r1);
-  for (; r < r1; r += kWave) k += (sts[r].code & 3) != kStNone;
+  class_range(q, n, Q, r, r1);
+  for (; r < r1; ++r) k += (sts[r].code & 3) != kStNone;
   off[(int64_t)blockIdx.y * (Q + 1) + q] = k;
 }
 
@@ -478,14 +478,14 @@ __global__ __launch_bounds__(kScanThreads) void k_live_scan(int *__restrict__ of
 __global__ __launch_bounds__(kBlock) void k_live_fill(int *__restrict__ list,
                                                       const int *__restrict__ off,
                                                       const RayState *__restrict__ st, int n,
-                                                      int chunk, int Q) {
+                                                      int Q) {
   const int q = blockIdx.x * kBlock + threadIdx.x;
   if (q >= Q) return;
   const RayState *sts = st + (int64_t)blockIdx.y * n;
   int *ls = list + (int64_t)blockIdx.y * n;
   int r, r1, k = off[(int64_t)blockIdx.y * (Q + 1) + q];
-  class_range(q, n, chunk, r, r1);
-  for (; r < r1; r += kWave)
+  class_range(q, n, Q, r, r1);
+  for (; r < r1; ++r)
     if ((sts[r].code & 3) != kStNone) ls[k++] = r;
 }
 
@@ -525,23 +525,37 @@ __global__ __launch_bounds__(kBlock) void k_ray_resume(double *__restrict__ dl,
 }
 
 // Channel synthesis: out[f] = sum over hit rays of exp(+j 2 pi (f0 + f df) d / c).
-// One wave per (block of kK samples, ray chunk); lanes walk the chunk's rays
-// and keep kK complex sums, from one FP64-reduced anchor and the two-term
-// recurrence z_{t+1} = 2 cos(w) z_t - z_{t-1} per ray; a butterfly reduces
-// the wave and the partials go to the workspace.
 //
-// One ray's kK terms added to the lane's sums (w = 1 for a hit; a miss, w =
-// 0, adds signed zeros, which leave every sum as it is: the sums start at +0
-// and +0 + -0 = +0).
-__device__ __forceinline__ void add_ray(double d, bool ok, double fc, double dfc,
-                                        float (&accr)[kK], float (&acci)[kK]) {
+// The round trips of a simulation (NaN for a ray without a hit) are first
+// compacted, in order, into its dense hit list (k_hit_count, k_live_scan,
+// k_hit_fill).  The list is split into nc(H) near-equal chunks of
+// consecutive hits (H hits; at least kHitsPerChunk a chunk, at most
+// kMaxChunks chunks); one wave per (chunk, block of kK samples) walks its
+// chunk's hits, lane l taking hits l, l + 64, ..., and keeps kK complex sums
+// per lane from one FP64-reduced anchor and the two-term recurrence
+// z_{t+1} = 2 cos(w) z_t - z_{t-1} per hit; a butterfly reduces the wave, and
+// the chunks' partials are added in chunk order in FP64 (k_synth_reduce).
+// Every step depends only on the hit list, so any two paths that produce the
+// same hits in the same order (the full march, the batched march, the
+// prefixed march of the live rays) give the same channel bit for bit.
+constexpr int kHitsPerChunk = 512;
+constexpr int kMaxChunks = 64;
+constexpr int kCompactPer = 4;                    // entries per thread of a tile
+constexpr int kCompactTile = kBlock * kCompactPer;
+
+__device__ __host__ __forceinline__ int hit_chunks(int hits) {
+  const int c = (hits + kHitsPerChunk - 1) / kHitsPerChunk;
+  return c < kMaxChunks ? c : kMaxChunks;
+}
+
+// One hit's kK terms added to the lane's sums.
+__device__ __forceinline__ void add_ray(double d, double fc, double dfc, float (&accr)[kK],
+                                        float (&acci)[kK]) {
 #pragma clang fp contract(off)
-  const double dd = ok ? d : 0.0;
   float ar, ai, rr, ri;
-  phasor_hw(dd * fc, ar, ai);
-  phasor_hw(dd * dfc, rr, ri);
-  const float w = ok ? 1.0f : 0.0f;
-  float zr = w * ar, zi = -w * ai;          // exp(+j ...)
+  phasor_hw(d * fc, ar, ai);
+  phasor_hw(d * dfc, rr, ri);
+  float zr = ar, zi = -ai;          // exp(+j ...)
   const float sr = rr, si = -ri;
   float pr = zr * sr - zi * si, pi_ = zr * si + zi * sr;
   const float c2 = sr + sr;
@@ -573,86 +587,125 @@ __device__ __forceinline__ void store_wave(float2 *__restrict__ out, int lane, i
     if (lane == t && t < nf) out[t] = make_float2(accr[t], acci[t]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_ray_synth(float2 *__restrict__ part,
-                                                      const double *__restrict__ dist,
-                                                      int nrays, int chunk, double f0c,
-                                                      double dfc, int F, int nfb,
-                                                      int64_t nitems, int nchunks) {
-  // simulation blockIdx.y of a batch: its rays and its partials
-  dist += (int64_t)blockIdx.y * nrays;
-  part += (int64_t)blockIdx.y * nchunks * F;
+// Source length of simulation s: its live-ray count from the prefix's
+// offsets (capped at the buffer's stride), or the fixed ray count.
+__device__ __forceinline__ int64_t src_len(const int *len, int64_t len_stride, int64_t m,
+                                           int s) {
+  return len ? min((int64_t)len[(int64_t)s * len_stride], m) : m;
+}
+
+// Tile b of simulation s (kCompactTile entries, entry e * kBlock + t of the
+// tile to thread t): which entries hold a hit, and each hit's rank among the
+// tile's hits (wave ballots; wave totals in LDS).  Returns the tile's count.
+__device__ __forceinline__ int tile_hits(const double *__restrict__ src, int64_t base,
+                                         int64_t m, double (&v)[kCompactPer],
+                                         int (&rank)[kCompactPer]) {
+  __shared__ int wtot[kCompactPer * kWavesPerBlock];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  bool ok[kCompactPer];
+#pragma unroll
+  for (int e = 0; e < kCompactPer; ++e) {
+    const int64_t p = base + (int64_t)e * kBlock + threadIdx.x;
+    v[e] = p < m ? src[p] : (double)NAN;
+    ok[e] = v[e] == v[e];
+  }
+#pragma unroll
+  for (int e = 0; e < kCompactPer; ++e) {
+    const uint64_t bal = __ballot(ok[e]);
+    rank[e] = ok[e] ? __popcll(bal & below) : -1;
+    if (lane == 0) wtot[e * kWavesPerBlock + w] = __popcll(bal);
+  }
+  __syncthreads();
+  // exclusive prefix over (e, wave) in entry order: e-major, then wave
+  int pre[kCompactPer], run = 0;
+#pragma unroll
+  for (int k = 0; k < kCompactPer * kWavesPerBlock; ++k) {
+    const int c = wtot[k];
+#pragma unroll
+    for (int e = 0; e < kCompactPer; ++e)
+      if (k == e * kWavesPerBlock + w) pre[e] = run;
+    run += c;
+  }
+  const int total = run;
+#pragma unroll
+  for (int e = 0; e < kCompactPer; ++e)
+    if (rank[e] >= 0) rank[e] += pre[e];
+  return total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hit_count(int *__restrict__ cnt,
+                                                      const double *__restrict__ src,
+                                                      int64_t stride, const int *len,
+                                                      int64_t len_stride, int nb) {
+  const int s = blockIdx.y;
+  const int64_t m = src_len(len, len_stride, stride, s);
+  const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+  double v[kCompactPer];
+  int rank[kCompactPer];
+  const int total = base < m ? tile_hits(src + (int64_t)s * stride, base, m, v, rank) : 0;
+  if (threadIdx.x == 0) cnt[(int64_t)s * (nb + 1) + blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hit_fill(double *__restrict__ dense,
+                                                     const int *__restrict__ off,
+                                                     const double *__restrict__ src,
+                                                     int64_t stride, const int *len,
+                                                     int64_t len_stride, int nb) {
+  const int s = blockIdx.y;
+  const int64_t m = src_len(len, len_stride, stride, s);
+  const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+  if (base >= m) return;
+  double v[kCompactPer];
+  int rank[kCompactPer];
+  tile_hits(src + (int64_t)s * stride, base, m, v, rank);
+  double *out = dense + (int64_t)s * stride + off[(int64_t)s * (nb + 1) + blockIdx.x];
+#pragma unroll
+  for (int e = 0; e < kCompactPer; ++e)
+    if (rank[e] >= 0) out[rank[e]] = v[e];
+}
+
+// Wave (chunk c, sample block fb) of simulation blockIdx.y; hits[s] is the
+// simulation's hit count (off[s][nb] of the compaction).
+__global__ __launch_bounds__(kBlock) void k_synth_dense(float2 *__restrict__ part,
+                                                        const double *__restrict__ dense,
+                                                        int64_t stride,
+                                                        const int *__restrict__ hits,
+                                                        int64_t hits_stride, double f0c,
+                                                        double dfc, int F, int nfb) {
+  const int s = blockIdx.y;
   const int lane = threadIdx.x % kWave;
-  const int64_t item = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  const bool live = item < nitems;
-  const int64_t it = live ? item : nitems - 1;
-  const int fb = (int)(it % nfb);
-  const int c = (int)(it / nfb);
+  const int item = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const int c = item / nfb, fb = item % nfb;
+  const int H = hits[(int64_t)s * hits_stride];
+  const int nc = hit_chunks(H);
+  if (c >= nc) return;
+  const int per = (H + nc - 1) / nc;
+  const int t0 = c * per, t1 = min(H, t0 + per);
+  const double *d = dense + (int64_t)s * stride;
   const int f_lo = fb * kK;
-  const int nf = min(kK, F - f_lo);
   const double fc = fma((double)f_lo, dfc, f0c);
   float accr[kK], acci[kK];
 #pragma unroll
   for (int t = 0; t < kK; ++t) accr[t] = acci[t] = 0.0f;
-  const int r0 = c * chunk;
-  const int r1 = r0 + min(chunk, nrays - r0);
-  for (int r = r0 + lane; r < r1; r += kWave) {
-    const double d = dist[r];
-    add_ray(d, d == d, fc, dfc, accr, acci);
-  }
-  store_wave(part + (size_t)c * F + f_lo, lane, live ? nf : 0, accr, acci);
+  for (int t = t0 + lane; t < t1; t += kWave) add_ray(d[t], fc, dfc, accr, acci);
+  store_wave(part + ((int64_t)s * kMaxChunks + c) * F + f_lo, lane, min(kK, F - f_lo), accr,
+             acci);
 }
 
-// The same sums over the live rays only, from the prefix's lists: one wave
-// per chunk c and simulation; lane l adds the hits of its class (c, l) in
-// the order k_ray_synth's lane would, once per block of kK samples, and the
-// misses it skips add signed zeros there, so the partials are k_ray_synth's
-// bit for bit.  A chunk without a hit writes its +0 partials directly.
-__global__ __launch_bounds__(kBlock) void k_ray_synth_live(float2 *__restrict__ part,
-                                                           const double *__restrict__ dl,
-                                                           const int *__restrict__ off, int Q,
-                                                           int live_max, double f0c,
-                                                           double dfc, int F, int nfb,
-                                                           int nchunks) {
-  dl += (int64_t)blockIdx.y * live_max;
-  off += (int64_t)blockIdx.y * (Q + 1);
-  part += (int64_t)blockIdx.y * nchunks * F;
-  const int lane = threadIdx.x % kWave;
-  const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (c >= nchunks) return;
-  const int q = c * kWave + lane;
-  const int t1 = min(off[q + 1], live_max), t0 = min(off[q], t1);
-  bool any = false;
-  for (int t = t0; t < t1; ++t) any |= dl[t] == dl[t];
-  float2 *out = part + (size_t)c * F;
-  if (__ballot(any) == 0) {
-    for (int f = lane; f < F; f += kWave) out[f] = make_float2(0.0f, 0.0f);
-    return;
-  }
-  for (int fb = 0; fb < nfb; ++fb) {
-    const int f_lo = fb * kK;
-    const double fc = fma((double)f_lo, dfc, f0c);
-    float accr[kK], acci[kK];
-#pragma unroll
-    for (int t = 0; t < kK; ++t) accr[t] = acci[t] = 0.0f;
-    for (int t = t0; t < t1; ++t) {
-      const double d = dl[t];
-      if (d == d) add_ray(d, true, fc, dfc, accr, acci);
-    }
-    store_wave(out + f_lo, lane, min(kK, F - f_lo), accr, acci);
-  }
-}
-
-// The chunks' partials of each sample added in chunk order in FP64; 16 loads
-// are issued before their additions (the additions keep their order), and a
-// workgroup is one wave, so a few hundred samples spread over as many CUs.
+// The chunks' partials of each sample added in chunk order in FP64 (16 loads
+// issued before their additions, which keep their order); a workgroup is one
+// wave, so a few hundred samples spread over as many CUs.  No hit: zeros.
 constexpr int kReduceBlock = 64;
 __global__ __launch_bounds__(kReduceBlock) void k_synth_reduce(float2 *__restrict__ out,
                                                                const float2 *__restrict__ part,
-                                                               int nchunks, int F) {
+                                                               const int *__restrict__ hits,
+                                                               int64_t hits_stride, int F) {
   const int f = blockIdx.x * kReduceBlock + threadIdx.x;
   if (f >= F) return;
+  const int nchunks = hit_chunks(hits[(int64_t)blockIdx.y * hits_stride]);
   out += (int64_t)blockIdx.y * F;
-  part += (int64_t)blockIdx.y * nchunks * F + f;
+  part += (int64_t)blockIdx.y * kMaxChunks * F + f;
   double sr = 0.0, si = 0.0;
   int c = 0;
   for (; c + 16 <= nchunks; c += 16) {
@@ -677,7 +730,11 @@ __global__ __launch_bounds__(kReduceBlock) void k_synth_reduce(float2 *__restric
 // pairs: DFT of each channel zero-padded to n_fft, bins [lo, hi) (target
 // bins >= hi_target zeroed first), each profile normalised to unit L2 norm,
 // loss = sum | |sim| - |target| |.  One block per pair; twiddles from the
-// exact integer phase k n mod n_fft.
+// exact integer phase k n mod n_fft, tabulated once per block in LDS when
+// n_fft <= kLossTable (the same sincospi values), and the pair's samples
+// staged in LDS when F <= kLossStage.
+constexpr int kLossTable = 2048;
+constexpr int kLossStage = 1024;
 __global__ __launch_bounds__(kBlock) void k_channel_loss(float *__restrict__ loss,
                                                          const float2 *__restrict__ sim,
                                                          const float2 *__restrict__ tgt,
@@ -690,20 +747,47 @@ __global__ __launch_bounds__(kBlock) void k_channel_loss(float *__restrict__ los
 #pragma clang fp contract(off)
   __shared__ double sh[3][kBlock];
   __shared__ float mag[2][1024];
+  __shared__ double2 tw[kLossTable];        // (cos, sin)(2 pi m / n_fft)
+  __shared__ float2 stage[2][kLossStage];
+  const bool table = n_fft <= kLossTable, staged = F <= kLossStage;
   const float2 *a = sim + (size_t)blockIdx.x * F;
   const float2 *b = tgt + (size_t)blockIdx.x * F;
+  if (table)
+    for (int m = threadIdx.x; m < n_fft; m += kBlock) {
+      double sn, cs;
+      sincospi(2.0 * m / n_fft, &sn, &cs);
+      tw[m] = make_double2(cs, sn);
+    }
+  if (staged)
+    for (int n = threadIdx.x; n < F; n += kBlock) {
+      stage[0][n] = a[n];
+      stage[1][n] = b[n];
+    }
+  __syncthreads();
+  if (staged) {
+    a = stage[0];
+    b = stage[1];
+  }
   const int nb = hi - lo;
   for (int k = lo + threadIdx.x; k < hi; k += kBlock) {
     double ar = 0, ai = 0, br = 0, bi = 0;
+    int m = 0;  // k n mod n_fft
     for (int n = 0; n < F; ++n) {
-      const int m = (int)(((int64_t)k * n) % n_fft);
       double s, cth;
-      sincospi(2.0 * m / n_fft, &s, &cth);  // exp(-j 2 pi k n / N) = cth - j s
+      if (table) {
+        const double2 t = tw[m];
+        cth = t.x;
+        s = t.y;
+      } else {
+        sincospi(2.0 * m / n_fft, &s, &cth);  // exp(-j 2 pi k n / N) = cth - j s
+      }
       const float2 x = a[n], y = b[n];
       ar += x.x * cth + x.y * s;
       ai += x.y * cth - x.x * s;
       br += y.x * cth + y.y * s;
       bi += y.y * cth - y.x * s;
+      m += k;
+      if (m >= n_fft) m -= n_fft;
     }
     mag[0][k - lo] = (float)sqrt(ar * ar + ai * ai);
     mag[1][k - lo] = k < hi_target ? (float)sqrt(br * br + bi * bi) : 0.0f;
@@ -739,10 +823,64 @@ __global__ __launch_bounds__(kBlock) void k_channel_loss(float *__restrict__ los
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int synth_chunks(int nrays) {
-  // ~256 rays per wave, at most 1024 chunks
-  int c = (nrays + 255) / 256;
-  return c < 1 ? 1 : (c > 1024 ? 1024 : c);
+// Classes of the prefix's live lists: about four rays each, at most 65536.
+int live_classes(int n) {
+  const int64_t c = ((int64_t)n + 3) / 4;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(c, 65536));
+}
+
+// The synthesis stage's buffers for n_sim simulations of up to m round trips
+// each: the dense hit lists, the compaction's tile offsets (the hit count of
+// simulation s at off[s][nb] and hits[s]) and the chunks' partials.
+struct SynthLayout {
+  double *dense;
+  int *off, *hits;
+  float2 *part;
+  size_t bytes;
+};
+
+int synth_tiles(int64_t m) { return (int)((m + kCompactTile - 1) / kCompactTile); }
+
+SynthLayout synth_layout(int64_t m, int samples, int n_sim, char *base) {
+  SynthLayout L;
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    char *q = base ? base + o : nullptr;
+    o += align256(b);
+    return q;
+  };
+  L.dense = (double *)take(sizeof(double) * (size_t)m * n_sim);
+  L.off = (int *)take(sizeof(int) * (size_t)(synth_tiles(m) + 1) * n_sim);
+  L.hits = (int *)take(sizeof(int) * (size_t)n_sim);
+  L.part = (float2 *)take(sizeof(float2) * (size_t)kMaxChunks * samples * n_sim);
+  L.bytes = o;
+  return L;
+}
+
+// Channels of n_sim simulations whose round trips are src[s * stride + i]
+// for i below the simulation's length (len[s * len_stride] capped at stride,
+// or stride when len is NULL): compaction, synthesis, reduction.
+void launch_synth(hipStream_t st, float2 *channels, const double *src, int64_t stride,
+                  const int *len, int64_t len_stride, int n_sim, double f0, double df,
+                  int samples, const SynthLayout &L) {
+  const int nb = synth_tiles(stride);
+  hipLaunchKernelGGL(k_hit_count, dim3((unsigned)nb, (unsigned)n_sim), dim3(kBlock), 0, st,
+                     L.off, src, stride, len, len_stride, nb);
+  hipLaunchKernelGGL(k_live_scan, dim3((unsigned)n_sim), dim3(kScanThreads), 0, st, L.off,
+                     L.hits, nb);
+  hipLaunchKernelGGL(k_hit_fill, dim3((unsigned)nb, (unsigned)n_sim), dim3(kBlock), 0, st,
+                     L.dense, (const int *)L.off, src, stride, len, len_stride, nb);
+  const int nfb = (samples + kK - 1) / kK;
+  const int items = kMaxChunks * nfb;
+  const double inv_c = 1.0 / MMN_C0;
+  hipLaunchKernelGGL(k_synth_dense,
+                     dim3((unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock),
+                          (unsigned)n_sim),
+                     dim3(kBlock), 0, st, L.part, (const double *)L.dense, stride,
+                     (const int *)L.hits, (int64_t)1, f0 * inv_c, df * inv_c, samples, nfb);
+  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kReduceBlock), (unsigned)n_sim),
+                     dim3(kReduceBlock), 0, st, channels, (const float2 *)L.part,
+                     (const int *)L.hits, (int64_t)1, samples);
 }
 
 bool geom_from(RayGeom &g, int width, int height, float fov_w, float fov_h,
@@ -793,10 +931,10 @@ extern "C" int mmn_ray_matching(float *intersection_pos, float *voxel_position,
 }
 
 extern "C" size_t mmn_simulate_channel_workspace_size(int width, int height, int samples) {
-  if (width <= 0 || height <= 0 || samples <= 0) return 0;
+  if (width <= 0 || height <= 0 || samples <= 0 || (int64_t)width * height > INT32_MAX)
+    return 0;
   const int n = width * height;
-  return align256(sizeof(double) * (size_t)n) +
-         align256(sizeof(float2) * (size_t)synth_chunks(n) * samples);
+  return align256(sizeof(double) * (size_t)n) + synth_layout(n, samples, 1, nullptr).bytes;
 }
 
 extern "C" int mmn_simulate_channel(float *channel, const float *grid, float tau, int width,
@@ -818,21 +956,13 @@ extern "C" int mmn_simulate_channel(float *channel, const float *grid, float tau
               "mmn_simulate_channel: workspace of %zu bytes needed", need);
   const int n = width * height;
   double *dist = (double *)workspace;
-  float2 *part = (float2 *)((char *)workspace + align256(sizeof(double) * (size_t)n));
+  const SynthLayout L =
+      synth_layout(n, samples, 1, (char *)workspace + align256(sizeof(double) * (size_t)n));
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_ray_dist<false>, dim3(mmn_blocks(n, kBlock)), dim3(kBlock), 0, st, dist,
                      grid, g, voxel_size, rx_loc[0], rx_loc[1], rx_loc[2], is_ti_radar,
                      (const SimParam *)nullptr, (int64_t)0);
-  const int nch = synth_chunks(n);
-  const int chunk = (n + nch - 1) / nch;
-  const int nfb = (samples + kK - 1) / kK;
-  const int64_t nitems = (int64_t)nfb * nch;
-  const double inv_c = 1.0 / MMN_C0;
-  hipLaunchKernelGGL(k_ray_synth, dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock)),
-                     dim3(kBlock), 0, st, part, dist, n, chunk, f0 * inv_c, df * inv_c,
-                     samples, nfb, nitems, nch);
-  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kReduceBlock)),
-                     dim3(kReduceBlock), 0, st, (float2 *)channel, part, nch, samples);
+  launch_synth(st, (float2 *)channel, dist, n, nullptr, 0, 1, f0, df, samples, L);
   return mmn_check_launch("mmn_simulate_channel");
 }
 
@@ -841,7 +971,7 @@ namespace {
 size_t batch_ws(int n, int samples, int nsim) {
   return align256(sizeof(SimParam) * (size_t)nsim) +
          align256(sizeof(double) * (size_t)n * nsim) +
-         align256(sizeof(float2) * (size_t)synth_chunks(n) * samples * nsim);
+         synth_layout(n, samples, nsim, nullptr).bytes;
 }
 
 }  // namespace
@@ -899,7 +1029,8 @@ extern "C" int mmn_simulate_channels(float *channels, const float *grid, int n_c
   char *w = (char *)workspace;
   SimParam *sp = (SimParam *)w;
   double *dist = (double *)(w + align256(sizeof(SimParam) * (size_t)n_sim));
-  float2 *part = (float2 *)((char *)dist + align256(sizeof(double) * (size_t)n * n_sim));
+  const SynthLayout L = synth_layout(n, samples, n_sim,
+                                     (char *)dist + align256(sizeof(double) * (size_t)n * n_sim));
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = hipMemcpyAsync(sp, prm.data(), sizeof(SimParam) * (size_t)n_sim,
                                 hipMemcpyHostToDevice, st);
@@ -911,18 +1042,7 @@ extern "C" int mmn_simulate_channels(float *channels, const float *grid, int n_c
   hipLaunchKernelGGL(k_ray_dist<true>, dim3(mmn_blocks(n, kBlock), (unsigned)n_sim),
                      dim3(kBlock), 0, st, dist, grid, g, voxel_size, 0.0, 0.0, 0.0, is_ti_radar,
                      (const SimParam *)sp, vox);
-  const int nch = synth_chunks(n);
-  const int chunk = (n + nch - 1) / nch;
-  const int nfb = (samples + kK - 1) / kK;
-  const int64_t nitems = (int64_t)nfb * nch;
-  const double inv_c = 1.0 / MMN_C0;
-  hipLaunchKernelGGL(k_ray_synth,
-                     dim3((unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock),
-                          (unsigned)n_sim),
-                     dim3(kBlock), 0, st, part, dist, n, chunk, f0 * inv_c, df * inv_c,
-                     samples, nfb, nitems, nch);
-  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kReduceBlock), (unsigned)n_sim),
-                     dim3(kReduceBlock), 0, st, (float2 *)channels, part, nch, samples);
+  launch_synth(st, (float2 *)channels, dist, n, nullptr, 0, n_sim, f0, df, samples, L);
   return mmn_check_launch("mmn_simulate_channels");
 }
 
@@ -936,7 +1056,7 @@ struct PrefixLayout {
 };
 
 PrefixLayout prefix_layout(int n, int n_sim, char *base) {
-  const int64_t Q = (int64_t)synth_chunks(n) * kWave;
+  const int64_t Q = live_classes(n);
   PrefixLayout L;
   size_t o = 0;
   auto take = [&](size_t b) {
@@ -1023,15 +1143,15 @@ extern "C" int mmn_ray_prefix(void *prefix, size_t prefix_bytes, int *live_count
   rc = copy_params("mmn_ray_prefix", L.sp, n_sim, comp, nullptr, eyes, rx_dirs, nullptr, st);
   if (rc) return rc;
   const int64_t vox = (int64_t)grid_res[0] * grid_res[1] * grid_res[2];
-  const int nch = synth_chunks(n), chunk = (n + nch - 1) / nch, Q = nch * kWave;
+  const int Q = live_classes(n);
   hipLaunchKernelGGL(k_ray_prefix, dim3(mmn_blocks(n, kBlock), (unsigned)n_sim), dim3(kBlock),
                      0, st, L.st, grid, g, (const SimParam *)L.sp, vox, tau_lo);
   hipLaunchKernelGGL(k_live_count, dim3(mmn_blocks(Q, kBlock), (unsigned)n_sim), dim3(kBlock),
-                     0, st, L.off, (const RayState *)L.st, n, chunk, Q);
+                     0, st, L.off, (const RayState *)L.st, n, Q);
   hipLaunchKernelGGL(k_live_scan, dim3((unsigned)n_sim), dim3(kScanThreads), 0, st, L.off,
                      L.tot, Q);
   hipLaunchKernelGGL(k_live_fill, dim3(mmn_blocks(Q, kBlock), (unsigned)n_sim), dim3(kBlock),
-                     0, st, L.list, (const int *)L.off, (const RayState *)L.st, n, chunk, Q);
+                     0, st, L.list, (const int *)L.off, (const RayState *)L.st, n, Q);
   rc = mmn_check_launch("mmn_ray_prefix");
   if (rc) return rc;
   hipError_t e = hipMemcpyAsync(live_counts, L.tot, sizeof(int) * (size_t)n_sim,
@@ -1050,10 +1170,10 @@ extern "C" size_t mmn_simulate_channels_prefixed_workspace_size(int width, int h
   if (width <= 0 || height <= 0 || samples <= 0 || n_sim <= 0 || live_max < 0 ||
       (int64_t)width * height > INT32_MAX)
     return 0;
-  const int n = width * height;
+  const int lm = std::max(live_max, 1);
   return align256(sizeof(SimParam) * (size_t)n_sim) +
-         align256(sizeof(double) * (size_t)std::max(live_max, 1) * n_sim) +
-         align256(sizeof(float2) * (size_t)synth_chunks(n) * samples * n_sim);
+         align256(sizeof(double) * (size_t)lm * n_sim) +
+         synth_layout(lm, samples, n_sim, nullptr).bytes;
 }
 
 extern "C" int mmn_simulate_channels_prefixed(
@@ -1089,28 +1209,23 @@ extern "C" int mmn_simulate_channels_prefixed(
               "mmn_simulate_channels_prefixed: workspace of %zu bytes needed", need);
   char *w = (char *)workspace;
   SimParam *sp = (SimParam *)w;
+  const int lm = std::max(live_max, 1);
   double *dl = (double *)(w + align256(sizeof(SimParam) * (size_t)n_sim));
-  float2 *part = (float2 *)((char *)dl +
-                            align256(sizeof(double) * (size_t)std::max(live_max, 1) * n_sim));
+  const SynthLayout S =
+      synth_layout(lm, samples, n_sim, (char *)dl + align256(sizeof(double) * (size_t)lm * n_sim));
   hipStream_t st = (hipStream_t)stream;
   rc = copy_params("mmn_simulate_channels_prefixed", sp, n_sim, comp, taus, eyes, rx_dirs,
                    rx_locs, st);
   if (rc) return rc;
   const int64_t vox = (int64_t)grid_res[0] * grid_res[1] * grid_res[2];
-  const int nch = synth_chunks(n), Q = nch * kWave;
-  const int lm = std::max(live_max, 1);
+  const int Q = live_classes(n);
   hipLaunchKernelGGL(k_ray_resume, dim3(mmn_blocks(lm, kBlock), (unsigned)n_sim), dim3(kBlock),
                      0, st, dl, grid, g, voxel_size, is_ti_radar, (const SimParam *)sp, vox,
                      (const RayState *)L.st, (const int *)L.list, (const int *)L.off, Q, lm);
-  const int nfb = (samples + kK - 1) / kK;
-  const double inv_c = 1.0 / MMN_C0;
-  hipLaunchKernelGGL(k_ray_synth_live,
-                     dim3((unsigned)((nch + kWavesPerBlock - 1) / kWavesPerBlock),
-                          (unsigned)n_sim),
-                     dim3(kBlock), 0, st, part, (const double *)dl, (const int *)L.off, Q, lm,
-                     f0 * inv_c, df * inv_c, samples, nfb, nch);
-  hipLaunchKernelGGL(k_synth_reduce, dim3(mmn_blocks(samples, kReduceBlock), (unsigned)n_sim),
-                     dim3(kReduceBlock), 0, st, (float2 *)channels, part, nch, samples);
+  // the live rays' round trips (NaN: no hit at this offset), in ray order;
+  // simulation s has off[s][Q] of them
+  launch_synth(st, (float2 *)channels, dl, lm, (const int *)L.off + Q, (int64_t)Q + 1, n_sim,
+               f0, df, samples, S);
   return mmn_check_launch("mmn_simulate_channels_prefixed");
 }
 

@@ -303,7 +303,6 @@ def search_taus(sdf, rx_locs, rx_dirs, channels, bounding_box, wavelengths, voxe
                             dtype=torch.float32) for i in range(n_sdf)]
     tau_grid = torch.stack(torch.meshgrid(*vectors, indexing="ij"), dim=-1)[skip:, ...]
     shape = tau_grid.shape[:-1]
-    losses = torch.zeros(shape, dtype=torch.float32)
     order = torch.randperm(len(rx_locs), generator=generator)[:num_rx]
     dev = sdf.device
     tgt = channels[order].to(dev)
@@ -317,12 +316,15 @@ def search_taus(sdf, rx_locs, rx_dirs, channels, bounding_box, wavelengths, voxe
     # the march up to the first offset-dependent step, once for the search
     prefix = _channel_prefix(bounding_box, width, height, width_fov, height_fov, grid_cm,
                              locs, dirs, tau_bounds)
-    for flat_i in range(int(np.prod(shape))):
-        idx = np.unravel_index(flat_i, shape)
-        taus = tau_grid[idx]
+    # the offsets' losses stay on the device until the loop ends (one read-back)
+    n_flat = int(np.prod(shape))
+    losses = torch.zeros(n_flat, dtype=torch.float32, device=dev)
+    for flat_i in range(n_flat):
+        taus = tau_grid[np.unravel_index(flat_i, shape)]
         sims = _simulate_channel_batch(bounding_box, voxel_size, width, height, width_fov,
                                        height_fov, grid_cm, locs, dirs, wl, taus, tau_bounds,
                                        is_real_data, prefix=prefix)
-        losses[idx] = batch_loss(sims, tgt, is_real_data).sum().cpu()
+        losses[flat_i] = batch_loss(sims, tgt, is_real_data).sum()
+    losses = losses.cpu().reshape(shape)
     taus = _select_tau(tau_grid.numpy(), losses.numpy())
     return taus, tau_grid, losses
