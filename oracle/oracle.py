@@ -325,6 +325,67 @@ def py_li_threshold(img, exponent, max_iter=1000):
     return float(t_next + lo), k
 
 
+class LiAbort(Exception):
+    """The reference's callback limit was exceeded (sdf.py:53-58)."""
+
+
+def py_skimage_li(image, max_callbacks=None):
+    """skimage.filters.threshold_li of scikit-image 0.24.0 -- the version the
+    reference pins (requirements.txt:25) and calls at sdf.py:60-70, not
+    installed here -- restated from its published algorithm for a float
+    image, in the image's own dtype with NumPy's own mean and log (a float32
+    image keeps float32 means, NumPy's pairwise sums, as in the reference's
+    run):
+
+      NaNs dropped; a constant image returns its value; infinities dropped;
+      x = image - min; tolerance = half the smallest gap between distinct
+      values of x; t_next = mean(x), t_curr = -2 tolerance; the callback is
+      called once on the initial guess; while |t_next - t_curr| > tolerance:
+      t_curr = t_next, m_f = mean(x > t_curr), m_b = mean(x <= t_curr),
+      stop if m_b == 0, t_next = (m_b - m_f) / (log m_b - log m_f), callback;
+      threshold = t_next + min.
+
+    Returns (threshold, callbacks).  With ``max_callbacks`` (the reference
+    passes a callback that raises on call 1001) raises LiAbort on the call
+    past it."""
+    image = np.asarray(image)
+    image = image[~np.isnan(image)]
+    if image.size == 0:
+        return np.nan, 0
+    if np.all(image == image.flat[0]):
+        return image.flat[0], 0
+    image = image[np.isfinite(image)]
+    if image.size == 0:
+        return 0.0, 0
+    image_min = np.min(image)
+    image = image - image_min
+    tolerance = np.min(np.diff(np.unique(image))) / 2
+    t_next = np.mean(image)
+    t_curr = -2 * tolerance
+    calls = 1
+    while abs(t_next - t_curr) > tolerance:
+        t_curr = t_next
+        foreground = image > t_curr
+        mean_fore = np.mean(image[foreground])
+        mean_back = np.mean(image[~foreground])
+        if mean_back == 0:
+            break
+        t_next = (mean_back - mean_fore) / (np.log(mean_back) - np.log(mean_fore))
+        calls += 1
+        if max_callbacks is not None and calls > max_callbacks:
+            raise LiAbort(calls)
+    return t_next + image_min, calls
+
+
+def skimage_li_mask(img, exponent):
+    """The reference's binary image for one exponent (sdf.py:62-67):
+    |img|**exponent in the image's float32 and the restated threshold_li;
+    returns (mask |img|**exp < threshold, threshold, callbacks, |img|**exp)."""
+    abs_img = np.abs(np.asarray(img, np.complex64)) ** exponent
+    thr, calls = py_skimage_li(abs_img)
+    return abs_img < thr, thr, calls, abs_img
+
+
 # ----------------------------------------------------------------------------
 # Seeds as _construct_rsdf_components picks them (sdf.py:180-191).
 # ----------------------------------------------------------------------------
