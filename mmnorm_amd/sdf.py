@@ -5,7 +5,8 @@ and out.
 
 * The Li threshold of each exponent (sdf.py:46-70) runs on the GPU
   (``ops.li_threshold``); the binary-image volume test (sdf.py:77-84) uses
-  ``volume_fn`` (see there).
+  ``volume_fn``: by default ``hull_face_box_volume``, open3d's documented
+  minimal-box algorithm restated over scipy's qhull.
 * The 6-connected labelling (cc3d, sdf.py:89) and the per-label areas
   (sdf.py:100) run on the GPU (``ops.ccl6``, ``ops.label_sizes``); the
   reference's size rule (sdf.py:92-119) is applied to the areas as written.
@@ -37,12 +38,9 @@ def _device():
 
 
 def pca_box_volume(points):
-    """Volume of the box aligned with the principal axes of ``points``.
-
-    Stands in for open3d's minimal oriented bounding box (sdf.py:79-81),
-    which is not available here: it is an upper bound of that volume (equal
-    for box-like point sets), so the volume test can only be stricter.
-    """
+    """Volume of the box aligned with the principal axes of ``points``: a
+    cheaper ``volume_fn`` for ``_separate_binary_image`` (equal to
+    hull_face_box_volume for box-like point sets)."""
     p = np.asarray(points, dtype=np.float64)
     if p.shape[0] < 2:
         return 0.0
@@ -53,7 +51,46 @@ def pca_box_volume(points):
     return float(np.prod(ext)) if ext.size == 3 else 0.0
 
 
-def _separate_binary_image(sum_img, coords_full, volume_fn=pca_box_volume):
+def hull_face_box_volume(points, block=256):
+    """Volume of open3d's "minimal" oriented bounding box (the volume test of
+    sdf.py:79-81 calls ``get_minimal_oriented_bounding_box().volume()``),
+    from the algorithm open3d documents for it: the convex hull of the
+    points, and for each hull triangle the axis-aligned box of the hull's
+    vertices in that triangle's frame -- an edge, the in-plane direction
+    perpendicular to it, the normal -- the smallest of those volumes.  The
+    hull is qhull's (scipy.spatial.ConvexHull), as open3d's is; every edge of
+    a triangle is tried, so the volume is at most open3d's, whose frame takes
+    one edge per triangle in its hull's vertex order.  Not the true minimal
+    box: for a regular tetrahedron this gives sqrt(2)/2 edge^3, the true
+    minimum is edge^3 / (2 sqrt 2).  A flat or degenerate set (no 3-D hull)
+    has volume 0."""
+    from scipy.spatial import ConvexHull
+    from scipy.spatial import QhullError
+    p = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    if p.shape[0] < 4:
+        return 0.0
+    try:
+        hull = ConvexHull(p)
+    except QhullError:
+        return 0.0
+    hv = p[hull.vertices]
+    tri = p[hull.simplices]                       # (T, 3, 3)
+    w = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    best = np.inf
+    for e in range(3):
+        u = tri[:, (e + 1) % 3] - tri[:, e]
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        v = np.cross(w, u)
+        frames = np.stack([u, v, w], axis=1)      # (T, 3 axes, 3)
+        for lo in range(0, len(frames), block):
+            q = np.einsum("tak,nk->tan", frames[lo:lo + block], hv)
+            ext = q.max(axis=2) - q.min(axis=2)
+            best = min(best, float(np.min(np.prod(ext, axis=1))))
+    return best
+
+
+def _separate_binary_image(sum_img, coords_full, volume_fn=hull_face_box_volume):
     """Binary image of the SAR image (sdf.py:33-86): True where |sum_img|**exp is
     below the Li threshold, for the first exponent of (2, 2.5, 3.5, 4) whose Li
     iteration ends within 1000 callbacks and whose kept voxels span at most

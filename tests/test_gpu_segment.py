@@ -190,7 +190,7 @@ def test_find_connected_components_dropin():
     assert len(valid) > 0
 
 
-def test_separate_binary_image_dropin():
+def test_separate_binary_image_dropin(oracle):
     rng = np.random.default_rng(4)
     shape = (32, 24, 16)
     img = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
@@ -201,6 +201,11 @@ def test_separate_binary_image_dropin():
     got = sdf._separate_binary_image(img, coords, volume_fn=lambda p: 0.0)
     _, _, _, below = ops.li_threshold(torch.from_numpy(img).to(dev()), 2, below=True)
     assert np.array_equal(got, below.cpu().numpy())
+    # the default volume (open3d's face-framed hull boxes) passes on this
+    # 3.2 x 2.4 x 1.6 cm grid: the first exponent's, the reference's image
+    got = sdf._separate_binary_image(img, coords)
+    assert np.array_equal(got, below.cpu().numpy())
+    assert np.array_equal(got, oracle.skimage_li_mask(img, 2)[0])
     # volume test never passes: the last exponent's
     got = sdf._separate_binary_image(img, coords, volume_fn=lambda p: 1.0)
     _, _, _, below = ops.li_threshold(torch.from_numpy(img).to(dev()), 4, below=True)
