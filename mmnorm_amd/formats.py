@@ -10,7 +10,10 @@ of this package and of the reference can hand data to each other:
   quadrature parts.
 
 Files are written with ``numpy.savez_compressed`` as the reference does and
-read with ``allow_pickle=False``: every stored field is numeric.
+read with ``allow_pickle=False``: every stored field is numeric.  The loader
+side of a raw capture -- the ADC files, the trajectory files, the robot-run
+records (pickles, read by a restricted unpickler), the experiment JSON and
+the dataset's folder scheme -- follows generic_loader.py:36-67,484-630.
 """
 from __future__ import annotations
 
@@ -168,6 +171,53 @@ def load_robot_loc_files(path):
         for k, v in cur.items():
             combined[k] = np.concatenate((combined[k], v)) if k in combined else v
     return combined
+
+
+# Constructors a robot-run record may need: containers, scalars and NumPy
+# arrays.  Anything else in a pickle is refused before it is called.
+_PICKLE_ALLOWED = {
+    ("builtins", n) for n in ("list", "dict", "tuple", "set", "frozenset", "str", "bytes",
+                              "bytearray", "int", "float", "complex", "bool", "slice", "range")
+} | {
+    (m, n) for m in ("numpy.core.multiarray", "numpy._core.multiarray")
+    for n in ("_reconstruct", "scalar")
+} | {
+    (m, "_frombuffer") for m in ("numpy.core.numeric", "numpy._core.numeric")
+} | {("numpy", "ndarray"), ("numpy", "dtype"), ("collections", "OrderedDict"),
+     ("datetime", "datetime"), ("datetime", "date"), ("datetime", "timedelta")}
+
+
+def _load_record(f):
+    import pickle
+
+    class _Records(pickle.Unpickler):
+        def find_class(self, module, name):
+            if (module, name) not in _PICKLE_ALLOWED:
+                raise pickle.UnpicklingError(
+                    f"robot record refers to {module}.{name}, which is not a container, "
+                    f"scalar or array constructor")
+            return super().find_class(module, name)
+
+    return _Records(f).load()
+
+
+def load_robot_files(path):
+    """The robot-run records of `path` (files named 'continuous*', Python
+    pickles of dictionaries), merged key by key in sorted name order by
+    concatenation, 'pattern' skipped and then set to 0.5
+    (generic_loader.py:577-612).  Unlike the reference's pickle.load, the
+    records are read by an unpickler that builds only containers, scalars
+    and NumPy arrays and refuses any other callable a file names."""
+    robot = {}
+    for filename in sorted(f for f in os.listdir(path) if f[:10] == "continuous"):
+        with open(os.path.join(path, filename), "rb") as f:
+            cur = _load_record(f)
+        for k, v in cur.items():
+            if k == "pattern":
+                continue
+            robot[k] = robot[k] + v if k in robot else v
+    robot["pattern"] = 0.5
+    return robot
 
 
 def load_exp_json(path):

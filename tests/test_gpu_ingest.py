@@ -7,6 +7,7 @@ point scatterer's chirps (int16 I/Q, four RX lanes, the TI range offset) seen
 from a raster the robot sweeps row by row; checks: every frame's pose, the
 caller against the composed NumPy drop-ins, the focus and the normal."""
 import json
+import pickle
 
 import numpy as np
 import pytest
@@ -61,12 +62,20 @@ def build_dataset(root):
     with open(f"{root}/experiment_info.json", "w") as f:
         json.dump({"aperture size": "almost-square", "rx offset": "x", "background": "table"}, f)
     robot = {"all_radar_filenames": names, "all_radar_created_ts": created}
+    # the robot runs as the rig writes them: pickled dictionaries, one per run
+    half = len(names) // 2
+    for k, (lo, hi) in enumerate(((0, half), (half, len(names)))):
+        with open(f"{root}/continuous_{k}.pkl", "wb") as f:
+            pickle.dump({"all_radar_filenames": names[lo:hi],
+                         "all_radar_created_ts": created[lo:hi], "pattern": 1.0}, f)
     return params, robot, np.array(truth), target, (xl, yl, zl)
 
 
 def test_captures_to_normals(tmp_path):
     root = str(tmp_path)
-    params, robot, truth, target, (xl, yl, zl) = build_dataset(root)
+    params, robot_written, truth, target, (xl, yl, zl) = build_dataset(root)
+    robot = formats.load_robot_files(root)
+    assert robot == dict(robot_written, pattern=0.5)
     radar_data = formats.load_radar_files(root)
     robot_loc = formats.load_robot_loc_files(root)
     exp = formats.load_exp_json(root)

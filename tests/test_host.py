@@ -201,6 +201,41 @@ def test_robot_loc_files_trim_rule(tmp_path):
     assert formats.load_exp_json(str(tmp_path))["aperture size"] == "almost-square"
 
 
+class _Planted:
+    """Pickles as a call of os.getcwd: a record that names a callable."""
+
+    def __reduce__(self):
+        import os
+        return (os.getcwd, ())
+
+
+def test_robot_files_merge_and_refuse_callables(tmp_path):
+    """generic_loader.py:577-612: 'continuous*' records merged key by key in
+    sorted name order (lists concatenated), 'pattern' replaced by 0.5; other
+    files ignored.  A record naming a callable is refused unread."""
+    import pickle
+    from mmnorm_amd import formats
+    with open(tmp_path / "continuous_b.pkl", "wb") as f:
+        pickle.dump({"all_radar_filenames": ["3", "4"], "all_radar_created_ts": [3.0, 4.0],
+                     "pattern": 2, "extra": np.arange(3)}, f)
+    with open(tmp_path / "continuous_b2.pkl", "wb") as f:
+        pickle.dump({"extra": np.arange(3, 6.0)}, f, protocol=5)
+    with open(tmp_path / "continuous_a.pkl", "wb") as f:
+        pickle.dump({"all_radar_filenames": ["1"], "all_radar_created_ts": [1.0],
+                     "pattern": 7}, f, protocol=5)
+    with open(tmp_path / "simple_0.pkl", "wb") as f:
+        pickle.dump({"all_radar_filenames": ["x"]}, f)
+    r = formats.load_robot_files(str(tmp_path))
+    assert r["all_radar_filenames"] == ["1", "3", "4"]
+    assert r["all_radar_created_ts"] == [1.0, 3.0, 4.0]
+    # arrays merge with +, as the reference's robot_data[key] + cur_file[key]
+    assert r["pattern"] == 0.5 and np.array_equal(r["extra"], np.arange(3) + np.arange(3, 6.0))
+    with open(tmp_path / "continuous_c.pkl", "wb") as f:
+        pickle.dump({"all_radar_filenames": [_Planted()]}, f)
+    with pytest.raises(pickle.UnpicklingError, match="posix.getcwd|os.getcwd"):
+        formats.load_robot_files(str(tmp_path))
+
+
 @pytest.mark.parametrize("shape,use_4_rx,rows", [((5, 7), False, 5), ((5, 7, 1), False, 5),
                                                  ((5, 7, 4), True, 20), ((5, 7, 2), False, 5)])
 def test_prep_inputs_reads_rows_as_the_reference_kernel(shape, use_4_rx, rows):
