@@ -85,6 +85,24 @@ def load_sdf_file(path):
         return {k: f[k] for k in f.files}
 
 
+def save_optimization_file(path, data):
+    """Write an isosurface-stage output (generic_loader.py:448-461:
+    opt_output<ext>.npz in the processed folder's optimization/)."""
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    np.savez_compressed(path, **data)
+
+
+def load_optimization_file(path):
+    """The stored dictionary, or None when the file is missing
+    (generic_loader.py:463-481)."""
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
 def read_adc_bin(src, num_lanes=4):
     """Complex ADC samples (n, num_lanes) from a 77 GHz capture: a path, bytes
     or an int16 array; row r holds lanes' I in values 0..3 and Q in 4..7."""

@@ -15,6 +15,8 @@ channel filter, the peak picking of the tau selection and the point cloud.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import torch
 
@@ -328,3 +330,97 @@ def search_taus(sdf, rx_locs, rx_dirs, channels, bounding_box, wavelengths, voxe
     losses = losses.cpu().reshape(shape)
     taus = _select_tau(tau_grid.numpy(), losses.numpy())
     return taus, tau_grid, losses
+
+
+def read_sdf_file(path):
+    """_read_sdf_file (isosurface_optimization.py:30-83) on the stage file at
+    ``path`` (the sdf.npz of sdf.py:226-253): channels scaled by 1e-5/6,
+    per-component grids with UNDEF_VAL outside each component and the tau
+    bounds (prepare_components), the antennas, look directions (float32),
+    channels and wavelengths on the device.  The per-component normals the
+    reference also forms are not used by the search and are not built."""
+    from . import formats
+    data = formats.load_sdf_file(path)
+    if data is None:
+        raise FileNotFoundError(path)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    sdf, tau_bounds = prepare_components(data["rsdf"], data["normals"],
+                                         data["component_labels"], data["valid_labels"])
+    channels = torch.from_numpy(np.asarray(data["channels"]))
+    channels = channels * (0.00001 / 6)
+    return {
+        "sdf": sdf, "tau_bounds": tau_bounds,
+        "rx_locs": torch.from_numpy(np.asarray(data["rx_locs"])).to(dev),
+        "rx_dirs": torch.from_numpy(np.asarray(data["rx_dirs"])).to(torch.float32).to(dev),
+        "channels": channels.to(dev),
+        "bounding_box": data["bounding_box"],
+        "wavelengths": torch.from_numpy(np.asarray(data["wavelengths"])).to(dev),
+        "voxel_size": data["voxel_size"], "num_samples": int(data["num_samples"]),
+        "coords": data["coords"], "sum_img": data["sum_img"],
+    }
+
+
+def run_stage(sdf_path, out_dir=None, ext="", is_real_data=True, **search_kw):
+    """The isosurface stage of isosurface_optimization.py:379-540 on a stage
+    file: read it, search the offsets (search_taus with the reference's
+    2048^2 rays, +-80 degrees, 50 antennas unless overridden), keep the voxels
+    within 1 mm of the chosen isosurfaces, and write
+    opt_output_optimization_result<ext>.npz (taus, points) and
+    opt_output_full_data<ext>.npz (sdf, coords, sum_img, num_samples,
+    all_taus, all_losses) to ``out_dir``, as the reference's
+    save_optimization_file calls do.  Returns the two dictionaries."""
+    from . import formats
+    d = read_sdf_file(sdf_path)
+    taus, tau_grid, losses = search_taus(
+        d["sdf"], d["rx_locs"], d["rx_dirs"], d["channels"], d["bounding_box"],
+        d["wavelengths"], d["voxel_size"], d["tau_bounds"], d["num_samples"],
+        is_real_data=is_real_data, **search_kw)
+    sdf_host = d["sdf"].cpu().numpy()
+    result = {"taus": taus, "points": _get_final_point_cloud(sdf_host, taus, d["coords"])}
+    full = {"sdf": sdf_host, "coords": d["coords"], "sum_img": d["sum_img"],
+            "num_samples": d["num_samples"], "all_taus": tau_grid.numpy(),
+            "all_losses": losses.numpy()}
+    if out_dir is not None:
+        formats.save_optimization_file(
+            os.path.join(out_dir, f"opt_output_optimization_result{ext}.npz"), result)
+        formats.save_optimization_file(
+            os.path.join(out_dir, f"opt_output_full_data{ext}.npz"), full)
+    return result, full
+
+
+def main(argv=None):
+    """Command line of isosurface_optimization.py:379-395: the stage file and
+    the output folder from the dataset's folder scheme (--root, --id, --name,
+    --is_los, --ext; experiment 2 as there), or given directly (--sdf,
+    --out)."""
+    import argparse
+    from . import formats
+    ap = argparse.ArgumentParser(description="isosurface offset search on the GPU")
+    ap.add_argument("--ext", default="tmp")
+    ap.add_argument("--id", default="tmp")
+    ap.add_argument("--name", default="tmp")
+    ap.add_argument("--is_los", default="y")
+    ap.add_argument("--overwrite_existing", default="n")
+    ap.add_argument("--root", default=".", help="folder holding the dataset")
+    ap.add_argument("--sdf", default=None, help="stage file (instead of the folder scheme)")
+    ap.add_argument("--out", default=None, help="output folder (with --sdf)")
+    ap.add_argument("--rays", type=int, default=2048, help="rays per side per antenna")
+    a = ap.parse_args(argv)
+    if a.sdf is None:
+        proc = formats.dataset_path(a.root, a.id, a.name, "77_ghz", is_sim=False,
+                                    is_los=a.is_los in ("y", "Y"), exp_num="2",
+                                    is_processed=True)
+        sdf_path, out = f"{proc}/sdf/sdf{a.ext}.npz", f"{proc}/optimization"
+    else:
+        sdf_path, out = a.sdf, a.out
+    done = out and os.path.exists(os.path.join(out, f"opt_output_optimization_result{a.ext}.npz"))
+    if done and a.overwrite_existing not in ("y", "Y"):
+        print("The output file already exists and overwrite_existing is not set: skipping.")
+        return 0
+    result, _ = run_stage(sdf_path, out, ext=a.ext, width=a.rays, height=a.rays)
+    print(f"taus {np.asarray(result['taus']).tolist()}, {len(result['points'])} points")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
