@@ -20,6 +20,7 @@ and out.
 """
 from __future__ import annotations
 
+import os
 import sys
 
 import numpy as np
@@ -278,3 +279,64 @@ def _construct_rsdf_components(all_normals, coords_full, labeled, valid_labels, 
     delta_x = x_locs[1] - x_locs[0]
     seeds = component_seeds(all_normals, coords_full, labeled, valid_labels)
     return rsdf_components(all_normals, labeled, seeds, delta_x)
+
+
+def compute_rsdf_file(normal_path, sdf_path, volume_fn=hull_face_box_volume, params=None):
+    """The RSDF stage of compute_rsdf (sdf.py:255-300) on explicit paths:
+    read normal_data.npz, the binary image (its voxels' normals set to NaN),
+    unit normals, the components and their RSDF, and write sdf.npz with the
+    keys of _save_sdf_file (sdf.py:226-253; the 77 GHz 'almost-square'
+    chirp's wavelengths from ``params``, image_processor.PARAMS by default).
+    Returns the written dictionary."""
+    from . import formats, image_processor
+    sum_img, (xl, yl, zl), antenna_locs, channels, normals = formats.load_normal_data(
+        normal_path)
+    m1, m2, m3 = np.meshgrid(yl, xl, zl)
+    coords_full = np.concatenate((m2[..., None], m1[..., None], m3[..., None]), axis=-1)
+    filter_idx = _separate_binary_image(sum_img, coords_full, volume_fn)
+    all_normals = np.array(normals, copy=True)
+    all_normals[filter_idx] = np.nan
+    all_normals /= np.linalg.norm(all_normals, axis=-1)[..., np.newaxis].repeat(3, axis=-1)
+    labeled, valid_labels = _find_connected_components(all_normals, xl)
+    rsdf = _construct_rsdf_components(all_normals, coords_full, labeled, valid_labels, xl)
+    p = image_processor.get_radar_parameters("77_ghz", False, "almost-square", params)
+    data = formats.sdf_record(sum_img, xl, yl, zl, antenna_locs, channels, all_normals, rsdf,
+                              labeled, valid_labels, coords_full, p["min_f"], p["max_f"],
+                              p["num_samples"])
+    formats.save_sdf_file(sdf_path, data)
+    return data
+
+
+def compute_rsdf(obj_id, name, is_los, ext, overwrite_existing, root=".", **kw):
+    """compute_rsdf (sdf.py:255-300) over the dataset's folder scheme under
+    ``root`` (experiment 2, as there): normals/normal_data<ext>.npz in,
+    sdf/sdf<ext>.npz out; skipped when the output exists and
+    ``overwrite_existing`` is false."""
+    from . import formats
+    proc = formats.dataset_path(root, obj_id, name, "77_ghz", is_sim=False, is_los=is_los,
+                                exp_num="2", is_processed=True)
+    out = f"{proc}/sdf/sdf{ext}.npz"
+    if not overwrite_existing and os.path.exists(out):
+        print("The output file already exists and overwrite_existing is not set: skipping.")
+        return None
+    return compute_rsdf_file(f"{proc}/normals/normal_data{ext}.npz", out, **kw)
+
+
+def main(argv=None):
+    """The command line of sdf.py:290-310 (plus --root, the dataset folder)."""
+    import argparse
+    ap = argparse.ArgumentParser(description="RSDF of an object's surface normals on the GPU")
+    ap.add_argument("--name", default="hammer")
+    ap.add_argument("--id", default="048")
+    ap.add_argument("--ext", default="")
+    ap.add_argument("--is_los", default="y")
+    ap.add_argument("--overwrite_existing", default="n")
+    ap.add_argument("--root", default=".", help="folder holding the dataset")
+    a = ap.parse_args(argv)
+    yes = lambda v: v in ("y", "Y")
+    compute_rsdf(a.id, a.name, yes(a.is_los), a.ext, yes(a.overwrite_existing), root=a.root)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
