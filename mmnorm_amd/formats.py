@@ -147,6 +147,19 @@ def dataset_path(root, obj_id, name, radar_type, is_sim=False, is_los=True, exp_
     return f"{base}/{exp}/{los}/{proc}/radars/{radar_type}"
 
 
+def find_obj_angles(root, obj_id, name, exp_num="2", path_to_data="sample_data"):
+    """The object's rotation for a real-data experiment: the angle folder
+    (<x>_<y>_<z>) under robot_collected that holds exp<exp_num>
+    (generic_loader.py:69-87)."""
+    data_folder = f"{root}/{path_to_data}/{obj_id}_{name}/robot_collected"
+    for angle_folder in os.listdir(data_folder):
+        for exp_folder in os.listdir(f"{data_folder}/{angle_folder}"):
+            if exp_folder == f"exp{exp_num}":
+                return [int(a) for a in angle_folder.split("_")]
+    raise FileNotFoundError(f"Couldn't find any folder matching exp{exp_num} for object "
+                            f"{obj_id}_{name}")
+
+
 def radar_file_timestamp(filename):
     """The capture timestamp in a 77 GHz file name: characters 8-17, or 9-20
     when those are not an integer (generic_loader.py:522-525)."""

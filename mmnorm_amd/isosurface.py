@@ -407,9 +407,10 @@ def main(argv=None):
     ap.add_argument("--rays", type=int, default=2048, help="rays per side per antenna")
     a = ap.parse_args(argv)
     if a.sdf is None:
+        angles = formats.find_obj_angles(a.root, a.id, a.name, "2")
         proc = formats.dataset_path(a.root, a.id, a.name, "77_ghz", is_sim=False,
                                     is_los=a.is_los in ("y", "Y"), exp_num="2",
-                                    is_processed=True)
+                                    angles=angles, is_processed=True)
         sdf_path, out = f"{proc}/sdf/sdf{a.ext}.npz", f"{proc}/optimization"
     else:
         sdf_path, out = a.sdf, a.out

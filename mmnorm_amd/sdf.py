@@ -309,12 +309,14 @@ def compute_rsdf_file(normal_path, sdf_path, volume_fn=hull_face_box_volume, par
 
 def compute_rsdf(obj_id, name, is_los, ext, overwrite_existing, root=".", **kw):
     """compute_rsdf (sdf.py:255-300) over the dataset's folder scheme under
-    ``root`` (experiment 2, as there): normals/normal_data<ext>.npz in,
+    ``root`` (experiment 2 and the angle folder that holds it, as there):
+    normals/normal_data<ext>.npz in,
     sdf/sdf<ext>.npz out; skipped when the output exists and
     ``overwrite_existing`` is false."""
     from . import formats
+    angles = formats.find_obj_angles(root, obj_id, name, "2")
     proc = formats.dataset_path(root, obj_id, name, "77_ghz", is_sim=False, is_los=is_los,
-                                exp_num="2", is_processed=True)
+                                exp_num="2", angles=angles, is_processed=True)
     out = f"{proc}/sdf/sdf{ext}.npz"
     if not overwrite_existing and os.path.exists(out):
         print("The output file already exists and overwrite_existing is not set: skipping.")

@@ -114,3 +114,44 @@ def test_captures_to_normals(tmp_path):
     assert np.unravel_index(np.argmax(plane), plane.shape) == (10, 10)
     n = normals[10, 10, 5]
     assert not np.isnan(n[0]) and np.degrees(np.arccos(n[2])) < 5.0
+
+
+def test_normal_estimation_stage_from_the_folder_scheme(tmp_path):
+    """mmwave_normal_estimation.py:15-72 as a stage: the same captures laid out
+    as the dataset stores them (radar_data/ for the .bin files, the
+    experiment JSON in the exp2 folder, an angle folder found by its exp2),
+    normal_data.npz out with the reference's keys, equal to the caller run on
+    the same files; a second run is skipped."""
+    import os
+    import shutil
+    from mmnorm_amd import normal_estimation
+    root = str(tmp_path)
+    at = dict(is_los=True, exp_num="2", angles=(0, 90, 0))
+    raw = formats.dataset_path(root, "048", "hammer", "77_ghz", **at)
+    os.makedirs(raw)
+    params, _, _, _, _ = build_dataset(raw)
+    os.makedirs(f"{raw}/radar_data")
+    for f in os.listdir(raw):
+        if f.endswith(".bin"):
+            shutil.move(f"{raw}/{f}", f"{raw}/radar_data/{f}")
+    shutil.move(f"{raw}/experiment_info.json",
+                formats.dataset_path(root, "048", "hammer", "77_ghz", is_json=True, **at)
+                + "/experiment_info.json")
+    out = normal_estimation.estimate_normals("048", "hammer", True, "_s", False, root=root,
+                                             params=params, grid=GRID)
+    assert out == formats.dataset_path(root, "048", "hammer", "77_ghz", is_processed=True,
+                                       **at) + "/normals/normal_data_s.npz"
+    sum_img, axes, ant, ch, normals = formats.load_normal_data(out)
+    # the caller on the same files
+    proc = ip.ImageProcessor(False, params=params, grid=GRID)
+    exp = formats.load_exp_json(formats.dataset_path(root, "048", "hammer", "77_ghz",
+                                                     is_json=True, **at))
+    data = proc.correlate_locs_and_meas("77_ghz", formats.load_robot_files(raw),
+                                        formats.load_radar_files(f"{raw}/radar_data"), exp,
+                                        formats.load_robot_loc_files(raw), speed="speed_8")
+    want = proc.generate_mmwave_surface_normals("77_ghz", data, exp)
+    assert np.array_equal(sum_img, want[0]) and np.array_equal(normals, want[2], equal_nan=True)
+    assert all(np.array_equal(a, b) for a, b in zip(axes, want[1]))
+    assert np.array_equal(ant, want[3]) and np.array_equal(ch, want[4])
+    assert normal_estimation.estimate_normals("048", "hammer", True, "_s", False, root=root,
+                                              params=params, grid=GRID) is None
