@@ -311,3 +311,16 @@ def test_hull_face_box_degenerate_sets():
     flat = np.random.default_rng(1).uniform(size=(50, 3))
     flat[:, 2] = 0.5
     assert sdf.hull_face_box_volume(flat) == 0.0
+
+
+def test_find_obj_angles(tmp_path):
+    """generic_loader.py:69-87: the angle folder under robot_collected that
+    holds exp<n>; none: an error."""
+    from mmnorm_amd import formats
+    base = tmp_path / "sample_data" / "048_hammer" / "robot_collected"
+    (base / "0_0_0" / "exp1").mkdir(parents=True)
+    (base / "0_90_-45" / "exp2").mkdir(parents=True)
+    assert formats.find_obj_angles(str(tmp_path), "048", "hammer", "2") == [0, 90, -45]
+    assert formats.find_obj_angles(str(tmp_path), "048", "hammer", "1") == [0, 0, 0]
+    with pytest.raises(FileNotFoundError):
+        formats.find_obj_angles(str(tmp_path), "048", "hammer", "3")
