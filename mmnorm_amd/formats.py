@@ -215,6 +215,7 @@ _PICKLE_ALLOWED = {
 } | {
     (m, "_frombuffer") for m in ("numpy.core.numeric", "numpy._core.numeric")
 } | {("numpy", "ndarray"), ("numpy", "dtype"), ("collections", "OrderedDict"),
+     ("_codecs", "encode"),  # str -> bytes, how protocol 2 stores an array's data
      ("datetime", "datetime"), ("datetime", "date"), ("datetime", "timedelta")}
 
 

@@ -219,7 +219,7 @@ def test_robot_files_merge_and_refuse_callables(tmp_path):
         pickle.dump({"all_radar_filenames": ["3", "4"], "all_radar_created_ts": [3.0, 4.0],
                      "pattern": 2, "extra": np.arange(3)}, f)
     with open(tmp_path / "continuous_b2.pkl", "wb") as f:
-        pickle.dump({"extra": np.arange(3, 6.0)}, f, protocol=5)
+        pickle.dump({"extra": np.arange(3, 6.0)}, f, protocol=2)
     with open(tmp_path / "continuous_a.pkl", "wb") as f:
         pickle.dump({"all_radar_filenames": ["1"], "all_radar_created_ts": [1.0],
                      "pattern": 7}, f, protocol=5)
